@@ -1,0 +1,197 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings of the CPU restatement (``liboracle.so``, built from phs_oracle.cpp) and,
+where it was built, of the reference's own phs_cpu.cpp (``_ref/libphs_ref.so``).  Only
+tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may import this
+package, and only to check or to time the CPU baseline; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libphs_ref.so")
+REF_SRC = "/root/reference"
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_L = ctypes.c_long
+_D = ctypes.c_double
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so, and _ref/libphs_ref.so where the reference tree is present."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-B" if force else "-s", "liboracle.so"], cwd=HERE, check=True)
+    if os.path.isdir(REF_SRC) and (force or not os.path.exists(REF_LIB)):
+        subprocess.run(["make", "-B" if force else "-s", "ref"], cwd=HERE, check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.oracle_forward.restype = _I
+        _lib.oracle_forward_dual.restype = _I
+        _lib.oracle_constants.restype = _I
+        _lib.oracle_mass_sums.restype = _I
+        _lib.oracle_map_msq.restype = None
+        _lib.oracle_map_ct.restype = None
+        _lib.oracle_map_msq.argtypes = [_I, _D, _D, _D, _D, _D, _D, _P, _P, _P]
+        _lib.oracle_map_ct.argtypes = [_I, _D, _P, _P, _P, _P]
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(REF_LIB)
+        _ref.ref_forward.restype = _I
+        _ref.ref_time_forward.restype = _D
+    return _ref
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _tree_args(proc):
+    d1, d2, hc, mid, mm, mw, flv = proc.ctypes_args()
+    return [d1, d2, hc, mid, mm, mw, flv]
+
+
+def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
+    """Oracle forward pass: returns dict(p, factor, volume, ok[, msq])."""
+    x = np.ascontiguousarray(x, np.float64)
+    ch = np.ascontiguousarray(channels, np.int32)
+    n = ch.shape[0]
+    assert x.shape == (n, proc.n_x)
+    p = np.zeros((n, proc.n_out, 4))
+    f = np.zeros(n)
+    v = np.zeros(n)
+    ok = np.zeros(n, np.uint8)
+    msq = np.zeros((n, proc.n_prt_out)) if want_msq else None
+    t = _tree_args(proc)
+    e1 = None if E1 is None else np.ascontiguousarray(E1, np.float64)
+    e2 = None if E2 is None else np.ascontiguousarray(E2, np.float64)
+    rc = lib().oracle_forward(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                              *[_ptr(a) for a in t], _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n),
+                              _ptr(ch), _I(proc.n_x), _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok),
+                              _ptr(msq))
+    if rc:
+        raise ValueError(f"oracle_forward failed ({rc})")
+    out = dict(p=p, factor=f, volume=v, ok=ok.astype(bool))
+    if want_msq:
+        out["msq"] = msq
+    return out
+
+
+def forward_dual(proc, x, channels, x_tan=None, m_tan=None, w_tan=None, K=1):
+    """Dual-number forward: tangents of p and factor along K seeded directions."""
+    x = np.ascontiguousarray(x, np.float64)
+    ch = np.ascontiguousarray(channels, np.int32)
+    n = ch.shape[0]
+    p = np.zeros((n, proc.n_out, 4))
+    f = np.zeros(n)
+    ok = np.zeros(n, np.uint8)
+    pt = np.zeros((n, proc.n_out, 4, K))
+    ft = np.zeros((n, K))
+    cvt = (lambda a: None if a is None else np.ascontiguousarray(a, np.float64))
+    xt, mt, wt = cvt(x_tan), cvt(m_tan), cvt(w_tan)
+    t = _tree_args(proc)
+    rc = lib().oracle_forward_dual(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                                   *[_ptr(a) for a in t], _D(proc.sqrts), _L(n), _ptr(ch),
+                                   _I(proc.n_x), _ptr(x), _I(K), _ptr(xt), _ptr(mt), _ptr(wt),
+                                   _ptr(p), _ptr(f), _ptr(ok), _ptr(pt), _ptr(ft))
+    if rc:
+        raise ValueError(f"oracle_forward_dual failed ({rc})")
+    return dict(p=p, factor=f, ok=ok.astype(bool), p_tan=pt, factor_tan=ft)
+
+
+def constants(proc, sqrts=None):
+    n = proc.n_channels * proc.n_prt_out * 3
+    a = np.zeros(n)
+    b = np.zeros(n)
+    t = _tree_args(proc)
+    rc = lib().oracle_constants(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                                *[_ptr(q) for q in t], _D(proc.sqrts if sqrts is None else sqrts),
+                                _ptr(a), _ptr(b))
+    if rc:
+        raise ValueError("oracle_constants failed")
+    shape = (proc.n_channels, proc.n_prt_out, 3)
+    return a.reshape(shape), b.reshape(shape)
+
+
+def mass_sums(proc):
+    out = np.zeros(proc.n_channels * proc.n_prt_out)
+    d1, d2, hc, mid, mm, mw, flv = proc.ctypes_args()
+    rc = lib().oracle_mass_sums(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels), _ptr(d1),
+                                _ptr(d2), _ptr(hc), _ptr(mid), _ptr(flv), _ptr(out))
+    if rc:
+        raise ValueError("oracle_mass_sums failed")
+    return out.reshape(proc.n_channels, proc.n_prt_out)
+
+
+def map_msq(map_id, x, s, m, w, msq_min, msq_max, a):
+    a = np.ascontiguousarray(a, np.float64)
+    msq = ctypes.c_double()
+    f = ctypes.c_double()
+    lib().oracle_map_msq(map_id, x, s, m, w, msq_min, msq_max, _ptr(a), ctypes.byref(msq),
+                         ctypes.byref(f))
+    return msq.value, f.value
+
+
+def map_ct(map_id, x, b):
+    b = np.ascontiguousarray(b, np.float64)
+    ct, st, f = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().oracle_map_ct(map_id, x, _ptr(b), ctypes.byref(ct), ctypes.byref(st), ctypes.byref(f))
+    return ct.value, st.value, f.value
+
+
+def ref_forward(proc, x, channels, want_msq=False):
+    """The compiled reference phs_cpu.cpp on the same inputs (fixed sqrt(s))."""
+    x = np.array(x, np.float64, order="C")  # the reference takes a non-const pointer
+    ch = np.ascontiguousarray(channels, np.int32)
+    n = ch.shape[0]
+    p = np.zeros((n, proc.n_out, 4))
+    f = np.zeros(n)
+    v = np.zeros(n)
+    ok = np.zeros(n, np.uint8)
+    msq = np.zeros((n, proc.n_prt_out)) if want_msq else None
+    t = _tree_args(proc)
+    rc = ref().ref_forward(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                           *[_ptr(a) for a in t], _D(proc.sqrts), _L(n), _ptr(ch), _I(proc.n_x),
+                           _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok), _ptr(msq))
+    if rc:
+        raise ValueError(f"ref_forward failed ({rc})")
+    out = dict(p=p, factor=f, volume=v, ok=ok.astype(bool))
+    if want_msq:
+        out["msq"] = msq
+    return out
+
+
+def ref_time(proc, x, channels):
+    """Seconds the reference's single-thread generation loop takes, and its ok count."""
+    x = np.array(x, np.float64, order="C")
+    ch = np.ascontiguousarray(channels, np.int32)
+    n_ok = ctypes.c_long()
+    t = _tree_args(proc)
+    sec = ref().ref_time_forward(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                                 *[_ptr(a) for a in t], _D(proc.sqrts), _L(ch.shape[0]), _ptr(ch),
+                                 _I(proc.n_x), _ptr(x), ctypes.byref(n_ok))
+    return sec, n_ok.value

@@ -1,0 +1,529 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of the reference's serial phase-space algorithm (phs_cpu.cpp,
+// Whizard's recursion).  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load this library, and only as the checker: the product
+// path (gpu_phs_amd, libgpu_phs.so) never calls it.
+//
+// Every function cites the reference file:line it restates.  The scalar type is a
+// template parameter so that the same code runs in double (the oracle proper) and
+// in forward-mode dual numbers (the derivative reference for the adjoint kernel).
+//
+// Pinning (what checks this restatement, see tests/test_oracle.py):
+//   * bitwise against the compiled reference phs_cpu.cpp (oracle/_ref) on seeded
+//     randoms for every map id and for the ttH / bwbw trees;
+//   * Whizard's own momenta in samples/ttH.ref and samples/bwbw.ref (1e-6, the
+//     randoms there carry 9 decimals);
+//   * closed forms: 2->2 |p| = sqrt(lambda(s,m^2,m^2))/(2 sqrt s), volume
+//     s^(n-2)/(4^(n-1) (2 pi)^(2n+1)), map end points, f_map = (d msq/dx)/s and
+//     f_ct = (d ct/dx)/2 by finite differences, momentum conservation, on-shell
+//     masses;
+//   * the dual-number path against central finite differences of the double path.
+//   Parts with no reference counterpart and therefore pinned by invariants only:
+//   the per-event sqrt(s) (reduces to the fixed-sqrt(s) path when E1 = E2), the
+//   lab-frame boost (phs.cu:1286-1301; identity for E1 = E2, Lorentz invariants and
+//   total momentum otherwise).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+// Constants as the reference defines them (mappings.h:4-7).
+const double PI_ = 3.14159265358979323846;
+const double TWOPI_ = 6.28318530717958647693;
+const double TWOPI2_ = 39.47841760435743447534;
+const double TWOPI5_ = 9792.62991312900650440772;
+
+// Map ids (phs.h:8-17).
+enum { MAP_NO = 0, MAP_SCHANNEL = 1, MAP_TCHANNEL = 2, MAP_UCHANNEL = 3, MAP_RADIATION = 4,
+       MAP_COLLINEAR = 5, MAP_INFRARED = 6, MAP_STEP_E = 11, MAP_STEP_H = 12, MAP_ONSHELL = 99 };
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers (value + KD tangents).  Only what the algorithm uses.
+constexpr int KD = 24;
+struct Dual {
+  double v;
+  double d[KD];
+  Dual() : v(0) { std::memset(d, 0, sizeof d); }
+  Dual(double x) : v(x) { std::memset(d, 0, sizeof d); }
+};
+inline Dual operator+(const Dual& a, const Dual& b) { Dual r(a.v + b.v); for (int i = 0; i < KD; i++) r.d[i] = a.d[i] + b.d[i]; return r; }
+inline Dual operator-(const Dual& a, const Dual& b) { Dual r(a.v - b.v); for (int i = 0; i < KD; i++) r.d[i] = a.d[i] - b.d[i]; return r; }
+inline Dual operator-(const Dual& a) { Dual r(-a.v); for (int i = 0; i < KD; i++) r.d[i] = -a.d[i]; return r; }
+inline Dual operator*(const Dual& a, const Dual& b) { Dual r(a.v * b.v); for (int i = 0; i < KD; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+inline Dual operator/(const Dual& a, const Dual& b) {
+  Dual r(a.v / b.v);
+  for (int i = 0; i < KD; i++) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v);
+  return r;
+}
+inline Dual operator+(const Dual& a, double b) { return a + Dual(b); }
+inline Dual operator+(double a, const Dual& b) { return Dual(a) + b; }
+inline Dual operator-(const Dual& a, double b) { return a - Dual(b); }
+inline Dual operator-(double a, const Dual& b) { return Dual(a) - b; }
+inline Dual operator*(const Dual& a, double b) { Dual r(a.v * b); for (int i = 0; i < KD; i++) r.d[i] = a.d[i] * b; return r; }
+inline Dual operator*(double a, const Dual& b) { return b * a; }
+inline Dual operator/(const Dual& a, double b) { Dual r(a.v / b); for (int i = 0; i < KD; i++) r.d[i] = a.d[i] / b; return r; }
+inline Dual operator/(double a, const Dual& b) { return Dual(a) / b; }
+inline Dual& operator*=(Dual& a, const Dual& b) { a = a * b; return a; }
+inline Dual chain(const Dual& a, double f, double df) { Dual r(f); for (int i = 0; i < KD; i++) r.d[i] = df * a.d[i]; return r; }
+inline Dual sqrt(const Dual& a) { double s = std::sqrt(a.v); return chain(a, s, 0.5 / s); }
+inline Dual exp(const Dual& a) { double e = std::exp(a.v); return chain(a, e, e); }
+inline Dual log(const Dual& a) { return chain(a, std::log(a.v), 1.0 / a.v); }
+inline Dual sin(const Dual& a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
+inline Dual cos(const Dual& a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
+inline Dual tan(const Dual& a) { double t = std::tan(a.v); return chain(a, t, 1.0 + t * t); }
+inline Dual atan(const Dual& a) { return chain(a, std::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
+inline double val(const Dual& a) { return a.v; }
+inline double val(double a) { return a; }
+
+using std::sqrt; using std::exp; using std::log; using std::sin; using std::cos; using std::tan; using std::atan;
+
+template <class T> T tmax(const T& a, double b) { return val(a) < b ? T(b) : a; }  // std::max(a, b) keeps a on ties
+inline double pow2(double q) { return std::pow(q, 2); }  // the reference's pow(q, 2)
+inline Dual pow2(const Dual& q) { return q * q; }
+
+// ---------------------------------------------------------------------------
+// Process description in the reference's index form: branch index = bincode - 1.
+struct Proc {
+  int n_in = 2, n_out = 0, n_prt = 0, n_prt_out = 0, root = 0, n_channels = 0;
+  std::vector<int> d1, d2, hc;      // [c * n_prt + i]; daughters as indices, -1 = none
+  std::vector<int> map_id;          // [c * n_prt_out + i]
+  std::vector<double> mass_sum;     // [c * n_prt_out + i]
+  std::vector<double> flv;          // [n_in + n_out]
+};
+
+template <class T> struct MapPar {
+  std::vector<T> m, w, a, b;        // a, b: [c * n_prt_out * 3 + 3 * i + k]
+};
+
+// phs.cu:61-78 (set_mass_sum): minimum invariant mass of every branch = sum of leaf masses.
+// The leaf mass index ld2(bincode) + 2 assumes two incoming particles, as the reference does.
+void set_mass_sum(const Proc& P, int c, double* ms, int b) {
+  if (P.hc[c * P.n_prt + b]) {
+    int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
+    set_mass_sum(P, c, ms, k1);
+    set_mass_sum(P, c, ms, k2);
+    ms[b] = ms[k1] + ms[k2];
+  } else {
+    int ld2 = 0, bb = b + 1;
+    while (bb > 1) { bb /= 2; ld2++; }
+    ms[b] = P.flv[ld2 + 2];
+  }
+}
+
+// phs_cpu.cpp:452-531 (init_mapping_constants_cpu): the a[3] (msq map) and b[3] (cos-theta
+// map) constants of every (channel, branch), evaluated at one sqrt(s).
+template <class T>
+void init_constants(const Proc& P, MapPar<T>& M, int c, double sqrts) {
+  double m_tot = P.mass_sum[c * P.n_prt_out + P.root];
+  for (int i = 0; i < P.n_prt_out; i++) {
+    int id = P.map_id[c * P.n_prt_out + i];
+    T* a = &M.a[(c * P.n_prt_out + i) * 3];
+    T* b = &M.b[(c * P.n_prt_out + i) * 3];
+    T m = M.m[c * P.n_prt_out + i], w = M.w[c * P.n_prt_out + i];
+    double m_min = P.mass_sum[c * P.n_prt_out + i];
+    double m_max = sqrts - m_tot + m_min;
+    double msq_min = m_min * m_min, msq_max = m_max * m_max, s = sqrts * sqrts;
+    T msq0;
+    switch (id) {
+      case MAP_NO:
+        a[0] = 0; a[1] = msq_max - msq_min; a[2] = a[1] / s; break;
+      case MAP_SCHANNEL:
+        msq0 = m * m;
+        a[0] = atan((msq_min - msq0) / (m * w));
+        a[1] = atan((msq_max - msq0) / (m * w));
+        a[2] = (a[1] - a[0]) * (m * w) / s; break;
+      case MAP_RADIATION: case MAP_COLLINEAR: case MAP_INFRARED:
+        msq0 = (id == MAP_RADIATION) ? w * w : m * m;
+        a[0] = msq0;
+        a[1] = log((msq_max - msq_min) / msq0 + 1);
+        a[2] = a[1] / s; break;
+      case MAP_TCHANNEL: case MAP_UCHANNEL:
+        msq0 = m * m;
+        a[0] = msq0;
+        a[1] = 2 * log((msq_max - msq_min) / (2 * msq0) + 1);
+        a[2] = a[1] / s; break;
+      case MAP_STEP_E:
+        a[2] = tmax(2 * m * w / (msq_max - msq_min), 0.01);
+        a[1] = exp(-(m * m - msq_min) / (msq_max - msq_min) / a[2]);
+        a[0] = 1 - a[2] * log((1 + a[1] * exp(1 / a[2])) / (1 + a[1])); break;
+      case MAP_STEP_H: {
+        a[2] = (m * m - msq_min) / (msq_max - msq_min);
+        a[1] = tmax(pow2(2 * m * w / (msq_max - msq_min)) / a[2], 0.000001);
+        a[0] = (1 + sqrt(1 + 4 * a[1] / (1 - a[2]))) / 2; break;
+      }
+      default: break;
+    }
+    switch (id) {
+      case MAP_TCHANNEL: case MAP_UCHANNEL: case MAP_COLLINEAR:
+        b[0] = m * m / s; b[1] = log((b[0] + 1) / b[0]); b[2] = 0; break;
+      default:
+        b[0] = 0; b[1] = 0; b[2] = 0; break;
+    }
+  }
+}
+
+// phs_cpu.cpp:12-61 (mapping_msq_from_x_cpu).
+template <class T>
+void map_msq(int type, const T& x, double s, const T& m, const T& w, double msq_min, double msq_max,
+             const T* a, T* msq, T* factor) {
+  T msq1, tmp, z;
+  switch (type) {
+    case MAP_NO:
+      *msq = (1 - x) * msq_min + x * msq_max;
+      *factor = T((msq_max - msq_min) / s);
+      break;
+    case MAP_SCHANNEL:
+      z = (1 - x) * a[0] + x * a[1];
+      if (-PI_ / 2 < val(z) && val(z) < PI_ / 2) {
+        tmp = tan(z);
+        *msq = m * (m + w * tmp);
+        *factor = a[2] * (1 + tmp * tmp);
+      } else {
+        *msq = 0; *factor = 0;
+      }
+      break;
+    case MAP_COLLINEAR: case MAP_INFRARED: case MAP_RADIATION:
+      msq1 = a[0] * exp(x * a[1]);
+      *msq = msq1 - a[0] + msq_min;
+      *factor = a[2] * msq1;
+      break;
+    case MAP_TCHANNEL: case MAP_UCHANNEL:
+      if (val(x) < 0.5) {
+        msq1 = a[0] * exp(x * a[1]);
+        *msq = msq1 - a[0] + msq_min;
+      } else {
+        msq1 = a[0] * exp((1 - x) * a[1]);
+        *msq = -(msq1 - a[0]) + msq_max;
+      }
+      *factor = a[2] * msq1;
+      break;
+    case MAP_STEP_H:
+      z = a[1] / (a[0] - x) - a[1] / a[0] + a[2] * x;
+      *msq = z * msq_max + (1 - z) * msq_min;
+      *factor = (a[1] / ((a[0] - x) * (a[0] - x)) + a[2]) * (msq_max - msq_min) / s;
+      break;
+    case MAP_STEP_E:
+      tmp = exp(-x * a[0] / a[2]) * (1 + a[1]);
+      z = -a[2] * log(tmp - a[1]);
+      *msq = z * msq_max + (1 - z) * msq_min;
+      *factor = a[0] / (1 - a[1] / tmp) * (msq_max - msq_min) / s;
+      break;
+  }
+}
+
+// phs_cpu.cpp:112-146 (mapping_ct_from_x_cpu).
+template <class T>
+void map_ct(int type, const T& x, const T* b, T* ct, T* st, T* factor) {
+  T tmp;
+  switch (type) {
+    case MAP_NO: case MAP_SCHANNEL: case MAP_INFRARED: case MAP_RADIATION: case MAP_STEP_E: case MAP_STEP_H:
+      tmp = 2 * (1 - x);
+      *ct = 1 - tmp;
+      *st = sqrt(tmp * (2 - tmp));
+      *factor = 1;
+      break;
+    case MAP_COLLINEAR: case MAP_TCHANNEL: case MAP_UCHANNEL:
+      if (val(x) < 0.5) {
+        tmp = b[0] * exp(2 * x * b[1]);
+        *ct = tmp - b[0] - 1;
+      } else {
+        tmp = b[0] * exp(2 * (1 - x) * b[1]);
+        *ct = -(tmp - b[0]) + 1;
+      }
+      if (val(*ct) >= -1 && val(*ct) <= 1) {
+        *st = sqrt(1 - *ct * *ct);
+        *factor = tmp * b[1];
+      } else {
+        *ct = 1; *st = 0; *factor = 0;
+      }
+      break;
+  }
+}
+
+template <class T> struct Ev {
+  std::vector<T> p2;        // msq per branch index (prt[].p2 of the reference)
+  std::vector<T> pd;        // p_decay per branch index
+  std::vector<T> mom;       // 4 per branch index (prt[].p)
+};
+
+// phs_cpu.cpp:174-234 (set_msq_cpu): post-order recursion, msq randoms consumed in
+// post-order (k1 subtree, k2 subtree, self), root excluded; early return on !ok.
+template <class T>
+void set_msq(const Proc& P, const MapPar<T>& M, int c, int b, double m_tot, const T* x, int* id_x,
+             double sqrts, Ev<T>& E, T* factor, T* volume, bool* ok) {
+  int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
+  T f1, f2, v1, v2;
+  if (P.hc[c * P.n_prt + k1]) {
+    set_msq(P, M, c, k1, m_tot, x, id_x, sqrts, E, &f1, &v1, ok);
+    if (!*ok) return;
+  } else { f1 = 1; v1 = 1; }
+  if (P.hc[c * P.n_prt + k2]) {
+    set_msq(P, M, c, k2, m_tot, x, id_x, sqrts, E, &f2, &v2, ok);
+    if (!*ok) return;
+  } else { f2 = 1; v2 = 1; }
+
+  double m_max;
+  if (b == P.root) {
+    E.p2[b] = sqrts * sqrts;
+    m_max = sqrts;
+    *factor = f1 * f2;
+    *volume = v1 * v2 / (4 * TWOPI5_);
+  } else {
+    int ib = c * P.n_prt_out + b;
+    double m_min = P.mass_sum[ib];
+    m_max = sqrts - m_tot + m_min;
+    map_msq(P.map_id[ib], x[*id_x], sqrts * sqrts, M.m[ib], M.w[ib], m_min * m_min, m_max * m_max,
+            &M.a[ib * 3], &E.p2[b], factor);
+    (*id_x)++;
+    // The reference tests a local that is always 0 here (phs_cpu.cpp:202,210), so this
+    // branch is always taken.
+    *factor *= f1 * f2;
+    *volume = v1 * v2 * sqrts * sqrts / (4 * TWOPI2_);
+  }
+  if (*ok) {
+    T this_msq = E.p2[b], msq1 = E.p2[k1], msq2 = E.p2[k2];
+    T m1 = sqrt(msq1), m2 = sqrt(msq2), m = sqrt(this_msq);
+    T lda = (this_msq - msq1 - msq2) * (this_msq - msq1 - msq2) - 4 * msq1 * msq2;
+    if (val(lda) > 0 && val(m) > val(m1 + m2) && val(m) <= m_max) {
+      E.pd[k1] = sqrt(lda) / (2 * m);
+      E.pd[k2] = -E.pd[k1];
+      *factor *= sqrt(lda) / this_msq;
+    } else {
+      *ok = false;
+    }
+  }
+}
+
+// phs_cpu.cpp:283-345 (set_angles_cpu): pre-order recursion; phi then cos-theta randoms
+// per internal branch; L_new = L0 * B_z(bg) R_z(phi) R_y(theta).
+template <class T>
+void set_angles(const Proc& P, const MapPar<T>& M, int c, int b, const T* x, int* idx_x, T* factor,
+                Ev<T>& E, const T L0[4][4]) {
+  T p = E.pd[b];
+  T m = sqrt(E.p2[b]);
+  T En = sqrt(E.p2[b] + p * p);
+  for (int i = 0; i < 4; i++) E.mom[4 * b + i] = L0[i][0] * En + L0[i][3] * p;
+  if (P.hc[c * P.n_prt + b]) {
+    int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
+    T bg = val(m) > 0 ? p / m : T(0);
+    T gamma = sqrt(1 + bg * bg);
+    T phi = x[*idx_x] * TWOPI_;
+    (*idx_x)++;
+    T cp = cos(phi), sp = sin(phi);
+    T xx = x[*idx_x];
+    (*idx_x)++;
+    int ib = c * P.n_prt_out + b;
+    T ct, st, f;
+    map_ct(P.map_id[ib], xx, &M.b[ib * 3], &ct, &st, &f);
+    *factor *= f;
+    T L1[4][4];
+    L1[0][0] = gamma; L1[0][1] = -bg * st;    L1[0][2] = 0;  L1[0][3] = bg * ct;
+    L1[1][0] = 0;     L1[1][1] = ct * cp;     L1[1][2] = -sp; L1[1][3] = st * cp;
+    L1[2][0] = 0;     L1[2][1] = ct * sp;     L1[2][2] = cp; L1[2][3] = st * sp;
+    L1[3][0] = bg;    L1[3][1] = -gamma * st; L1[3][2] = 0;  L1[3][3] = gamma * ct;
+    T Ln[4][4];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        Ln[i][j] = 0;
+        for (int k = 0; k < 4; k++) Ln[i][j] = Ln[i][j] + L0[i][k] * L1[k][j];
+      }
+    set_angles(P, M, c, k1, x, idx_x, factor, E, Ln);
+    set_angles(P, M, c, k2, x, idx_x, factor, E, Ln);
+  }
+}
+
+// Import the caller's tree in the reference's bincode form (whizard_gpu_phs.cpp:79-87,
+// daughters as bincodes, 0 = none) and convert to indices as init_phs_gpu does
+// (phs.cu:392-397).
+bool build_proc(Proc& P, int n_in, int n_out, int n_channels, const int* d1, const int* d2,
+                const int* hc, const int* map_ids, const double* flv) {
+  if (n_in != 2 || n_out < 2 || n_out > 12 || n_channels < 1) return false;
+  P.n_in = n_in; P.n_out = n_out; P.n_channels = n_channels;
+  P.n_prt = (1 << (n_in + n_out)) - 1;
+  P.n_prt_out = (1 << n_out) - 1;
+  P.root = P.n_prt_out - 1;
+  P.d1.assign(d1, d1 + (size_t)n_channels * P.n_prt);
+  P.d2.assign(d2, d2 + (size_t)n_channels * P.n_prt);
+  for (auto& v : P.d1) v -= 1;
+  for (auto& v : P.d2) v -= 1;
+  P.hc.assign(hc, hc + (size_t)n_channels * P.n_prt);
+  P.map_id.assign(map_ids, map_ids + (size_t)n_channels * P.n_prt_out);
+  P.flv.assign(flv, flv + n_in + n_out);
+  P.mass_sum.assign((size_t)n_channels * P.n_prt_out, 0.0);
+  for (int c = 0; c < n_channels; c++) set_mass_sum(P, c, &P.mass_sum[(size_t)c * P.n_prt_out], P.root);
+  return true;
+}
+
+// One event, phs_cpu.cpp:582-600 (reset prt, init_msq_cpu, set_msq_cpu, set_angles_cpu),
+// then the lab-frame boost of the final leaves, phs.cu:1286-1301, when E1 != E2.
+template <class T>
+bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg_lab, const T* x,
+               Ev<T>& E, T* factor, T* volume, T* p_out, double* msq_out) {
+  int n = P.n_prt;
+  E.p2.assign(n, T(0)); E.pd.assign(n, T(0)); E.mom.assign(4 * n, T(0));
+  // phs_cpu.cpp:541-547 (init_msq_cpu): leaf msq = flavour mass squared.
+  for (int i = 0, p = 1; i < P.n_out; i++, p *= 2) E.p2[p - 1] = P.flv[P.n_in + i] * P.flv[P.n_in + i];
+  bool ok = true;
+  int id_x = 0;
+  double m_tot = P.mass_sum[c * P.n_prt_out + P.root];
+  set_msq(P, M, c, P.root, m_tot, x, &id_x, sqrts, E, factor, volume, &ok);
+  if (msq_out)
+    for (int i = 0; i < P.n_prt_out; i++) msq_out[i] = val(E.p2[i]);
+  if (ok) {
+    T L0[4][4];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) L0[i][j] = (i == j) ? 1.0 : 0.0;
+    set_angles(P, M, c, P.root, x, &id_x, factor, E, L0);
+    for (int j = 0, b = 1; j < P.n_out; j++, b *= 2)
+      for (int mu = 0; mu < 4; mu++) p_out[4 * j + mu] = E.mom[4 * (b - 1) + mu];
+    if (bg_lab != 0) {
+      double g = std::sqrt(1 + bg_lab * bg_lab);
+      for (int j = 0; j < P.n_out; j++) {
+        T p0 = g * p_out[4 * j] + bg_lab * p_out[4 * j + 3];
+        T p3 = bg_lab * p_out[4 * j] + g * p_out[4 * j + 3];
+        p_out[4 * j] = p0; p_out[4 * j + 3] = p3;
+      }
+    }
+  }
+  return ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Sizes of the reference's derived quantities (main.cpp:271-285).
+int oracle_n_prt(int n_in, int n_out) { return (1 << (n_in + n_out)) - 1; }
+int oracle_n_prt_out(int n_out) { return (1 << n_out) - 1; }
+
+// Mass sums per (channel, branch index), phs.cu:61-78.  Returns 0 on success.
+int oracle_mass_sums(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                     const int* map_ids, const double* flv, double* out) {
+  Proc P;
+  if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  std::memcpy(out, P.mass_sum.data(), P.mass_sum.size() * sizeof(double));
+  return 0;
+}
+
+// Mapping constants a[3], b[3] per (channel, branch index) at one sqrt(s),
+// phs_cpu.cpp:452-531.  a_out, b_out: [n_channels][n_prt_out][3].
+int oracle_constants(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                     const int* map_ids, const double* map_m, const double* map_w, const double* flv,
+                     double sqrts, double* a_out, double* b_out) {
+  Proc P;
+  if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  MapPar<double> M;
+  size_t nb = (size_t)n_channels * P.n_prt_out;
+  M.m.assign(map_m, map_m + nb); M.w.assign(map_w, map_w + nb);
+  M.a.assign(3 * nb, 0.0); M.b.assign(3 * nb, 0.0);
+  for (int c = 0; c < n_channels; c++) init_constants(P, M, c, sqrts);
+  std::memcpy(a_out, M.a.data(), 3 * nb * sizeof(double));
+  std::memcpy(b_out, M.b.data(), 3 * nb * sizeof(double));
+  return 0;
+}
+
+// Single-map probes for closed-form tests: phs_cpu.cpp:12-61 and 112-146.
+void oracle_map_msq(int type, double x, double s, double m, double w, double msq_min, double msq_max,
+                    const double* a, double* msq, double* factor) {
+  map_msq<double>(type, x, s, m, w, msq_min, msq_max, a, msq, factor);
+}
+void oracle_map_ct(int type, double x, const double* b, double* ct, double* st, double* factor) {
+  map_ct<double>(type, x, b, ct, st, factor);
+}
+
+// Forward generation of n_events.
+//   trees: d1, d2, hc [n_channels][n_prt] (bincodes, 0 = none), map_ids / map_m / map_w
+//   [n_channels][n_prt_out] indexed by bincode - 1, flv [n_in + n_out] (incoming first).
+//   E1, E2: per-event beam energies, or NULL for both beams at sqrts / 2.
+//   channels [n_events] 0-based; x [n_events][n_x] event-major.
+//   Outputs: p [n_events][n_out][4], factors, volumes [n_events], oks [n_events] (0/1),
+//   msq [n_events][n_prt_out] (may be NULL).  Non-ok events keep p = 0 and whatever factor
+//   and volume the early return left, as the reference does.  Returns 0 on success.
+int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                   const int* map_ids, const double* map_m, const double* map_w, const double* flv,
+                   double sqrts, const double* E1, const double* E2, long n_events, const int* channels,
+                   int n_x, const double* x, double* p, double* factors, double* volumes,
+                   unsigned char* oks, double* msq) {
+  Proc P;
+  if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  if (n_x < 3 * n_out - 4) return 2;
+  MapPar<double> M;
+  size_t nb = (size_t)n_channels * P.n_prt_out;
+  M.m.assign(map_m, map_m + nb); M.w.assign(map_w, map_w + nb);
+  M.a.assign(3 * nb, 0.0); M.b.assign(3 * nb, 0.0);
+  double cur_sqrts = sqrts;
+  for (int c = 0; c < n_channels; c++) init_constants(P, M, c, cur_sqrts);
+  Ev<double> E;
+  for (long i = 0; i < n_events; i++) {
+    int c = channels[i];
+    if (c < 0 || c >= n_channels) return 3;
+    double s_ev = sqrts, bg_lab = 0;
+    if (E1 && E2) {
+      // phs.cu:1313-1317 (per-event sqrt(s)) and 1289 (lab boost parameter).
+      s_ev = 2 * std::sqrt(E1[i] * E2[i]);
+      bg_lab = (E1[i] - E2[i]) / (2 * std::sqrt(E1[i] * E2[i]));
+      if (s_ev != cur_sqrts) {
+        cur_sqrts = s_ev;
+        for (int cc = 0; cc < n_channels; cc++) init_constants(P, M, cc, cur_sqrts);
+      }
+    }
+    double f = 0, v = 0;
+    std::memset(&p[4 * n_out * i], 0, 4 * n_out * sizeof(double));
+    bool ok = one_event<double>(P, M, c, s_ev, bg_lab, &x[(size_t)n_x * i], E, &f, &v, &p[4 * n_out * i],
+                                msq ? &msq[(size_t)P.n_prt_out * i] : nullptr);
+    factors[i] = f; volumes[i] = v; oks[i] = ok ? 1 : 0;
+  }
+  return 0;
+}
+
+// Forward-mode derivatives (dual numbers) of the same generation, K <= 24 tangent
+// directions: x_tan [n_events][n_x][K], m_tan / w_tan [n_channels][n_prt_out][K] seed the
+// directions of the randoms and of the map masses / widths.  Outputs p_tan
+// [n_events][n_out][4][K] and factor_tan [n_events][K] (zero for non-ok events), plus the
+// primal values as oracle_forward (fixed sqrts only).
+int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                        const int* map_ids, const double* map_m, const double* map_w, const double* flv,
+                        double sqrts, long n_events, const int* channels, int n_x, const double* x,
+                        int K, const double* x_tan, const double* m_tan, const double* w_tan,
+                        double* p, double* factors, unsigned char* oks, double* p_tan, double* factor_tan) {
+  if (K < 1 || K > KD) return 4;
+  Proc P;
+  if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  MapPar<Dual> M;
+  size_t nb = (size_t)n_channels * P.n_prt_out;
+  M.m.resize(nb); M.w.resize(nb); M.a.resize(3 * nb); M.b.resize(3 * nb);
+  for (size_t i = 0; i < nb; i++) {
+    M.m[i] = Dual(map_m[i]); M.w[i] = Dual(map_w[i]);
+    for (int k = 0; k < K; k++) {
+      M.m[i].d[k] = m_tan ? m_tan[i * K + k] : 0.0;
+      M.w[i].d[k] = w_tan ? w_tan[i * K + k] : 0.0;
+    }
+  }
+  for (int c = 0; c < n_channels; c++) init_constants(P, M, c, sqrts);
+  Ev<Dual> E;
+  std::vector<Dual> xd(n_x), pd(4 * n_out);
+  for (long i = 0; i < n_events; i++) {
+    int c = channels[i];
+    if (c < 0 || c >= n_channels) return 3;
+    for (int k = 0; k < n_x; k++) {
+      xd[k] = Dual(x[(size_t)n_x * i + k]);
+      for (int t = 0; t < K; t++) xd[k].d[t] = x_tan ? x_tan[((size_t)n_x * i + k) * K + t] : 0.0;
+    }
+    for (auto& q : pd) q = Dual(0.0);
+    Dual f, v;
+    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, xd.data(), E, &f, &v, pd.data(), nullptr);
+    oks[i] = ok ? 1 : 0;
+    factors[i] = ok ? f.v : 0;
+    for (int j = 0; j < 4 * n_out; j++) {
+      p[4 * n_out * i + j] = ok ? pd[j].v : 0;
+      for (int t = 0; t < K; t++) p_tan[((size_t)4 * n_out * i + j) * K + t] = ok ? pd[j].d[t] : 0;
+    }
+    for (int t = 0; t < K; t++) factor_tan[(size_t)K * i + t] = ok ? f.d[t] : 0;
+  }
+  return 0;
+}
+
+}  // extern "C"
