@@ -1,0 +1,179 @@
+"""Benchmark: events/s of the phase-space generator on one MI355X (FP64).
+
+Default workload: the 2->4 double-resonance process (three channels, 1e7 events, eight
+FP64 randoms per event) run as one batched launch over all its channels.  One step = one
+forward pass over all 1e7 events, randoms resident in HBM, outputs (momenta, factor, volume,
+ok) written to HBM.  The JSON line also carries the plain per-channel launches and the
+batched uneven mix as extra fields, the roofline of the dominant kernel and the reference's
+own CPU algorithm timed on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1] [--no-extra]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(wl, torch, G):
+    """Device randoms (seeded U(0,1) per segment) and output buffers of a workload."""
+    handles = [G.ProcessHandle(p) for p in wl.processes]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(wl.seed & 0x7FFFFFFFFFFFFFFF)
+    segs = []
+    for s in wl.segments:
+        ph = handles[s.process]
+        x = torch.rand((s.n_events, ph.n_x), dtype=torch.float64, device="cuda", generator=gen)
+        segs.append((ph, s.channel, G.EventBuffers.allocate(x, ph.n_out)))
+    return handles, segs
+
+
+def time_steps(fn, steps, warmup, torch):
+    """Median-free wall timing of `steps` calls bracketed by synchronize, plus HIP-event time
+    of the same region on the launch stream (torch's current stream)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, start.elapsed_time(end) / 1e3
+
+
+def run_workload(name, steps, warmup, torch, G, W, tune=True):
+    wl = W.workload(name)
+    handles, segs = make_inputs(wl, torch, G)
+    plan = G.Plan(segs)
+    tuned = plan.autotune(reps=3) if tune else ""
+    torch.cuda.synchronize()
+    wall, ev_s = time_steps(plan.run_graph, steps, warmup, torch)
+    plain_wall, plain_ev = time_steps(plan.run_plain, max(1, steps // 2), 1, torch)
+    n = wl.n_events
+    res = dict(workload=name, n_events=n, n_segments=len(segs),
+               batched_ev_s=n * steps / wall, batched_kernel_ms=ev_s / steps * 1e3,
+               plain_ev_s=n * max(1, steps // 2) / plain_wall, plain_ms=plain_ev / max(1, steps // 2) * 1e3,
+               algorithmic_bytes=wl.algorithmic_bytes(), config=plan.config(), tune=tuned[:200],
+               wall_s=wall, event_s=ev_s)
+    res["batched_vs_plain"] = res["batched_ev_s"] / res["plain_ev_s"]
+    return res, plan, segs, wl
+
+
+def cpu_baseline(wl_name, W):
+    """The reference's own phs_cpu.cpp (oracle/_ref build) on one host core, over a bounded
+    sample of the workload's events (repeated until ~10 s of CPU work)."""
+    import numpy as np
+    import oracle
+    wl = W.workload(wl_name, scale=0.2)   # 2e6 events for the 1e7-event workloads
+    proc = wl.processes[0]
+    ch = np.concatenate([np.full(s.n_events, s.channel, np.int32) for s in wl.segments])
+    x = np.random.default_rng(wl.seed).random((ch.shape[0], proc.n_x))
+    kind = "reference" if oracle.ref_available() else "port"
+    total_t, total_n, reps = 0.0, 0, 0
+    while total_t < 10.0 and reps < 5:
+        if kind == "reference":
+            sec, _ = oracle.ref_time(proc, x, ch)
+        else:
+            t0 = time.perf_counter()
+            oracle.forward(proc, x, ch)
+            sec = time.perf_counter() - t0
+        total_t += sec
+        total_n += ch.shape[0]
+        reps += 1
+    return dict(value=total_n / total_t, unit="events/s", cores=1, kind=kind,
+                sample=f"{ch.shape[0]} events of {wl_name} (seeded U(0,1) randoms, its channel split) x {reps} passes, "
+                       f"single thread, {total_t:.1f} s; reference phs_cpu.cpp built -O3")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import gpu_phs_amd as G
+    from gpu_phs_amd import workloads as W
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    G.lib()
+    log(f"workload {args.workload}: setup")
+    main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W)
+    log(f"workload {args.workload}: {main_res['batched_ev_s']:.4g} ev/s batched, "
+        f"{main_res['plain_ev_s']:.4g} ev/s plain")
+    kernel_s = main_res["batched_kernel_ms"] / 1e3
+    achieved = main_res["algorithmic_bytes"] / kernel_s / 1e9
+    extra = {}
+    if not args.no_extra:
+        del plan, segs
+        torch.cuda.empty_cache()
+        for name in ("c0", "c2", "c3"):
+            if name == args.workload:
+                continue
+            log(f"workload {name}: setup")
+            r, p2, s2, _ = run_workload(name, max(3, args.steps // 2), 1, torch, G, W)
+            extra[name] = {k: r[k] for k in ("n_events", "batched_ev_s", "plain_ev_s", "batched_vs_plain",
+                                              "batched_kernel_ms", "config")}
+            extra[name]["hbm_frac"] = r["algorithmic_bytes"] / (r["batched_kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
+            log(f"workload {name}: {r['batched_ev_s']:.4g} ev/s batched, {r['plain_ev_s']:.4g} plain")
+            del p2, s2
+            torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(args.workload, W)
+    steps = args.steps
+    out = {
+        "metric": "events/s @1 GPU (FP64, <=1e-12 rel vs phs_cpu), batched vs plain; fwd+adjoint ev/s",
+        "value": main_res["batched_ev_s"],
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_res["wall_s"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded U(0,1) FP64 randoms generated on the device",
+        "config": {"workload": f"{args.workload}: {W.DESCRIPTIONS[args.workload]}",
+                   "n_events": main_res["n_events"], "channels": main_res["n_segments"],
+                   "launch": "one batched launch replayed from a hipGraph", "tuned": main_res["config"]},
+        "plain_ev_s": main_res["plain_ev_s"],
+        "batched_vs_plain": main_res["batched_vs_plain"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "forward_kernel", "kernel_ms": main_res["batched_kernel_ms"],
+                     "bytes_per_launch": main_res["algorithmic_bytes"]},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,223 @@
+"""gpu_phs for MI355X: recursive phase-space generation (msq mapping, angles, boost chain)
+as hand-written HIP kernels behind the reference's C ABI.
+
+The native library ``libgpu_phs.so`` (built from ``csrc/``) is the product; this module binds
+its C ABI (include/gpu_phs.h) with ctypes.  PyTorch only provides device memory and streams.
+There is no CPU fallback: without the native library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from .process import Process  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpu_phs.so")
+CSRC = os.path.join(HERE, "csrc")
+
+GPU_PHS_OK = 0
+
+
+class GpuPhsError(RuntimeError):
+    pass
+
+
+def build(force: bool = True) -> str:
+    """Compile libgpu_phs.so for gfx950 (hipcc; works without a GPU)."""
+    args = ["make", "-C", CSRC, "-j8"]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+class _Segment(ctypes.Structure):
+    _fields_ = [("process", ctypes.c_int), ("channel", ctypes.c_int), ("n_events", ctypes.c_int64),
+                ("x", ctypes.c_void_p), ("p", ctypes.c_void_p), ("factors", ctypes.c_void_p),
+                ("volumes", ctypes.c_void_p), ("oks", ctypes.c_void_p)]
+
+
+_lib = None
+
+_SIGS = {
+    "gpu_phs_last_error": (ctypes.c_int, []),
+    "gpu_phs_last_error_message": (ctypes.c_char_p, []),
+    "gpu_phs_process_create": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.c_void_p] * 9 + [ctypes.c_double] * 2),
+    "gpu_phs_process_load_ref": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_double]),
+    "gpu_phs_process_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "gpu_phs_process_info": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3),
+    "gpu_phs_process_trees": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 7),
+    "gpu_phs_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gpu_phs_plan_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "gpu_phs_plan_set_config": (ctypes.c_int, [ctypes.c_int] * 5),
+    "gpu_phs_plan_get_config": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 4),
+    "gpu_phs_plan_run": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "gpu_phs_plan_run_plain": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "gpu_phs_plan_run_graph": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "gpu_phs_plan_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    "gpu_phs_plan_info": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3),
+}
+
+
+def lib():
+    """The loaded native library.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpuPhsError(f"{LIB_PATH} is missing: build it first (gpu_phs_amd.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what=""):
+    if rc < 0:
+        msg = lib().gpu_phs_last_error_message().decode()
+        raise GpuPhsError(f"{what}: {msg} ({rc})")
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class ProcessHandle:
+    """A compiled process (its trees turned into kernel programs for fixed beams)."""
+
+    def __init__(self, proc: Process, E1=None, E2=None):
+        self.proc = proc
+        E1 = proc.sqrts / 2 if E1 is None else E1
+        E2 = proc.sqrts / 2 if E2 is None else E2
+        d1, d2, hc, mid, mm, mw, flv = proc.ctypes_args()
+        fr = np.ascontiguousarray(proc.friends_or_zero(), np.int32)
+        flw = np.ascontiguousarray(proc.flv_widths, np.float64)
+        self._keep = (d1, d2, hc, mid, mm, mw, flv, fr, flw)
+        self.handle = _check(lib().gpu_phs_process_create(
+            proc.n_in, proc.n_out, proc.n_channels, _ptr(d1), _ptr(d2), _ptr(hc), _ptr(fr), _ptr(mid),
+            _ptr(mm), _ptr(mw), _ptr(flv), _ptr(flw), E1, E2), "process_create")
+
+    @property
+    def n_x(self):
+        return self.proc.n_x
+
+    @property
+    def n_out(self):
+        return self.proc.n_out
+
+    def close(self):
+        if self.handle >= 0 and _lib is not None:
+            lib().gpu_phs_process_destroy(self.handle)
+        self.handle = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_ref(path: str, E1: float, E2: float) -> int:
+    """Process handle from a reference-format .ref file (trees only)."""
+    return _check(lib().gpu_phs_process_load_ref(path.encode(), E1, E2), "process_load_ref")
+
+
+class EventBuffers:
+    """Device buffers of one segment (torch tensors on the current device)."""
+
+    def __init__(self, x, p, factor, volume, ok):
+        self.x, self.p, self.factor, self.volume, self.ok = x, p, factor, volume, ok
+
+    @staticmethod
+    def allocate(x, n_out):
+        import torch
+        n = x.shape[0]
+        dev = x.device
+        return EventBuffers(x, torch.empty((n, n_out, 4), dtype=torch.float64, device=dev),
+                            torch.empty(n, dtype=torch.float64, device=dev),
+                            torch.empty(n, dtype=torch.float64, device=dev),
+                            torch.empty(n, dtype=torch.uint8, device=dev))
+
+
+class Plan:
+    """A batch of (process, channel, event range) segments run as one launch.
+
+    ``segments``: list of ``(ProcessHandle, channel, EventBuffers)``.  The tensors must stay
+    alive and keep their storage while the plan is used.
+    """
+
+    def __init__(self, segments, block=256, ept=1, grid=0, order=0):
+        arr = (_Segment * len(segments))()
+        self._keep = []
+        for i, (ph, ch, buf) in enumerate(segments):
+            for t in (buf.x, buf.p, buf.factor, buf.volume, buf.ok):
+                if not t.is_contiguous():
+                    raise ValueError("segment tensors must be contiguous")
+            if buf.x.shape[1] != ph.n_x or tuple(buf.p.shape[1:]) != (ph.n_out, 4):
+                raise ValueError("segment buffer shapes do not match the process")
+            arr[i] = _Segment(ph.handle, ch, buf.x.shape[0], buf.x.data_ptr(), buf.p.data_ptr(),
+                              buf.factor.data_ptr(), buf.volume.data_ptr(), buf.ok.data_ptr())
+            self._keep.append((ph, buf))
+        self.handle = _check(lib().gpu_phs_plan_create(arr, len(segments)), "plan_create")
+        self.set_config(block, ept, grid, order)
+
+    def set_config(self, block=256, ept=1, grid=0, order=0):
+        _check(lib().gpu_phs_plan_set_config(self.handle, block, ept, grid, order), "plan_set_config")
+
+    def config(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().gpu_phs_plan_get_config(self.handle, *[ctypes.byref(q) for q in v]), "plan_get_config")
+        return dict(block=v[0].value, ept=v[1].value, grid=v[2].value, order=v[3].value)
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        _check(lib().gpu_phs_plan_info(self.handle, *[ctypes.byref(q) for q in v]), "plan_info")
+        return dict(n_tiles=v[0].value, n_segments=v[1].value, n_banks=v[2].value)
+
+    def run(self, stream=None):
+        _check(lib().gpu_phs_plan_run(self.handle, _stream_ptr(stream)), "plan_run")
+
+    def run_plain(self, stream=None):
+        _check(lib().gpu_phs_plan_run_plain(self.handle, _stream_ptr(stream)), "plan_run_plain")
+
+    def run_graph(self, stream=None):
+        _check(lib().gpu_phs_plan_run_graph(self.handle, _stream_ptr(stream)), "plan_run_graph")
+
+    def autotune(self, reps=3, stream=None):
+        buf = ctypes.create_string_buffer(8192)
+        _check(lib().gpu_phs_plan_autotune(self.handle, _stream_ptr(stream), reps, buf, len(buf)),
+               "plan_autotune")
+        return buf.value.decode()
+
+    def close(self):
+        if self.handle >= 0 and _lib is not None:
+            lib().gpu_phs_plan_destroy(self.handle)
+        self.handle = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate(ph: ProcessHandle, x, channel: int, stream=None, **cfg):
+    """One-channel convenience: forward pass of x (device [n, n_x] float64)."""
+    buf = EventBuffers.allocate(x, ph.n_out)
+    plan = Plan([(ph, channel, buf)], **cfg)
+    plan.run(stream)
+    return buf, plan

@@ -1,0 +1,430 @@
+// Process handles, batched plans, launches, hipGraph replay and the autotuner
+// (the additions of include/gpu_phs.h; no counterpart in the reference).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "phs_state.h"
+
+namespace gphs {
+
+namespace {
+thread_local int t_err = 0;
+thread_local std::string t_msg;
+std::map<int, std::unique_ptr<ProcessObj>> g_procs;
+std::map<int, std::unique_ptr<PlanObj>> g_plans;
+int g_next_proc = 0, g_next_plan = 0;
+}  // namespace
+
+int set_error(int code, const std::string& msg) {
+  t_err = code;
+  t_msg = msg;
+  return code;
+}
+
+int hip_error(hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return set_error(GPU_PHS_E_HIP, m);
+}
+
+std::mutex& api_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+int require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) return set_error(GPU_PHS_E_NODEVICE, "no GPU device is visible");
+  return GPU_PHS_OK;
+}
+
+int register_process(std::unique_ptr<ProcessObj> p) {
+  int h = g_next_proc++;
+  g_procs[h] = std::move(p);
+  return h;
+}
+ProcessObj* find_process(int h) {
+  auto it = g_procs.find(h);
+  return it == g_procs.end() ? nullptr : it->second.get();
+}
+int drop_process(int h) { return g_procs.erase(h) ? GPU_PHS_OK : GPU_PHS_E_HANDLE; }
+int register_plan(std::unique_ptr<PlanObj> p) {
+  int h = g_next_plan++;
+  g_plans[h] = std::move(p);
+  return h;
+}
+PlanObj* find_plan(int h) {
+  auto it = g_plans.find(h);
+  return it == g_plans.end() ? nullptr : it->second.get();
+}
+int drop_plan(int h) { return g_plans.erase(h) ? GPU_PHS_OK : GPU_PHS_E_HANDLE; }
+
+PlanObj::~PlanObj() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  if (dev) (void)hipFree(dev);
+}
+
+static void drop_graph(PlanObj& pl) {
+  if (pl.graph_exec) { (void)hipGraphExecDestroy(pl.graph_exec); pl.graph_exec = nullptr; }
+  if (pl.graph) { (void)hipGraphDestroy(pl.graph); pl.graph = nullptr; }
+}
+
+// Tables of a plan: channels of every process the segments use (deduplicated), their
+// programs, segment descriptors, and the tile table (tiles never cross a segment, so every
+// workgroup runs a single channel).
+int plan_build(PlanObj& pl) {
+  pl.chans.clear(); pl.nodes.clear(); pl.segs.clear(); pl.tiles.clear();
+  pl.seg_tile_begin.clear(); pl.seg_tile_end.clear();
+  pl.n_banks = 0;
+  std::map<std::pair<int, int>, int> chan_index;
+  for (const auto& us : pl.user) {
+    ProcessObj* po = find_process(us.process);
+    if (!po) return set_error(GPU_PHS_E_HANDLE, "segment refers to an unknown process");
+    const CompiledProcess& cp = po->cp;
+    if (us.channel < 0 || us.channel >= (int)cp.channels.size())
+      return set_error(GPU_PHS_E_ARG, "segment channel out of range");
+    if (us.n_events < 0) return set_error(GPU_PHS_E_ARG, "negative event count");
+    if (us.n_events > 0 && (!us.x || !us.p || !us.factors || !us.volumes || !us.oks))
+      return set_error(GPU_PHS_E_ARG, "segment buffers must not be NULL");
+    auto key = std::make_pair(us.process, us.channel);
+    int ci;
+    auto it = chan_index.find(key);
+    if (it == chan_index.end()) {
+      ChannelDesc cd = cp.channels[us.channel];
+      int node0 = (int)pl.nodes.size();
+      for (int i = 0; i < cd.n_nodes; i++) pl.nodes.push_back(cp.nodes[cd.node0 + i]);
+      cd.node0 = node0;
+      cd.param0 = node0;
+      ci = (int)pl.chans.size();
+      pl.chans.push_back(cd);
+      chan_index[key] = ci;
+      pl.n_banks = std::max(pl.n_banks, (int)cd.n_banks);
+    } else {
+      ci = it->second;
+    }
+    SegDesc sd{};
+    sd.channel = ci;
+    sd.n_events = us.n_events;
+    sd.x = us.x; sd.p = us.p; sd.factor = us.factors; sd.volume = us.volumes; sd.ok = us.oks;
+    pl.segs.push_back(sd);
+  }
+  if (pl.n_banks > 3) return set_error(GPU_PHS_E_TREE, "tree needs more than 3 save banks");
+  const int64_t tile = (int64_t)pl.lc.block * pl.lc.ept;
+  std::vector<TileDesc> tiles;
+  std::vector<int> tile_depth;
+  for (size_t s = 0; s < pl.segs.size(); s++) {
+    pl.seg_tile_begin.push_back((int)tiles.size());
+    for (int64_t e0 = 0; e0 < pl.segs[s].n_events; e0 += tile) {
+      TileDesc td{};
+      td.seg = (int32_t)s;
+      td.e0 = e0;
+      td.count = (int32_t)std::min<int64_t>(tile, pl.segs[s].n_events - e0);
+      tiles.push_back(td);
+      tile_depth.push_back(pl.chans[pl.segs[s].channel].n_nodes);
+    }
+    pl.seg_tile_end.push_back((int)tiles.size());
+  }
+  if (tiles.size() > (size_t)0x7fffffff) return set_error(GPU_PHS_E_ARG, "too many tiles");
+  if (pl.order == 1) {
+    // heaviest programs first, so the tail of the launch is made of short tiles
+    std::vector<int> idx(tiles.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return tile_depth[a] > tile_depth[b]; });
+    pl.tiles.resize(tiles.size());
+    for (size_t i = 0; i < idx.size(); i++) pl.tiles[i] = tiles[idx[i]];
+  } else {
+    pl.tiles = std::move(tiles);
+  }
+
+  // upload: one allocation, 256-byte aligned sections
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t o_tiles = 0, o_segs = al(pl.tiles.size() * sizeof(TileDesc));
+  size_t o_chans = o_segs + al(pl.segs.size() * sizeof(SegDesc));
+  size_t o_nodes = o_chans + al(pl.chans.size() * sizeof(ChannelDesc));
+  size_t total = o_nodes + al(pl.nodes.size() * sizeof(NodeCmd)) + 256;
+  drop_graph(pl);
+  if (total > pl.dev_bytes) {
+    if (pl.dev) (void)hipFree(pl.dev);
+    pl.dev = nullptr;
+    pl.dev_bytes = 0;
+    hipError_t e = hipMalloc(&pl.dev, total);
+    if (e != hipSuccess) return hip_error(e, "hipMalloc(plan tables)");
+    pl.dev_bytes = total;
+  }
+  std::vector<char> host(total, 0);
+  if (!pl.tiles.empty()) std::memcpy(host.data() + o_tiles, pl.tiles.data(), pl.tiles.size() * sizeof(TileDesc));
+  if (!pl.segs.empty()) std::memcpy(host.data() + o_segs, pl.segs.data(), pl.segs.size() * sizeof(SegDesc));
+  if (!pl.chans.empty()) std::memcpy(host.data() + o_chans, pl.chans.data(), pl.chans.size() * sizeof(ChannelDesc));
+  if (!pl.nodes.empty()) std::memcpy(host.data() + o_nodes, pl.nodes.data(), pl.nodes.size() * sizeof(NodeCmd));
+  hipError_t e = hipMemcpy(pl.dev, host.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_error(e, "hipMemcpy(plan tables)");
+  char* base = (char*)pl.dev;
+  pl.dp.tiles = (const TileDesc*)(base + o_tiles);
+  pl.dp.segs = (const SegDesc*)(base + o_segs);
+  pl.dp.chans = (const ChannelDesc*)(base + o_chans);
+  pl.dp.nodes = (const NodeCmd*)(base + o_nodes);
+  pl.dp.n_tiles = (int)pl.tiles.size();
+  pl.dp.n_banks = pl.n_banks;
+  return ok_status();
+}
+
+int plan_run(PlanObj& pl, hipStream_t stream) {
+  hipError_t e = launch_forward(pl.dp, 0, pl.dp.n_tiles, pl.lc, stream);
+  if (e != hipSuccess) return hip_error(e, "forward launch");
+  return ok_status();
+}
+
+int plan_run_plain(PlanObj& pl, hipStream_t stream) {
+  LaunchConfig lc = pl.lc;
+  lc.grid = 0;
+  for (size_t s = 0; s < pl.segs.size(); s++) {
+    hipError_t e = launch_forward(pl.dp, pl.seg_tile_begin[s], pl.seg_tile_end[s], lc, stream);
+    if (e != hipSuccess) return hip_error(e, "forward launch (plain)");
+  }
+  return ok_status();
+}
+
+}  // namespace gphs
+
+using namespace gphs;
+
+extern "C" {
+
+int gpu_phs_last_error(void) { return t_err; }
+const char* gpu_phs_last_error_message(void) { return t_msg.c_str(); }
+
+int gpu_phs_process_create(int n_in, int n_out, int n_channels, const int* d1, const int* d2,
+                           const int* has_children, const int* friends, const int* map_ids,
+                           const double* map_masses, const double* map_widths, const double* flv_masses,
+                           const double* flv_widths, double E1, double E2) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  if (n_in != 2 || n_out < 2 || n_out > kMaxOut || n_channels < 1)
+    return set_error(GPU_PHS_E_ARG, "need n_in = 2, 2 <= n_out <= 12, n_channels >= 1");
+  if (!d1 || !d2 || !has_children || !map_ids || !map_masses || !map_widths || !flv_masses)
+    return set_error(GPU_PHS_E_ARG, "tree arrays must not be NULL");
+  ProcessSpec sp;
+  sp.n_in = n_in; sp.n_out = n_out; sp.n_channels = n_channels;
+  size_t np = (size_t)n_channels * sp.n_prt(), npo = (size_t)n_channels * sp.n_prt_out();
+  sp.d1.assign(d1, d1 + np);
+  sp.d2.assign(d2, d2 + np);
+  sp.has_children.assign(has_children, has_children + np);
+  if (friends) sp.friends.assign(friends, friends + np);
+  else sp.friends.assign(np, 0);
+  sp.map_ids.assign(map_ids, map_ids + npo);
+  sp.map_masses.assign(map_masses, map_masses + npo);
+  sp.map_widths.assign(map_widths, map_widths + npo);
+  sp.flv_masses.assign(flv_masses, flv_masses + n_in + n_out);
+  if (flv_widths) sp.flv_widths.assign(flv_widths, flv_widths + n_in + n_out);
+  else sp.flv_widths.assign(n_in + n_out, 0.0);
+  sp.E1 = E1; sp.E2 = E2;
+  auto po = std::make_unique<ProcessObj>();
+  std::string err = compile_process(sp, po->cp);
+  if (!err.empty()) return set_error(GPU_PHS_E_TREE, err);
+  int h = register_process(std::move(po));
+  ok_status();
+  return h;
+}
+
+int gpu_phs_process_destroy(int process) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  if (drop_process(process) != GPU_PHS_OK) return set_error(GPU_PHS_E_HANDLE, "unknown process");
+  return ok_status();
+}
+
+int gpu_phs_process_info(int process, int* n_out, int* n_x, int* n_channels) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  ProcessObj* po = find_process(process);
+  if (!po) return set_error(GPU_PHS_E_HANDLE, "unknown process");
+  if (n_out) *n_out = po->cp.spec.n_out;
+  if (n_x) *n_x = po->cp.spec.n_x();
+  if (n_channels) *n_channels = po->cp.spec.n_channels;
+  return ok_status();
+}
+
+int gpu_phs_process_trees(int process, int* d1, int* d2, int* has_children, int* map_ids, double* map_masses,
+                          double* map_widths, double* flv_masses) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  ProcessObj* po = find_process(process);
+  if (!po) return set_error(GPU_PHS_E_HANDLE, "unknown process");
+  const ProcessSpec& sp = po->cp.spec;
+  auto cp = [](auto* dst, const auto& v) { if (dst) std::copy(v.begin(), v.end(), dst); };
+  cp(d1, sp.d1); cp(d2, sp.d2); cp(has_children, sp.has_children); cp(map_ids, sp.map_ids);
+  cp(map_masses, sp.map_masses); cp(map_widths, sp.map_widths); cp(flv_masses, sp.flv_masses);
+  return ok_status();
+}
+
+int gpu_phs_plan_create(const gpu_phs_segment* segments, int n_segments) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  if (n_segments < 1 || !segments) return set_error(GPU_PHS_E_ARG, "need at least one segment");
+  int rc = require_device();
+  if (rc) return rc;
+  auto pl = std::make_unique<PlanObj>();
+  pl->user.assign(segments, segments + n_segments);
+  rc = plan_build(*pl);
+  if (rc) return rc;
+  int h = register_plan(std::move(pl));
+  ok_status();
+  return h;
+}
+
+int gpu_phs_plan_destroy(int plan) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  if (drop_plan(plan) != GPU_PHS_OK) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  return ok_status();
+}
+
+int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1)
+    return set_error(GPU_PHS_E_ARG, "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}");
+  bool rebuild = block != pl->lc.block || ept != pl->lc.ept || order != pl->order;
+  pl->lc.block = block; pl->lc.ept = ept; pl->order = order;
+  if (grid != pl->lc.grid) drop_graph(*pl);
+  pl->lc.grid = grid;
+  if (rebuild) return plan_build(*pl);
+  return ok_status();
+}
+
+int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (block) *block = pl->lc.block;
+  if (ept) *ept = pl->lc.ept;
+  if (grid) *grid = pl->lc.grid;
+  if (order) *order = pl->order;
+  return ok_status();
+}
+
+int gpu_phs_plan_info(int plan, int* n_tiles, int* n_segments, int* n_banks) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (n_tiles) *n_tiles = pl->dp.n_tiles;
+  if (n_segments) *n_segments = (int)pl->segs.size();
+  if (n_banks) *n_banks = pl->n_banks;
+  return ok_status();
+}
+
+int gpu_phs_plan_run(int plan, void* stream) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  return plan_run(*pl, (hipStream_t)stream);
+}
+
+int gpu_phs_plan_run_plain(int plan, void* stream) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  return plan_run_plain(*pl, (hipStream_t)stream);
+}
+
+int gpu_phs_plan_run_graph(int plan, void* stream) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  hipError_t e;
+  if (!pl->graph_exec) {
+    if (!pl->capture_stream) {
+      e = hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_error(e, "hipStreamCreate");
+    }
+    e = hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_error(e, "hipStreamBeginCapture");
+    hipError_t le = launch_forward(pl->dp, 0, pl->dp.n_tiles, pl->lc, pl->capture_stream);
+    e = hipStreamEndCapture(pl->capture_stream, &pl->graph);
+    if (le != hipSuccess) return hip_error(le, "forward launch (capture)");
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&pl->graph_exec, pl->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+  }
+  e = hipGraphLaunch(pl->graph_exec, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_error(e, "hipGraphLaunch");
+  return ok_status();
+}
+
+int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int report_len) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  hipStream_t stream = (hipStream_t)stream_v;
+  if (reps < 1) reps = 3;
+  int dev = 0, n_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (n_cu < 1) n_cu = 256;
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    return set_error(GPU_PHS_E_HIP, "hipEventCreate");
+  auto time_cfg = [&](int block, int ept, int grid, int order, float* ms) -> int {
+    pl->lc.block = block; pl->lc.ept = ept; pl->lc.grid = grid; pl->order = order;
+    int rc = plan_build(*pl);
+    if (rc) return rc;
+    rc = plan_run(*pl, stream);   // warm-up
+    if (rc) return rc;
+    std::vector<float> t;
+    for (int r = 0; r < reps; r++) {
+      (void)hipEventRecord(a, stream);
+      rc = plan_run(*pl, stream);
+      if (rc) return rc;
+      (void)hipEventRecord(b, stream);
+      if (hipEventSynchronize(b) != hipSuccess) return set_error(GPU_PHS_E_HIP, "hipEventSynchronize");
+      float x = 0;
+      (void)hipEventElapsedTime(&x, a, b);
+      t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return GPU_PHS_OK;
+  };
+  struct Cfg { int block, ept, grid, order; float ms; };
+  std::vector<Cfg> tried;
+  Cfg best{pl->lc.block, pl->lc.ept, pl->lc.grid, pl->order, 1e30f};
+  auto consider = [&](int block, int ept, int grid, int order) -> int {
+    float ms = 0;
+    int rc = time_cfg(block, ept, grid, order, &ms);
+    if (rc) return rc;
+    tried.push_back({block, ept, grid, order, ms});
+    if (ms < best.ms) best = {block, ept, grid, order, ms};
+    return GPU_PHS_OK;
+  };
+  int rc = GPU_PHS_OK;
+  // stage 1: workgroup size x events per thread, one workgroup per tile
+  for (int block : {64, 128, 256})
+    for (int ept : {1, 2, 4})
+      if (!rc) rc = consider(block, ept, 0, 0);
+  // stage 2: persistent grids and tile order around the best shape
+  const int b0 = best.block, e0 = best.ept;
+  for (int k : {1, 2, 4, 8})
+    if (!rc) rc = consider(b0, e0, n_cu * k * (256 / b0), 0);
+  for (int order : {1})
+    if (!rc) rc = consider(b0, e0, best.grid, order);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) return rc;
+  pl->lc.block = best.block; pl->lc.ept = best.ept; pl->lc.grid = best.grid; pl->order = best.order;
+  rc = plan_build(*pl);
+  if (rc) return rc;
+  if (report && report_len > 0) {
+    std::string s;
+    char buf[96];
+    snprintf(buf, sizeof buf, "best block=%d ept=%d grid=%d order=%d %.4f ms;", best.block, best.ept, best.grid,
+             best.order, best.ms);
+    s += buf;
+    for (const auto& c : tried) {
+      snprintf(buf, sizeof buf, " %d/%d/%d/%d:%.4f", c.block, c.ept, c.grid, c.order, c.ms);
+      s += buf;
+    }
+    snprintf(report, report_len, "%s", s.c_str());
+  }
+  return ok_status();
+}
+
+}  // extern "C"

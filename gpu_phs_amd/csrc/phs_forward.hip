@@ -1,0 +1,176 @@
+// Forward phase-space kernel: msq mapping -> angles -> boost chain, fused, one lane per
+// event, all per-event state in VGPRs.
+//
+// Replaces the reference's pipeline of 3 + 2(n-1) + 2 launches with HBM-resident scratch
+// (_init_*, _apply_msq, _create_boosts_step_*, _apply_boost_targets, _boost_to_lab_frame;
+// phs.cu:1430-1560).  A workgroup runs one tile (a contiguous run of events of one channel);
+// the channel's program (NodeCmd records, phs_types.h) is read with scalar loads, so map
+// selection is a wave-uniform branch instead of the reference's per-lane function pointers
+// (mappings_gpu.hpp:312-363).  The boost matrix of the branch being visited stays in
+// registers; the matrix of a deferred subtree waits in one of NBANK save banks.
+#include <hip/hip_runtime.h>
+
+#include "phs_device.h"
+#include "phs_kernels.h"
+
+namespace gphs {
+
+namespace {
+
+template <int NBANK>
+struct Banks {
+  double L[NBANK][16];
+  double msq[NBANK], p[NBANK];
+
+  __device__ __forceinline__ void push(int d, const double (&M)[16], double m2, double pp) {
+#pragma unroll
+    for (int b = 0; b < NBANK; b++)
+      if (b == d) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) L[b][i] = M[i];
+        msq[b] = m2; p[b] = pp;
+      }
+  }
+  __device__ __forceinline__ void pop(int d, double (&M)[16], double& m2, double& pp) const {
+#pragma unroll
+    for (int b = 0; b < NBANK; b++)
+      if (b == d) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) M[i] = L[b][i];
+        m2 = msq[b]; pp = p[b];
+      }
+  }
+};
+
+template <>
+struct Banks<0> {
+  __device__ __forceinline__ void push(int, const double (&)[16], double, double) {}
+  __device__ __forceinline__ void pop(int, double (&)[16], double&, double&) const {}
+};
+
+// One event of channel `cd` (program `prog`).
+template <int NBANK>
+__device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
+                                              const SegDesc& sd, int64_t e, bool valid) {
+  const gdouble* __restrict__ xr = as_global(sd.x) + e * cd.n_x;
+  gdouble* __restrict__ pr = as_global(sd.p) + e * (int64_t)cd.n_out * 4;
+  const double s = cd.s;
+
+  // boost of the parent branch; the root's parent is the identity (phs_cpu.cpp:568-573)
+  double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  Banks<NBANK> bank;
+  double cur_msq = s;    // root: msq = s (phs_cpu.cpp:195)
+  double cur_p = 0.0;    // root: p_decay = 0 (never set by the reference)
+  double factor = 1.0;
+  bool ok = true;
+  int depth = 0;
+
+  for (int i = 0; i < cd.n_nodes; ++i) {
+    const NodeCmd& nc = prog[i];
+    // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547)
+    double msq0, f0, msq1, f1;
+    if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; }
+    else msq_map(nc.k[0], xr[nc.k[0].xi], s, msq0, f0);
+    if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; }
+    else msq_map(nc.k[1], xr[nc.k[1].xi], s, msq1, f1);
+
+    // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
+    const double m = sqrt(cur_msq);
+    const double m1 = sqrt(msq0), m2 = sqrt(msq1);
+    const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
+    ok = ok && (lda > 0 && m > m1 + m2 && m <= nc.m_max);
+    const double sl = sqrt(lda);
+    const double pk = sl / (2 * m);
+    factor *= f0 * f1 * (sl / cur_msq);
+
+    // angles and the boost of this branch (phs_cpu.cpp:299-340)
+    const double bg = m > 0 ? cur_p / m : 0.0;
+    const double g = sqrt(1 + bg * bg);
+    const double phi = xr[nc.x_phi] * GPHS_TWOPI;
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    double ct, st, fct;
+    ct_map(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1, ct, st, fct);
+    factor *= fct;
+    compose(M, g, bg, ct, st, cp, sp);
+
+    // final-state daughters: p = L (E, 0, 0, +-p) (phs_cpu.cpp:286-291), then the lab boost
+    // of the final leaves (phs.cu:1286-1301)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      if (nc.k[c].leaf) {
+        const double pc = c == 0 ? pk : -pk;
+        const double E = sqrt(nc.k[c].msq_leaf + pc * pc);
+        double q0 = M[0] * E + M[3] * pc;
+        const double q1 = M[4] * E + M[7] * pc;
+        const double q2 = M[8] * E + M[11] * pc;
+        double q3 = M[12] * E + M[15] * pc;
+        if (cd.bg_lab != 0) {
+          const double b0 = cd.g_lab * q0 + cd.bg_lab * q3;
+          const double b3 = cd.bg_lab * q0 + cd.g_lab * q3;
+          q0 = b0; q3 = b3;
+        }
+        if (valid) {
+          gdouble* o = pr + 4 * nc.k[c].out;
+          o[0] = q0; o[1] = q1; o[2] = q2; o[3] = q3;
+        }
+      }
+    }
+
+    // next branch: a daughter (matrix stays), or the deferred one from the save bank
+    const double msq_push = nc.push == 0 ? msq0 : msq1;
+    const double p_push = nc.push == 0 ? pk : -pk;
+    if (nc.push >= 0) { bank.push(depth, M, msq_push, p_push); depth++; }
+    if (nc.desc >= 0) {
+      cur_msq = nc.desc == 0 ? msq0 : msq1;
+      cur_p = nc.desc == 0 ? pk : -pk;
+    } else if (depth > 0) {
+      depth--;
+      bank.pop(depth, M, cur_msq, cur_p);
+    }
+  }
+  if (valid) {
+    as_global(sd.factor)[e] = factor;
+    as_global(sd.volume)[e] = cd.volume;
+    as_global(sd.ok)[e] = ok ? 1 : 0;
+  }
+}
+
+template <int NBANK>
+__global__ __launch_bounds__(256) void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end,
+                                                      const SegDesc* __restrict__ segs,
+                                                      const ChannelDesc* __restrict__ chans,
+                                                      const NodeCmd* __restrict__ nodes, int ept) {
+  for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
+    const TileDesc td = tiles[t];
+    const SegDesc sd = segs[td.seg];
+    const ChannelDesc cd = chans[sd.channel];
+    const NodeCmd* prog = nodes + cd.node0;
+    for (int k = 0; k < ept; ++k) {
+      const int base = k * (int)blockDim.x;
+      if (base >= td.count) break;
+      const int local = base + (int)threadIdx.x;
+      const bool valid = local < td.count;
+      const int64_t e = td.e0 + (valid ? local : td.count - 1);
+      forward_event<NBANK>(cd, prog, sd, e, valid);
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream) {
+  if (t_end <= t_begin) return hipSuccess;
+  const int n = t_end - t_begin;
+  const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
+  const dim3 g(grid), b(lc.block);
+  switch (dp.n_banks) {
+    case 0: hipLaunchKernelGGL(forward_kernel<0>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
+    case 1: hipLaunchKernelGGL(forward_kernel<1>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
+    case 2: hipLaunchKernelGGL(forward_kernel<2>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
+    default: hipLaunchKernelGGL(forward_kernel<3>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gphs

@@ -1,0 +1,63 @@
+// Library-internal state shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpu_phs.h"
+#include "phs_kernels.h"
+#include "tree_compiler.h"
+
+namespace gphs {
+
+// ---- errors: every entry point records its status for gpu_phs_last_error()
+int set_error(int code, const std::string& msg);
+inline int ok_status() { return set_error(GPU_PHS_OK, ""); }
+int hip_error(hipError_t e, const char* what);
+
+std::mutex& api_mutex();
+
+struct ProcessObj {
+  CompiledProcess cp;
+};
+
+struct PlanObj {
+  std::vector<gpu_phs_segment> user;       // segments as given
+  std::vector<ChannelDesc> chans;
+  std::vector<NodeCmd> nodes;
+  std::vector<SegDesc> segs;
+  std::vector<TileDesc> tiles;
+  std::vector<int> seg_tile_begin, seg_tile_end;  // tile ranges per segment (segment order)
+  int n_banks = 0;
+  LaunchConfig lc;
+  int order = 0;
+  void* dev = nullptr;                      // one allocation: tiles | segs | chans | nodes
+  size_t dev_bytes = 0;
+  DevicePlan dp;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+  ~PlanObj();
+};
+
+// Registry (guarded by api_mutex()).
+int register_process(std::unique_ptr<ProcessObj> p);
+ProcessObj* find_process(int h);
+int drop_process(int h);
+int register_plan(std::unique_ptr<PlanObj> p);
+PlanObj* find_plan(int h);
+int drop_plan(int h);
+
+// Build a plan's tables (host side) and upload them.  Returns a status.
+int plan_build(PlanObj& pl);
+int plan_run(PlanObj& pl, hipStream_t stream);
+int plan_run_plain(PlanObj& pl, hipStream_t stream);
+
+// Device check used by every entry point that touches the GPU.
+int require_device();
+
+}  // namespace gphs

@@ -1,0 +1,372 @@
+// The Whizard-facing C ABI (include/gpu_phs.h group 1): the reference's 14 c_whizard_* entry
+// points with the same signatures and index conventions (whizard_gpu_phs.cpp:12-249), on top
+// of the process/plan machinery of phs_api.cpp.  The reference keeps its state in process
+// globals (global_phs.cpp:6-27, phs.cu:27-41); this file keeps the equivalent in one static
+// module object, so the calls keep their order-dependent semantics.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "phs_state.h"
+
+namespace gphs {
+namespace {
+
+struct WhizardModule {
+  int n_channels = 0, n_in = 0, n_out = 0, n_x = 0;
+  bool sized = false, maps_ready = false, trees_ready = false, initialized = false;
+  ProcessSpec spec;                 // trees and maps as filled
+  std::vector<int> channel_ids;     // from c_whizard_init_channel_ids
+  int block = 256;
+  // compiled process for the current beams
+  int process = -1;
+  double E1 = -1, E2 = -1;
+  // device buffers kept between calls (grow-only)
+  double* x_d = nullptr; double* p_d = nullptr; double* f_d = nullptr; double* v_d = nullptr;
+  uint8_t* ok_d = nullptr;
+  size_t cap = 0;                   // events the buffers hold
+  int cap_nx = 0, cap_nout = 0;
+  double* last_p = nullptr;         // device momenta of the last generate call
+};
+
+WhizardModule& W() {
+  static WhizardModule w;
+  return w;
+}
+
+void free_buffers(WhizardModule& w) {
+  (void)hipFree(w.x_d); (void)hipFree(w.p_d); (void)hipFree(w.f_d); (void)hipFree(w.v_d); (void)hipFree(w.ok_d);
+  w.x_d = w.p_d = w.f_d = w.v_d = nullptr;
+  w.ok_d = nullptr;
+  w.cap = 0;
+  w.last_p = nullptr;
+}
+
+int ensure_buffers(WhizardModule& w, size_t n) {
+  if (n <= w.cap && w.cap_nx == w.n_x && w.cap_nout == w.n_out) return GPU_PHS_OK;
+  free_buffers(w);
+  hipError_t e;
+  if ((e = hipMalloc(&w.x_d, n * w.n_x * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.p_d, n * w.n_out * 4 * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.f_d, n * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.v_d, n * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.ok_d, n)) != hipSuccess) {
+    free_buffers(w);
+    return hip_error(e, "hipMalloc(event buffers)");
+  }
+  w.cap = n; w.cap_nx = w.n_x; w.cap_nout = w.n_out;
+  return GPU_PHS_OK;
+}
+
+int compile_for_beams(WhizardModule& w, double E1, double E2) {
+  if (w.process >= 0 && E1 == w.E1 && E2 == w.E2) return GPU_PHS_OK;
+  if (w.process >= 0) { drop_process(w.process); w.process = -1; }
+  ProcessSpec sp = w.spec;
+  sp.E1 = E1; sp.E2 = E2;
+  auto po = std::make_unique<ProcessObj>();
+  std::string err = compile_process(sp, po->cp);
+  if (!err.empty()) return set_error(GPU_PHS_E_TREE, err);
+  w.process = register_process(std::move(po));
+  w.E1 = E1; w.E2 = E2;
+  return GPU_PHS_OK;
+}
+
+}  // namespace
+}  // namespace gphs
+
+using namespace gphs;
+
+extern "C" {
+
+void c_whizard_set_particle_structure(int* n_channels, int* n_trees, int* n_groves, int* n_x, int* n_in, int* n_out) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  (void)n_trees; (void)n_groves;
+  WhizardModule& w = W();
+  if (!n_channels || !n_x || !n_in || !n_out || *n_channels < 1 || *n_in != 2 || *n_out < 2 || *n_out > kMaxOut) {
+    set_error(GPU_PHS_E_ARG, "set_particle_structure: need n_channels >= 1, n_in = 2, 2 <= n_out <= 12");
+    return;
+  }
+  w.n_channels = *n_channels; w.n_in = *n_in; w.n_out = *n_out; w.n_x = *n_x;
+  w.spec = ProcessSpec();
+  w.spec.n_in = w.n_in; w.spec.n_out = w.n_out; w.spec.n_channels = w.n_channels;
+  w.spec.flv_masses.assign(w.n_in + w.n_out, 0.0);
+  w.spec.flv_widths.assign(w.n_in + w.n_out, 0.0);
+  w.sized = true; w.maps_ready = w.trees_ready = w.initialized = false;
+  if (w.n_x != w.spec.n_x()) {
+    set_error(GPU_PHS_E_ARG, "set_particle_structure: n_x must be 3 n_out - 4");
+    return;
+  }
+  ok_status();
+}
+
+void c_whizard_set_flavors(double* masses, double* widths) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!w.sized) { set_error(GPU_PHS_E_STATE, "set_flavors before set_particle_structure"); return; }
+  for (int i = 0; i < w.n_in + w.n_out; i++) {
+    w.spec.flv_masses[i] = masses[i];
+    w.spec.flv_widths[i] = widths ? widths[i] : 0.0;
+  }
+  if (w.process >= 0) { drop_process(w.process); w.process = -1; }
+  ok_status();
+}
+
+void c_whizard_init_mappings(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!w.sized) { set_error(GPU_PHS_E_STATE, "init_mappings before set_particle_structure"); return; }
+  size_t n = (size_t)w.n_channels * w.spec.n_prt_out();
+  w.spec.map_ids.assign(n, 0);
+  w.spec.map_masses.assign(n, 0.0);
+  w.spec.map_widths.assign(n, 0.0);
+  w.maps_ready = true;
+  ok_status();
+}
+
+void c_whizard_fill_mapping(int* channel, int* map_ids, double* masses, double* widths) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!w.maps_ready) { set_error(GPU_PHS_E_STATE, "fill_mapping before init_mappings"); return; }
+  int c = *channel - 1;  // Fortran -> C index
+  if (c < 0 || c >= w.n_channels) { set_error(GPU_PHS_E_ARG, "fill_mapping: channel out of range"); return; }
+  const int n = w.spec.n_prt_out();
+  for (int i = 0; i < n; i++) {
+    w.spec.map_ids[c * n + i] = map_ids[i];
+    w.spec.map_masses[c * n + i] = masses[i];
+    w.spec.map_widths[c * n + i] = widths[i];
+  }
+  ok_status();
+}
+
+void c_whizard_init_tree_structures(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!w.sized) { set_error(GPU_PHS_E_STATE, "init_tree_structures before set_particle_structure"); return; }
+  size_t n = (size_t)w.n_channels * w.spec.n_prt();
+  w.spec.d1.assign(n, 0); w.spec.d2.assign(n, 0); w.spec.has_children.assign(n, 0); w.spec.friends.assign(n, 0);
+  w.trees_ready = true;
+  ok_status();
+}
+
+void c_whizard_fill_tree_structure(int* channel, int* daughters1, int* daughters2, int* has_children, int* friends) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!w.trees_ready) { set_error(GPU_PHS_E_STATE, "fill_tree_structure before init_tree_structures"); return; }
+  int c = *channel - 1;
+  if (c < 0 || c >= w.n_channels) { set_error(GPU_PHS_E_ARG, "fill_tree_structure: channel out of range"); return; }
+  const int n = w.spec.n_prt();
+  for (int i = 0; i < n; i++) {
+    w.spec.d1[c * n + i] = daughters1[i];
+    w.spec.d2[c * n + i] = daughters2[i];
+    w.spec.has_children[c * n + i] = has_children[i];
+    w.spec.friends[c * n + i] = friends ? friends[i] : 0;
+  }
+  ok_status();
+}
+
+void c_whizard_init_channel_ids(int* batch_size, int* n_channels, int* channel_limits) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!batch_size || !n_channels || !channel_limits || *batch_size < 0 || *n_channels < 1) {
+    set_error(GPU_PHS_E_ARG, "init_channel_ids: invalid arguments");
+    return;
+  }
+  // Event i belongs to the first channel c with i < channel_limits[c] (cumulative ends).
+  // The reference skips only leading empty channels (whizard_gpu_phs.cpp:93-97); every empty
+  // channel is skipped here.
+  w.channel_ids.assign(*batch_size, -1);
+  int c = 0;
+  for (int i = 0; i < *batch_size; i++) {
+    while (c < *n_channels && channel_limits[c] <= i) c++;
+    if (c >= *n_channels) {
+      set_error(GPU_PHS_E_ARG, "init_channel_ids: channel_limits end before batch_size");
+      w.channel_ids.clear();
+      return;
+    }
+    w.channel_ids[i] = c;
+  }
+  ok_status();
+}
+
+void c_whizard_set_threads(int* msq_threads, int* cb_threads, int* ab_threads) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  (void)cb_threads; (void)ab_threads;
+  int t = msq_threads ? *msq_threads : 256;
+  t = (t / 64) * 64;
+  if (t < 64) t = 64;
+  if (t > 256) t = 256;
+  W().block = t;
+  ok_status();
+}
+
+void c_whizard_init_gpu_phs(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  if (!w.sized || !w.maps_ready || !w.trees_ready) {
+    set_error(GPU_PHS_E_STATE, "init_gpu_phs before the structure, mappings and trees were set");
+    return;
+  }
+  // validate the trees now (constants are recompiled for the beams of each generate call)
+  ProcessSpec sp = w.spec;
+  sp.E1 = sp.E2 = 0.5 * 1e3;
+  CompiledProcess cp;
+  std::string err = compile_process(sp, cp);
+  if (!err.empty()) { set_error(GPU_PHS_E_TREE, err); return; }
+  if (require_device()) return;
+  w.initialized = true;
+  ok_status();
+}
+
+void c_whizard_finalize_gpu_phs(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  free_buffers(w);
+  if (w.process >= 0) drop_process(w.process);
+  w = WhizardModule();
+  ok_status();
+}
+
+void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_boost_to_lab, int* n_events,
+                                  int* n_channels, int* n_x, double* x, double* factors, double* volumes, bool* oks,
+                                  double* p, double* x_out) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  (void)requires_boost_to_lab;
+  WhizardModule& w = W();
+  if (!w.initialized) { set_error(GPU_PHS_E_STATE, "gen_phs_from_x_gpu before init_gpu_phs"); return; }
+  const int64_t n = *n_events;
+  if (n <= 0) { ok_status(); return; }
+  if (*n_x != w.n_x || *n_channels != w.n_channels) {
+    set_error(GPU_PHS_E_ARG, "gen_phs_from_x_gpu: n_x / n_channels differ from set_particle_structure");
+    return;
+  }
+  if ((int64_t)w.channel_ids.size() < n) {
+    set_error(GPU_PHS_E_STATE, "gen_phs_from_x_gpu: init_channel_ids covers fewer events");
+    return;
+  }
+  const double E1 = E1_in[0], E2 = E2_in[0];
+  for (int64_t i = 1; i < n; i++)
+    if (E1_in[i] != E1 || E2_in[i] != E2) {
+      set_error(GPU_PHS_E_ARG, "gen_phs_from_x_gpu: per-event beam energies that differ between events are not supported");
+      return;
+    }
+  if (compile_for_beams(w, E1, E2)) return;
+  if (ensure_buffers(w, (size_t)n)) return;
+  hipError_t e = hipMemcpy(w.x_d, x, (size_t)n * w.n_x * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { hip_error(e, "hipMemcpy(x)"); return; }
+
+  // segments = runs of equal channel id
+  std::vector<gpu_phs_segment> segs;
+  for (int64_t i = 0; i < n;) {
+    int64_t j = i;
+    while (j < n && w.channel_ids[j] == w.channel_ids[i]) j++;
+    gpu_phs_segment s{};
+    s.process = w.process;
+    s.channel = w.channel_ids[i];
+    s.n_events = j - i;
+    s.x = w.x_d + i * w.n_x;
+    s.p = w.p_d + i * w.n_out * 4;
+    s.factors = w.f_d + i;
+    s.volumes = w.v_d + i;
+    s.oks = w.ok_d + i;
+    segs.push_back(s);
+    i = j;
+  }
+  PlanObj pl;
+  pl.user = segs;
+  pl.lc.block = w.block;
+  if (plan_build(pl)) return;
+  if (plan_run(pl, nullptr)) return;
+  std::vector<double> f(n);
+  if ((e = hipMemcpy(p, w.p_d, (size_t)n * w.n_out * 4 * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(f.data(), w.f_d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(volumes, w.v_d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(oks, w.ok_d, (size_t)n, hipMemcpyDeviceToHost)) != hipSuccess) {
+    hip_error(e, "hipMemcpy(outputs)");
+    return;
+  }
+  // own-channel factor at [n_channels * i + c] (phs.cu:207); x_out channel-major (phs.cu:222)
+  const int nc = w.n_channels;
+  std::memset(factors, 0, (size_t)n * nc * sizeof(double));
+  for (int64_t i = 0; i < n; i++) factors[nc * i + w.channel_ids[i]] = f[i];
+  if (x_out) {
+    std::memset(x_out, 0, (size_t)n * nc * w.n_x * sizeof(double));
+    for (int64_t i = 0; i < n; i++)
+      std::memcpy(&x_out[(size_t)w.n_x * n * w.channel_ids[i] + (size_t)w.n_x * i], &x[(size_t)w.n_x * i],
+                  w.n_x * sizeof(double));
+  }
+  w.last_p = w.p_d;
+  ok_status();
+}
+
+void c_whizard_get_momentum_device_pointer(double* p) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  if (p) {
+    uint64_t addr = (uint64_t)(uintptr_t)W().last_p;
+    std::memcpy(p, &addr, sizeof addr);
+  }
+  ok_status();
+}
+
+void c_whizard_reset_generator(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  free_buffers(w);
+  std::fill(w.spec.flv_masses.begin(), w.spec.flv_masses.end(), 0.0);
+  std::fill(w.spec.flv_widths.begin(), w.spec.flv_widths.end(), 0.0);
+  if (w.process >= 0) { drop_process(w.process); w.process = -1; }
+  ok_status();
+}
+
+void c_whizard_show_module(void) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  WhizardModule& w = W();
+  const ProcessSpec& sp = w.spec;
+  printf("n_channels: %d\nn_ext_in: %d\nn_ext_out: %d\nn_ext_tot: %d\n", w.n_channels, w.n_in, w.n_out, w.n_in + w.n_out);
+  if (!w.sized) { printf(" STRUCTURE NOT SET UP!\n"); ok_status(); return; }
+  printf("n_prt: %d\nn_prt_out: %d\nroot_branch: %d\nn_branches: %d\nn_x: %d\n", sp.n_prt(), sp.n_prt_out(), sp.root(),
+         2 * w.n_out - 1, w.n_x);
+  printf("flv_masses:");
+  for (double m : sp.flv_masses) printf(" %lf", m);
+  printf("\nMappings:\n");
+  if (!w.maps_ready) printf(" MAPPINGS NOT SET UP!\n");
+  else
+    for (int c = 0; c < w.n_channels; c++) {
+      printf(" c: %d\n ids:", c);
+      for (int i = 0; i < sp.n_prt_out(); i++) printf(" %d", sp.map_ids[c * sp.n_prt_out() + i]);
+      printf("\n masses:");
+      for (int i = 0; i < sp.n_prt_out(); i++) printf(" %lf", sp.map_masses[c * sp.n_prt_out() + i]);
+      printf("\n widths:");
+      for (int i = 0; i < sp.n_prt_out(); i++) printf(" %lf", sp.map_widths[c * sp.n_prt_out() + i]);
+      printf("\n");
+    }
+  printf("Trees:\n");
+  if (!w.trees_ready) printf(" TREES NOT SET UP!\n");
+  else
+    for (int c = 0; c < w.n_channels; c++) {
+      printf(" channel %d:", c);
+      for (int i = 0; i < sp.n_prt(); i++)
+        if (sp.has_children[c * sp.n_prt() + i])
+          printf(" %d->(%d,%d)", i + 1, sp.d1[c * sp.n_prt() + i], sp.d2[c * sp.n_prt() + i]);
+      printf("\n");
+    }
+  if (w.initialized && w.process >= 0) {
+    ProcessObj* po = find_process(w.process);
+    if (po) {
+      printf("Programs (visit order: branch, phi random, daughters, next):\n");
+      for (size_t c = 0; c < po->cp.channels.size(); c++) {
+        const ChannelDesc& cd = po->cp.channels[c];
+        printf(" channel %zu:", c);
+        for (int i = 0; i < cd.n_nodes; i++) {
+          const NodeCmd& nd = po->cp.nodes[cd.node0 + i];
+          printf(" [%d x%d desc=%d push=%d]", po->cp.node_branch[cd.node0 + i] + 1, nd.x_phi, nd.desc, nd.push);
+        }
+        printf("\n");
+      }
+    }
+  }
+  printf("END OF MODULE\n");
+  fflush(stdout);
+  ok_status();
+}
+
+}  // extern "C"

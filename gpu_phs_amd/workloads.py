@@ -27,6 +27,16 @@ from .process import (MAP_COLLINEAR, MAP_INFRARED, MAP_NO, MAP_SCHANNEL, MAP_STE
                       MAP_TCHANNEL, MAP_UCHANNEL, Process, internal_nodes, make_process,
                       random_tree)
 
+DESCRIPTIONS = {
+    "c0": "2->2 e+e- -> t tbar at sqrt(s)=500 GeV, 1 channel, 2 randoms/event, 1e6 events",
+    "c1": "2->4 double resonance at sqrt(s)=500 GeV (BW M=80.4 G=2.1 + t-channel tree), 3 channels, "
+          "8 randoms/event, 1e7 events",
+    "c2": "2->4 massless multi-jet at sqrt(s)=1 TeV, 120 random-tree channels, 8 randoms/event, 1e7 events",
+    "c3": "batched uneven mix: channels of c0-c2 + 16 2->6 trees (6 x 173 GeV at 3 TeV), "
+          "Dirichlet(0.5) per-channel counts, 5e7 events",
+    "c4": "adjoint on the c1 trees, 1e6 events, loss = sum ok*factor*sum pT^2/s",
+}
+
 SEED_BASE = 0x9E3779B97F4A7C15
 M_E = 0.000511
 

@@ -1,0 +1,155 @@
+/* gpu_phs for MI355X — C ABI of libgpu_phs.so.
+ *
+ * Two groups of entry points:
+ *
+ * 1. The Whizard-facing interface, the same 14 extern "C" symbols with the same signatures,
+ *    argument meaning and index conventions as the reference's whizard_gpu_phs.cpp (each
+ *    declaration cites the function it replaces).  Arguments are passed by pointer, channel
+ *    numbers are 1-based, trees use bincodes, output buffers are caller-owned host arrays.
+ *    The reference returns nothing and reports nothing; this library keeps the void
+ *    signatures and records the status of every call for gpu_phs_last_error().
+ *
+ * 2. Additions with no counterpart in the reference: process handles, device-resident
+ *    batched plans (one launch, or one hipGraph, over many channels and processes),
+ *    per-segment plain launches for comparison, an autotuner, and the adjoint (gradients
+ *    with respect to the randoms and the map masses and widths).  These return a status
+ *    (GPU_PHS_OK = 0, negative on error) or a non-negative handle.
+ *
+ * Layouts (both groups): x [n_events][n_x] event-major, n_x = 3 n_out - 4, randoms consumed
+ * as msq randoms of the internal non-root branches in post-order, then (phi, cos theta) of
+ * every internal branch in pre-order; p [n_events][n_out][4] with final particle j at slot j
+ * (bincode 2^j); factor, volume [n_events]; ok [n_events] one byte (C++ bool / c_bool).
+ */
+#ifndef GPU_PHS_H
+#define GPU_PHS_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ---------------------------------------------------------------------------- */
+#define GPU_PHS_OK 0
+#define GPU_PHS_E_ARG (-1)        /* invalid argument or inconsistent sizes */
+#define GPU_PHS_E_TREE (-2)       /* invalid tree or unsupported map id */
+#define GPU_PHS_E_STATE (-3)      /* call out of order (e.g. generate before init) */
+#define GPU_PHS_E_HIP (-4)        /* HIP runtime error */
+#define GPU_PHS_E_HANDLE (-5)     /* unknown process or plan handle */
+#define GPU_PHS_E_IO (-6)         /* file could not be read or parsed */
+#define GPU_PHS_E_NODEVICE (-7)   /* no GPU visible */
+
+/* Status of the last call on this thread (0 = success) and its message. */
+int gpu_phs_last_error(void);
+const char* gpu_phs_last_error_message(void);
+
+/* ---- 1. Whizard interface (whizard_gpu_phs.cpp) ---------------------------------------- */
+
+/* whizard_gpu_phs.cpp:12-33: sizes of the process; allocates the flavour tables. */
+void c_whizard_set_particle_structure(int* n_channels, int* n_trees, int* n_groves, int* n_x, int* n_in,
+                                      int* n_out);
+/* whizard_gpu_phs.cpp:35-40: masses and widths of the n_in + n_out externals, incoming first. */
+void c_whizard_set_flavors(double* masses, double* widths);
+/* whizard_gpu_phs.cpp:42-55: allocates the per-channel mapping tables. */
+void c_whizard_init_mappings(void);
+/* whizard_gpu_phs.cpp:57-64: map ids, masses, widths of one channel (1-based), each
+ * n_prt_out = 2^n_out - 1 long, index = bincode - 1. */
+void c_whizard_fill_mapping(int* channel, int* map_ids, double* masses, double* widths);
+/* whizard_gpu_phs.cpp:66-77: allocates the per-channel tree tables. */
+void c_whizard_init_tree_structures(void);
+/* whizard_gpu_phs.cpp:79-87: one channel's tree (1-based channel), each n_prt =
+ * 2^(n_in+n_out) - 1 long: daughters as bincodes (0 = none), has_children flags, friends
+ * (bincode of the incoming particle whose axis a branch's polar angle refers to, 0 = none). */
+void c_whizard_fill_tree_structure(int* channel, int* daughters1, int* daughters2, int* has_children,
+                                   int* friends);
+/* whizard_gpu_phs.cpp:89-98: channel of every event of the next batch from cumulative
+ * per-channel end indices.  Unlike the reference, empty channels anywhere in the list are
+ * skipped correctly (the reference only skips leading ones). */
+void c_whizard_init_channel_ids(int* batch_size, int* n_channels, int* channel_limits);
+/* whizard_gpu_phs.cpp:100-104: thread counts of the reference's three kernels.  Here the
+ * first value (rounded to a multiple of 64, at most 256) sets the workgroup size of the fused
+ * kernel; the others are accepted and ignored. */
+void c_whizard_set_threads(int* msq_threads, int* cb_threads, int* ab_threads);
+/* whizard_gpu_phs.cpp:106-108: compiles the trees into kernel programs and uploads them. */
+void c_whizard_init_gpu_phs(void);
+/* whizard_gpu_phs.cpp:110-112: releases everything the calls above allocated. */
+void c_whizard_finalize_gpu_phs(void);
+/* whizard_gpu_phs.cpp:114-122: generates n_events events from host randoms x (event-major,
+ * n_x each) with per-event beam energies E1, E2.  Writes p [n_events][n_out][4], factors at
+ * [n_channels * i + c] (the event's own channel c; the other channels' entries, which the
+ * reference fills by inverse mapping, are set to 0), volumes [n_events], oks [n_events], and
+ * x_out [n_channels][n_events][n_x] (own channel: the input randoms; other channels: 0).
+ * requires_boost_to_lab is read but, as in the reference (phs.cu:1559), the lab boost is
+ * applied whenever E1 != E2. */
+void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_boost_to_lab, int* n_events,
+                                  int* n_channels, int* n_x, double* x, double* factors, double* volumes,
+                                  bool* oks, double* p, double* x_out);
+/* whizard_gpu_phs.cpp:124-126: the reference leaves this empty.  Here it writes, into the
+ * first 8 bytes of p, the device address of the momenta of the last generate call
+ * (layout [n_events][n_out][4]), or 0 if there is none. */
+void c_whizard_get_momentum_device_pointer(double* p);
+/* whizard_gpu_phs.cpp:128-130 (reset_phs, phs.cu:1303-1308): drops the flavour tables and
+ * the device buffers kept between calls. */
+void c_whizard_reset_generator(void);
+/* whizard_gpu_phs.cpp:132-249: prints the module state (sizes, maps, trees, programs). */
+void c_whizard_show_module(void);
+
+/* ---- 2. Additions ---------------------------------------------------------------------- */
+
+/* A process: n_channels trees in the reference format (see c_whizard_fill_tree_structure
+ * and c_whizard_fill_mapping; arrays are [n_channels][n_prt] and [n_channels][n_prt_out]),
+ * flavour masses [n_in + n_out], beams E1, E2 (sqrt(s) = 2 sqrt(E1 E2)).  friends may be
+ * NULL.  Returns a handle >= 0. */
+int gpu_phs_process_create(int n_in, int n_out, int n_channels, const int* d1, const int* d2,
+                           const int* has_children, const int* friends, const int* map_ids,
+                           const double* map_masses, const double* map_widths, const double* flv_masses,
+                           const double* flv_widths, double E1, double E2);
+/* A process from a reference-format .ref file (file_input.cpp:123-210 layout: header,
+ * trees, flavours; events are not read). */
+int gpu_phs_process_load_ref(const char* path, double E1, double E2);
+int gpu_phs_process_destroy(int process);
+/* n_out, n_x, n_channels of a process. */
+int gpu_phs_process_info(int process, int* n_out, int* n_x, int* n_channels);
+/* The process's tree arrays as created (sizes from gpu_phs_process_info); any pointer may
+ * be NULL. */
+int gpu_phs_process_trees(int process, int* d1, int* d2, int* has_children, int* map_ids,
+                          double* map_masses, double* map_widths, double* flv_masses);
+
+/* One contiguous run of events of one channel of one process, with device buffers. */
+typedef struct {
+  int process;            /* handle */
+  int channel;            /* 0-based */
+  int64_t n_events;
+  const double* x;        /* device [n_events][n_x] */
+  double* p;              /* device [n_events][n_out][4] */
+  double* factors;        /* device [n_events] */
+  double* volumes;        /* device [n_events] */
+  uint8_t* oks;           /* device [n_events] */
+} gpu_phs_segment;
+
+/* A plan: the tile table of all segments, uploaded once.  Returns a handle >= 0. */
+int gpu_phs_plan_create(const gpu_phs_segment* segments, int n_segments);
+int gpu_phs_plan_destroy(int plan);
+/* Launch shape: workgroup size (64..256, multiple of 64), events per thread per tile
+ * (1..16), persistent grid (0 = one workgroup per tile).  Changing block or ept rebuilds the
+ * tile table.  order: 0 = segment order, 1 = tiles of the deepest trees first. */
+int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order);
+int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order);
+/* Forward pass of every segment in one kernel launch on `stream` (hipStream_t, 0 = null). */
+int gpu_phs_plan_run(int plan, void* stream);
+/* Forward pass with one launch per segment (the plain per-channel form). */
+int gpu_phs_plan_run_plain(int plan, void* stream);
+/* Capture the single launch into a hipGraph (once) and replay it on `stream`. */
+int gpu_phs_plan_run_graph(int plan, void* stream);
+/* Try launch shapes, time each with hipEvents (median of `reps`), keep the fastest.  Writes
+ * a one-line summary to report (may be NULL).  Outputs are overwritten by the trial runs. */
+int gpu_phs_plan_autotune(int plan, void* stream, int reps, char* report, int report_len);
+/* Number of tiles and of workgroups of the single launch. */
+int gpu_phs_plan_info(int plan, int* n_tiles, int* n_segments, int* n_banks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPU_PHS_H */

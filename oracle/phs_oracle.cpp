@@ -248,6 +248,7 @@ template <class T> struct Ev {
   std::vector<T> p2;        // msq per branch index (prt[].p2 of the reference)
   std::vector<T> pd;        // p_decay per branch index
   std::vector<T> mom;       // 4 per branch index (prt[].p)
+  std::vector<double> scale;  // 4 per branch index: forward-error scale of mom
 };
 
 // phs_cpu.cpp:174-234 (set_msq_cpu): post-order recursion, msq randoms consumed in
@@ -300,13 +301,20 @@ void set_msq(const Proc& P, const MapPar<T>& M, int c, int b, double m_tot, cons
 
 // phs_cpu.cpp:283-345 (set_angles_cpu): pre-order recursion; phi then cos-theta randoms
 // per internal branch; L_new = L0 * B_z(bg) R_z(phi) R_y(theta).
+//
+// Alongside, the componentwise forward-error scale of every momentum component is carried:
+// A = |L1_root| |L1_a1| ... |L1_parent| composed in absolute values, and
+// scale^mu = A[mu][0] |E| + A[mu][3] |p| bounds the magnitude of the terms that form p^mu
+// (E.scale, per branch).  Two correct double-precision evaluations of the chain can differ
+// by a small multiple of eps * scale, however small p^mu itself is.
 template <class T>
 void set_angles(const Proc& P, const MapPar<T>& M, int c, int b, const T* x, int* idx_x, T* factor,
-                Ev<T>& E, const T L0[4][4]) {
+                Ev<T>& E, const T L0[4][4], const double A0[4][4]) {
   T p = E.pd[b];
   T m = sqrt(E.p2[b]);
   T En = sqrt(E.p2[b] + p * p);
   for (int i = 0; i < 4; i++) E.mom[4 * b + i] = L0[i][0] * En + L0[i][3] * p;
+  for (int i = 0; i < 4; i++) E.scale[4 * b + i] = A0[i][0] * std::fabs(val(En)) + A0[i][3] * std::fabs(val(p));
   if (P.hc[c * P.n_prt + b]) {
     int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
     T bg = val(m) > 0 ? p / m : T(0);
@@ -326,13 +334,18 @@ void set_angles(const Proc& P, const MapPar<T>& M, int c, int b, const T* x, int
     L1[2][0] = 0;     L1[2][1] = ct * sp;     L1[2][2] = cp; L1[2][3] = st * sp;
     L1[3][0] = bg;    L1[3][1] = -gamma * st; L1[3][2] = 0;  L1[3][3] = gamma * ct;
     T Ln[4][4];
+    double An[4][4];
     for (int i = 0; i < 4; i++)
       for (int j = 0; j < 4; j++) {
         Ln[i][j] = 0;
-        for (int k = 0; k < 4; k++) Ln[i][j] = Ln[i][j] + L0[i][k] * L1[k][j];
+        An[i][j] = 0;
+        for (int k = 0; k < 4; k++) {
+          Ln[i][j] = Ln[i][j] + L0[i][k] * L1[k][j];
+          An[i][j] += A0[i][k] * std::fabs(val(L1[k][j]));
+        }
       }
-    set_angles(P, M, c, k1, x, idx_x, factor, E, Ln);
-    set_angles(P, M, c, k2, x, idx_x, factor, E, Ln);
+    set_angles(P, M, c, k1, x, idx_x, factor, E, Ln, An);
+    set_angles(P, M, c, k2, x, idx_x, factor, E, Ln, An);
   }
 }
 
@@ -362,9 +375,9 @@ bool build_proc(Proc& P, int n_in, int n_out, int n_channels, const int* d1, con
 // then the lab-frame boost of the final leaves, phs.cu:1286-1301, when E1 != E2.
 template <class T>
 bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg_lab, const T* x,
-               Ev<T>& E, T* factor, T* volume, T* p_out, double* msq_out) {
+               Ev<T>& E, T* factor, T* volume, T* p_out, double* msq_out, double* scale_out) {
   int n = P.n_prt;
-  E.p2.assign(n, T(0)); E.pd.assign(n, T(0)); E.mom.assign(4 * n, T(0));
+  E.p2.assign(n, T(0)); E.pd.assign(n, T(0)); E.mom.assign(4 * n, T(0)); E.scale.assign(4 * n, 0.0);
   // phs_cpu.cpp:541-547 (init_msq_cpu): leaf msq = flavour mass squared.
   for (int i = 0, p = 1; i < P.n_out; i++, p *= 2) E.p2[p - 1] = P.flv[P.n_in + i] * P.flv[P.n_in + i];
   bool ok = true;
@@ -375,16 +388,26 @@ bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg
     for (int i = 0; i < P.n_prt_out; i++) msq_out[i] = val(E.p2[i]);
   if (ok) {
     T L0[4][4];
-    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) L0[i][j] = (i == j) ? 1.0 : 0.0;
-    set_angles(P, M, c, P.root, x, &id_x, factor, E, L0);
+    double A0[4][4];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) { L0[i][j] = (i == j) ? 1.0 : 0.0; A0[i][j] = (i == j) ? 1.0 : 0.0; }
+    set_angles(P, M, c, P.root, x, &id_x, factor, E, L0, A0);
     for (int j = 0, b = 1; j < P.n_out; j++, b *= 2)
-      for (int mu = 0; mu < 4; mu++) p_out[4 * j + mu] = E.mom[4 * (b - 1) + mu];
+      for (int mu = 0; mu < 4; mu++) {
+        p_out[4 * j + mu] = E.mom[4 * (b - 1) + mu];
+        if (scale_out) scale_out[4 * j + mu] = E.scale[4 * (b - 1) + mu];
+      }
     if (bg_lab != 0) {
       double g = std::sqrt(1 + bg_lab * bg_lab);
       for (int j = 0; j < P.n_out; j++) {
         T p0 = g * p_out[4 * j] + bg_lab * p_out[4 * j + 3];
         T p3 = bg_lab * p_out[4 * j] + g * p_out[4 * j + 3];
         p_out[4 * j] = p0; p_out[4 * j + 3] = p3;
+        if (scale_out) {
+          double s0 = g * scale_out[4 * j] + std::fabs(bg_lab) * scale_out[4 * j + 3];
+          double s3 = std::fabs(bg_lab) * scale_out[4 * j] + g * scale_out[4 * j + 3];
+          scale_out[4 * j] = s0; scale_out[4 * j + 3] = s3;
+        }
       }
     }
   }
@@ -440,13 +463,14 @@ void oracle_map_ct(int type, double x, const double* b, double* ct, double* st, 
 //   E1, E2: per-event beam energies, or NULL for both beams at sqrts / 2.
 //   channels [n_events] 0-based; x [n_events][n_x] event-major.
 //   Outputs: p [n_events][n_out][4], factors, volumes [n_events], oks [n_events] (0/1),
-//   msq [n_events][n_prt_out] (may be NULL).  Non-ok events keep p = 0 and whatever factor
+//   msq [n_events][n_prt_out] (may be NULL), scale [n_events][n_out][4] the forward-error
+//   scale of every momentum component (may be NULL; see set_angles).  Non-ok events keep p = 0 and whatever factor
 //   and volume the early return left, as the reference does.  Returns 0 on success.
 int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
                    const int* map_ids, const double* map_m, const double* map_w, const double* flv,
                    double sqrts, const double* E1, const double* E2, long n_events, const int* channels,
                    int n_x, const double* x, double* p, double* factors, double* volumes,
-                   unsigned char* oks, double* msq) {
+                   unsigned char* oks, double* msq, double* scale) {
   Proc P;
   if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
   if (n_x < 3 * n_out - 4) return 2;
@@ -472,8 +496,10 @@ int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int
     }
     double f = 0, v = 0;
     std::memset(&p[4 * n_out * i], 0, 4 * n_out * sizeof(double));
+    if (scale) std::memset(&scale[4 * n_out * i], 0, 4 * n_out * sizeof(double));
     bool ok = one_event<double>(P, M, c, s_ev, bg_lab, &x[(size_t)n_x * i], E, &f, &v, &p[4 * n_out * i],
-                                msq ? &msq[(size_t)P.n_prt_out * i] : nullptr);
+                                msq ? &msq[(size_t)P.n_prt_out * i] : nullptr,
+                                scale ? &scale[4 * n_out * i] : nullptr);
     factors[i] = f; volumes[i] = v; oks[i] = ok ? 1 : 0;
   }
   return 0;
@@ -514,7 +540,7 @@ int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, cons
     }
     for (auto& q : pd) q = Dual(0.0);
     Dual f, v;
-    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, xd.data(), E, &f, &v, pd.data(), nullptr);
+    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr);
     oks[i] = ok ? 1 : 0;
     factors[i] = ok ? f.v : 0;
     for (int j = 0; j < 4 * n_out; j++) {

@@ -1,0 +1,92 @@
+"""Parity criteria between the HIP path and the oracle (same randoms, same trees).
+
+Tolerance: 1e-12 relative (FP64).
+
+* ok flags: identical.  A disagreement is accepted only for an event within 1e-12 of a
+  kinematic threshold of the oracle (``threshold_margin`` below), and such events are
+  counted (``ok_mismatch_threshold``).
+* factor and volume of ok events: |d| <= 1e-12 |ref|.
+* momenta of ok events, component by component:
+  strict      |dp^mu| <= 1e-12 max(|p^mu|, E_j)             (energy-scaled relative)
+  conditioned |dp^mu| <= 1e-12 max(|p^mu|, E_j, scale^mu)
+  where scale^mu = (|L1_root| ... |L1_parent| |v_j|)^mu is the componentwise forward-error
+  scale of the boost chain that the oracle reports: the magnitude of the terms that are
+  summed into p^mu.  Every ok event must meet the conditioned bound; events that meet it but
+  not the strict one are ill-conditioned points (a massless daughter emitted against a large
+  boost, whose energy is a difference of terms up to 1e5 times larger), counted and reported
+  as ``ill_conditioned``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TOL = 1e-12
+
+
+def threshold_margin(proc, msq, channels, sqrts):
+    """Smallest relative distance of each event to one of the kinematic cuts of the oracle
+    (phs_cpu.cpp:225-226): lambda / msq^2, (m - m1 - m2) / m and (m_max - m) / m_max over the
+    internal branches, from the oracle's msq of every branch."""
+    n = msq.shape[0]
+    out = np.full(n, np.inf)
+    n_prt, root = proc.n_prt, proc.n_prt_out - 1
+    mtot_all = np.zeros(proc.n_channels)
+    import oracle
+    ms = oracle.mass_sums(proc)
+    for c in range(proc.n_channels):
+        sel = np.nonzero(channels == c)[0]
+        if sel.size == 0:
+            continue
+        m2 = msq[sel]
+        for b in range(proc.n_prt_out):
+            if not proc.has_children[c, b]:
+                continue
+            k1, k2 = proc.d1[c, b] - 1, proc.d2[c, b] - 1
+            b1, b2 = m2[:, k1], m2[:, k2]
+            a = np.full_like(b1, sqrts * sqrts) if b == root else m2[:, b]
+            lda = (a - b1 - b2) ** 2 - 4 * b1 * b2
+            with np.errstate(all="ignore"):
+                m, m1_, m2_ = np.sqrt(a), np.sqrt(b1), np.sqrt(b2)
+                mmax = sqrts if b == root else sqrts - ms[c, root] + ms[c, b]
+                r = np.minimum.reduce([np.abs(lda) / np.maximum(a * a, 1e-300),
+                                       np.abs(m - m1_ - m2_) / np.maximum(m, 1e-300),
+                                       np.abs(mmax - m) / mmax])
+            out[sel] = np.minimum(out[sel], np.nan_to_num(r, nan=0.0))
+    return out
+
+
+def compare(gpu, ref, tol=TOL, near_threshold=None):
+    ok_g = np.asarray(gpu["ok"]).astype(bool)
+    ok_r = np.asarray(ref["ok"]).astype(bool)
+    mism = ok_g != ok_r
+    both = ok_g & ok_r
+    pg, pr = np.asarray(gpu["p"])[both], np.asarray(ref["p"])[both]
+    strict_scale = np.maximum(np.abs(pr), np.abs(pr[..., :1]))
+    d = np.abs(pg - pr)
+    dp = d / np.where(strict_scale > 0, strict_scale, 1.0)
+    if "scale" in ref:
+        cond_scale = np.maximum(strict_scale, np.asarray(ref["scale"])[both])
+    else:
+        cond_scale = strict_scale
+    dpc = d / np.where(cond_scale > 0, cond_scale, 1.0)
+    fg, fr = np.asarray(gpu["factor"])[both], np.asarray(ref["factor"])[both]
+    df = np.abs(fg - fr) / np.where(np.abs(fr) > 0, np.abs(fr), 1.0)
+    vg, vr = np.asarray(gpu["volume"])[both], np.asarray(ref["volume"])[both]
+    dv = np.abs(vg - vr) / np.where(np.abs(vr) > 0, np.abs(vr), 1.0)
+    flat = lambda a: a.reshape(a.shape[0], -1)
+    strict_bad = (flat(dp) > tol).any(axis=1)
+    cond_bad = (flat(dpc) > tol).any(axis=1) | (df > tol) | (dv > tol) | ~np.isfinite(fg) | \
+        ~np.isfinite(flat(pg)).all(axis=1)
+    n_thr = int((mism & near_threshold).sum()) if near_threshold is not None else 0
+    return dict(n=len(ok_g), n_ok=int(ok_r.sum()), ok_mismatch=int(mism.sum()),
+                ok_mismatch_threshold=n_thr, ok_mismatch_unexplained=int(mism.sum()) - n_thr,
+                max_dp=float(dp.max()) if dp.size else 0.0,
+                max_dp_conditioned=float(dpc.max()) if dpc.size else 0.0,
+                max_df=float(df.max()) if df.size else 0.0, max_dv=float(dv.max()) if dv.size else 0.0,
+                ill_conditioned=int((strict_bad & ~cond_bad).sum()), n_bad=int(cond_bad.sum()),
+                bitwise_p=float((pg == pr).all(axis=(1, 2)).mean()) if pg.size else 1.0)
+
+
+def assert_parity(r):
+    assert r["ok_mismatch_unexplained"] == 0, r
+    assert r["n_bad"] == 0, r

@@ -1,0 +1,153 @@
+"""GPU parity of the forward kernel against the oracle and the reference's golden vectors."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from gpu_phs_amd import process as P
+from gpu_phs_amd import workloads as W
+from parity import assert_parity, compare, threshold_margin
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def load_golden(path):
+    z = np.load(path)
+    proc = P.Process(int(z["n_in"]), int(z["n_out"]), z["d1"], z["d2"], z["has_children"], z["map_ids"],
+                     z["map_masses"], z["map_widths"], z["flv_masses"], z["flv_widths"], float(z["sqrts"]))
+    ref = dict(p=z["p"], factor=z["factor"], volume=z["volume"], ok=z["ok"], msq=z["msq"])
+    return proc, z["x"], z["channels"], ref
+
+
+def run_gpu(proc, x, channels, **cfg):
+    """Events grouped by channel into segments (one plan), results in the caller's order."""
+    torch = _torch()
+    import gpu_phs_amd as G
+    ph = G.ProcessHandle(proc)
+    order = np.argsort(channels, kind="stable")
+    xs = torch.from_numpy(np.ascontiguousarray(x[order])).cuda()
+    chs = channels[order]
+    segs, bufs = [], []
+    start = 0
+    for c in range(proc.n_channels):
+        n = int((chs == c).sum())
+        if n == 0:
+            continue
+        buf = G.EventBuffers.allocate(xs[start:start + n], proc.n_out)
+        segs.append((ph, c, buf))
+        bufs.append(buf)
+        start += n
+    plan = G.Plan(segs, **cfg)
+    plan.run()
+    torch.cuda.synchronize()
+    cat = lambda k: torch.cat([getattr(b, k) for b in bufs]).cpu().numpy()
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    out = dict(p=cat("p")[inv], factor=cat("factor")[inv], volume=cat("volume")[inv],
+               ok=cat("ok")[inv].astype(bool))
+    return out, plan, bufs
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "golden_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[7:-4])
+def test_forward_matches_golden(path):
+    proc, x, ch, ref = load_golden(path)
+    out, _, _ = run_gpu(proc, x, ch)
+    ref["scale"] = oracle.forward(proc, x, ch)["scale"]
+    near = threshold_margin(proc, ref["msq"], ch, proc.sqrts) < 1e-12
+    r = compare(out, ref, near_threshold=near)
+    print(os.path.basename(path), r)
+    assert_parity(r)
+
+
+@pytest.mark.parametrize("name", ["c0", "c1", "c2", "2to6"])
+@pytest.mark.parametrize("cfg", [dict(block=256, ept=1), dict(block=64, ept=3), dict(block=128, ept=2, grid=7)])
+def test_forward_matches_oracle(name, cfg):
+    proc = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6}[name]()
+    rng = np.random.default_rng(7)
+    n = 30011
+    ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
+    x = rng.random((n, proc.n_x))
+    out, _, _ = run_gpu(proc, x, ch, **cfg)
+    ref = oracle.forward(proc, x, ch, want_msq=True)
+    near = threshold_margin(proc, ref["msq"], ch, proc.sqrts) < 1e-12
+    r = compare(out, ref, near_threshold=near)
+    print(name, cfg, r)
+    assert_parity(r)
+
+
+def test_plain_batched_graph_identical():
+    torch = _torch()
+    import gpu_phs_amd as G
+    wl = W.workload("c3", scale=0.002)
+    handles = [G.ProcessHandle(p) for p in wl.processes]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(wl.seed & 0x7FFFFFFF)
+    segs, bufs = [], []
+    for s in wl.segments:
+        ph = handles[s.process]
+        x = torch.rand((s.n_events, ph.n_x), dtype=torch.float64, device="cuda", generator=gen)
+        b = G.EventBuffers.allocate(x, ph.n_out)
+        segs.append((ph, s.channel, b))
+        bufs.append(b)
+    plan = G.Plan(segs)
+    results = []
+    for mode in ("run", "run_plain", "run_graph", "run_graph"):
+        for b in bufs:
+            b.p.fill_(0)
+            b.factor.fill_(0)
+        getattr(plan, mode)()
+        torch.cuda.synchronize()
+        results.append([(b.p.clone(), b.factor.clone(), b.ok.clone()) for b in bufs])
+    for r in results[1:]:
+        for (p0, f0, o0), (p1, f1, o1) in zip(results[0], r):
+            assert torch.equal(o0, o1)
+            okm = o0.bool()
+            assert torch.equal(p0[okm], p1[okm]) and torch.equal(f0[okm], f1[okm])
+    # and the batched result equals the oracle per segment
+    for s, b in zip(wl.segments, bufs):
+        proc = wl.processes[s.process]
+        ch = np.full(s.n_events, s.channel, np.int32)
+        ref = oracle.forward(proc, b.x.cpu().numpy(), ch)
+        out = dict(p=b.p.cpu().numpy(), factor=b.factor.cpu().numpy(), volume=b.volume.cpu().numpy(),
+                   ok=b.ok.cpu().numpy().astype(bool))
+        r = compare(out, ref)
+        assert_parity(r)
+
+
+def test_ragged_and_empty_segments():
+    torch = _torch()
+    import gpu_phs_amd as G
+    proc = W.process_c1()
+    ph = G.ProcessHandle(proc)
+    rng = np.random.default_rng(3)
+    sizes = [0, 1, 63, 64, 65, 255, 256, 257, 1000]
+    segs, bufs, xs = [], [], []
+    for i, n in enumerate(sizes):
+        x = rng.random((n, proc.n_x))
+        xs.append(x)
+        b = G.EventBuffers.allocate(torch.from_numpy(x).cuda(), proc.n_out)
+        segs.append((ph, i % 3, b))
+        bufs.append(b)
+    for cfg in (dict(block=64, ept=1), dict(block=256, ept=4), dict(block=192, ept=2, grid=3)):
+        plan = G.Plan(segs, **cfg)
+        plan.run()
+        torch.cuda.synchronize()
+        for i, (n, b) in enumerate(zip(sizes, bufs)):
+            if n == 0:
+                continue
+            ref = oracle.forward(proc, xs[i], np.full(n, i % 3, np.int32))
+            out = dict(p=b.p.cpu().numpy(), factor=b.factor.cpu().numpy(), volume=b.volume.cpu().numpy(),
+                       ok=b.ok.cpu().numpy().astype(bool))
+            r = compare(out, ref)
+            assert_parity(r)
