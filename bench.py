@@ -57,11 +57,15 @@ def time_steps(fn, steps, warmup, torch):
     return wall, start.elapsed_time(end) / 1e3
 
 
-def run_workload(name, steps, warmup, torch, G, W, tune=True):
+def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None):
     wl = W.workload(name)
     handles, segs = make_inputs(wl, torch, G)
     plan = G.Plan(segs)
-    tuned = plan.autotune(reps=3) if tune else ""
+    if fixed:
+        plan.set_config(**fixed)
+        tuned = ""
+    else:
+        tuned = plan.autotune(reps=3) if tune else ""
     torch.cuda.synchronize()
     wall, ev_s = time_steps(plan.run_graph, steps, warmup, torch)
     plain_wall, plain_ev = time_steps(plan.run_plain, max(1, steps // 2), 1, torch)
@@ -109,6 +113,11 @@ def main():
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="",
+                    help="fixed launch config block,ept,grid,order,waves_per_eu,lds_bank instead of autotuning")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus}).",
@@ -121,8 +130,22 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     G.lib()
+    if args.calibrate_copy > 0:
+        n = int(args.calibrate_copy * 2 ** 30) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "fetch_expected_kib": n * 8 / 1024,
+                          "write_expected_kib": n * 8 / 1024}))
+        return
+    fixed = None
+    if args.config:
+        v = [int(t) for t in args.config.split(",")]
+        fixed = dict(zip(("block", "ept", "grid", "order", "waves_per_eu", "lds_bank"), v))
     log(f"workload {args.workload}: setup")
-    main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W)
+    main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W, fixed=fixed)
     log(f"workload {args.workload}: {main_res['batched_ev_s']:.4g} ev/s batched, "
         f"{main_res['plain_ev_s']:.4g} ev/s plain")
     kernel_s = main_res["batched_kernel_ms"] / 1e3

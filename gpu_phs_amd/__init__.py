@@ -53,14 +53,23 @@ _SIGS = {
     "gpu_phs_process_trees": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 7),
     "gpu_phs_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gpu_phs_plan_destroy": (ctypes.c_int, [ctypes.c_int]),
-    "gpu_phs_plan_set_config": (ctypes.c_int, [ctypes.c_int] * 5),
-    "gpu_phs_plan_get_config": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 4),
+    "gpu_phs_plan_set_config": (ctypes.c_int, [ctypes.c_int] * 7),
+    "gpu_phs_plan_get_config": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 6),
     "gpu_phs_plan_run": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_run_plain": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_run_graph": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "gpu_phs_plan_info": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3),
+    "gpu_phs_plan_adjoint": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_phs_plan_n_params": (ctypes.c_int, [ctypes.c_int]),
+    "gpu_phs_plan_param_map": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int]),
 }
+
+
+class _AdjSegment(ctypes.Structure):
+    _fields_ = [("g_p", ctypes.c_void_p), ("g_f", ctypes.c_void_p), ("g_x", ctypes.c_void_p),
+                ("loss", ctypes.c_void_p)]
 
 
 def lib():
@@ -160,7 +169,7 @@ class Plan:
     alive and keep their storage while the plan is used.
     """
 
-    def __init__(self, segments, block=256, ept=1, grid=0, order=0):
+    def __init__(self, segments, block=256, ept=1, grid=0, order=0, waves_per_eu=0, lds_bank=0):
         arr = (_Segment * len(segments))()
         self._keep = []
         for i, (ph, ch, buf) in enumerate(segments):
@@ -173,15 +182,17 @@ class Plan:
                               buf.factor.data_ptr(), buf.volume.data_ptr(), buf.ok.data_ptr())
             self._keep.append((ph, buf))
         self.handle = _check(lib().gpu_phs_plan_create(arr, len(segments)), "plan_create")
-        self.set_config(block, ept, grid, order)
+        self.set_config(block, ept, grid, order, waves_per_eu, lds_bank)
 
-    def set_config(self, block=256, ept=1, grid=0, order=0):
-        _check(lib().gpu_phs_plan_set_config(self.handle, block, ept, grid, order), "plan_set_config")
+    def set_config(self, block=256, ept=1, grid=0, order=0, waves_per_eu=0, lds_bank=0):
+        _check(lib().gpu_phs_plan_set_config(self.handle, block, ept, grid, order, waves_per_eu, lds_bank),
+               "plan_set_config")
 
     def config(self):
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(6)]
         _check(lib().gpu_phs_plan_get_config(self.handle, *[ctypes.byref(q) for q in v]), "plan_get_config")
-        return dict(block=v[0].value, ept=v[1].value, grid=v[2].value, order=v[3].value)
+        return dict(block=v[0].value, ept=v[1].value, grid=v[2].value, order=v[3].value,
+                    waves_per_eu=v[4].value, lds_bank=v[5].value)
 
     def info(self):
         v = [ctypes.c_int() for _ in range(3)]
@@ -196,6 +207,40 @@ class Plan:
 
     def run_graph(self, stream=None):
         _check(lib().gpu_phs_plan_run_graph(self.handle, _stream_ptr(stream)), "plan_run_graph")
+
+    def n_params(self):
+        return _check(lib().gpu_phs_plan_n_params(self.handle), "plan_n_params")
+
+    def param_map(self):
+        """(process handle, channel, bincode) of every (M, Gamma) parameter slot."""
+        n = self.n_params() // 2
+        pr, ch, bc = (np.zeros(n, np.int32) for _ in range(3))
+        _check(lib().gpu_phs_plan_param_map(self.handle, _ptr(pr), _ptr(ch), _ptr(bc), n), "plan_param_map")
+        return pr, ch, bc
+
+    def adjoint(self, g_x, g_p=None, g_f=None, loss=None, loss_mode=0, write_forward=True, gparam=None,
+                stream=None):
+        """Reverse-mode pass over every segment: fills the per-segment tensors g_x (and loss), and
+        returns gparam [n_params] (d/dM, d/dGamma interleaved per slot of param_map())."""
+        import torch
+        n = len(self._keep)
+        arr = (_AdjSegment * n)()
+        keep = []
+        for i in range(n):
+            def ptr(t):
+                if t is None or t[i] is None:
+                    return None
+                if not t[i].is_contiguous():
+                    raise ValueError("adjoint tensors must be contiguous")
+                keep.append(t[i])
+                return t[i].data_ptr()
+            arr[i] = _AdjSegment(ptr(g_p), ptr(g_f), ptr(g_x), ptr(loss))
+        if gparam is None:
+            dev = self._keep[0][1].x.device
+            gparam = torch.empty(max(self.n_params(), 1), dtype=torch.float64, device=dev)
+        _check(lib().gpu_phs_plan_adjoint(self.handle, arr, loss_mode, 1 if write_forward else 0,
+                                          ctypes.c_void_p(gparam.data_ptr()), _stream_ptr(stream)), "plan_adjoint")
+        return gparam
 
     def autotune(self, reps=3, stream=None):
         buf = ctypes.create_string_buffer(8192)

@@ -1,0 +1,438 @@
+// Reverse-mode derivatives of one event: d(momenta, factor)/d(randoms, M, Gamma).
+//
+// No counterpart in the reference.  The forward pass is the one of phs_forward.hip (the
+// reference's phs_cpu.cpp algorithm, visit order of the compiled program); it records a
+// small tape per branch (the boost matrix after the branch and the branch's incoming msq, m,
+// p_decay).  The reverse sweep visits the branches backwards and mirrors the forward's save
+// bank: the adjoint of a deferred subtree waits in a stash until its parent is reached.
+// Local quantities (maps, lambda, angles) are recomputed from the tape instead of stored.
+//
+// Everything here is __host__ __device__ so the same code can be checked on the CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "phs_types.h"
+
+namespace gphs {
+
+#define GPHS_HD __host__ __device__ __forceinline__
+
+constexpr double kTwoPi = 6.28318530717958647693;
+constexpr double kPi = 3.14159265358979323846;
+
+// msq map of an internal daughter with the intermediates its derivative needs.
+struct MapEval {
+  double msq, f;
+  double t;     // BW: tan z; EXP/TU: exp(u a1); STEP_E: tmp; STEP_H: 1/(a0 - x)
+  double aux;   // TU: sign (+1 lower half, -1 upper); STEP_E: exp(-x a0 / a2)
+  bool in;      // BW: z inside (-pi/2, pi/2)
+};
+
+GPHS_HD MapEval msq_eval(const ChildCmd& c, double x, double s) {
+  MapEval r{0, 0, 0, 0, true};
+  switch (c.kind) {
+    case MK_LINEAR:
+      r.msq = (1 - x) * c.msq_min + x * c.msq_max;
+      r.f = c.f_lin;
+      break;
+    case MK_BREIT_WIGNER: {
+      double z = (1 - x) * c.a0 + x * c.a1;
+      r.t = tan(z);
+      r.in = (-kPi / 2 < z) && (z < kPi / 2);
+      r.msq = r.in ? c.m * (c.m + c.w * r.t) : 0.0;
+      r.f = r.in ? c.a2 * (1 + r.t * r.t) : 0.0;
+      break;
+    }
+    case MK_EXP: {
+      r.t = exp(x * c.a1);
+      double m1 = c.a0 * r.t;
+      r.msq = m1 - c.a0 + c.msq_min;
+      r.f = c.a2 * m1;
+      break;
+    }
+    case MK_TU: {
+      bool lo = x < 0.5;
+      r.aux = lo ? 1.0 : -1.0;
+      r.t = exp((lo ? x : (1 - x)) * c.a1);
+      double m1 = c.a0 * r.t;
+      r.msq = lo ? (m1 - c.a0 + c.msq_min) : (-(m1 - c.a0) + c.msq_max);
+      r.f = c.a2 * m1;
+      break;
+    }
+    case MK_STEP_H: {
+      double z = c.a1 / (c.a0 - x) - c.a1 / c.a0 + c.a2 * x;
+      r.t = 1 / (c.a0 - x);
+      r.msq = z * c.msq_max + (1 - z) * c.msq_min;
+      r.f = (c.a1 / ((c.a0 - x) * (c.a0 - x)) + c.a2) * (c.msq_max - c.msq_min) / s;
+      break;
+    }
+    default: {  // MK_STEP_E
+      r.aux = exp(-x * c.a0 / c.a2);
+      r.t = r.aux * (1 + c.a1);
+      double z = -c.a2 * log(r.t - c.a1);
+      r.msq = z * c.msq_max + (1 - z) * c.msq_min;
+      r.f = c.a0 / (1 - c.a1 / r.t) * (c.msq_max - c.msq_min) / s;
+      break;
+    }
+  }
+  return r;
+}
+
+// Adjoint of msq_eval: given gmsq, gf, accumulate d/dx and d/d(M, Gamma) of the map.
+GPHS_HD void msq_eval_adj(const ChildCmd& c, double x, double s, const MapEval& r, double gmsq, double gf, double& gx,
+                          double& gM, double& gW) {
+  double ga0 = 0, ga1 = 0, ga2 = 0, gm_direct = 0, gw_direct = 0;
+  gx = 0;
+  switch (c.kind) {
+    case MK_LINEAR:
+      gx = gmsq * (c.msq_max - c.msq_min);
+      break;
+    case MK_BREIT_WIGNER:
+      if (r.in) {
+        const double sec2 = 1 + r.t * r.t;
+        const double gt = gmsq * c.m * c.w + gf * c.a2 * 2 * r.t;
+        const double gz = gt * sec2;
+        gx = gz * (c.a1 - c.a0);
+        ga0 = gz * (1 - x);
+        ga1 = gz * x;
+        ga2 = gf * sec2;
+        gm_direct = gmsq * (2 * c.m + c.w * r.t);
+        gw_direct = gmsq * c.m * r.t;
+      }
+      break;
+    case MK_EXP: {
+      const double m1 = c.a0 * r.t;
+      const double gm1 = gmsq + gf * c.a2;
+      ga0 = gm1 * r.t - gmsq;
+      ga1 = gm1 * m1 * x;
+      gx = gm1 * m1 * c.a1;
+      ga2 = gf * m1;
+      break;
+    }
+    case MK_TU: {
+      const double m1 = c.a0 * r.t;
+      const double sg = r.aux;                 // msq = sg * m1 - sg * a0 + const
+      const double u = sg > 0 ? x : 1 - x;
+      const double gm1 = sg * gmsq + gf * c.a2;
+      ga0 = gm1 * r.t - sg * gmsq;
+      ga1 = gm1 * m1 * u;
+      gx = sg * gm1 * m1 * c.a1;
+      ga2 = gf * m1;
+      break;
+    }
+    case MK_STEP_H: {
+      const double d = c.msq_max - c.msq_min, cs = d / s, rr = r.t;
+      const double gz = gmsq * d;
+      gx = gz * (c.a1 * rr * rr + c.a2) + gf * 2 * c.a1 * rr * rr * rr * cs;
+      ga0 = gz * (-c.a1 * rr * rr + c.a1 / (c.a0 * c.a0)) - gf * 2 * c.a1 * rr * rr * rr * cs;
+      ga1 = gz * (rr - 1 / c.a0) + gf * rr * rr * cs;
+      ga2 = gz * x + gf * cs;
+      break;
+    }
+    default: {  // MK_STEP_E: tmp = e (1 + a1), v = tmp - a1, z = -a2 log v, f = cs a0 tmp / v
+      const double d = c.msq_max - c.msq_min, cs = d / s;
+      const double tmp = r.t, e = r.aux, v = tmp - c.a1;
+      const double gz = gmsq * d;
+      const double gv = gz * (-c.a2 / v);
+      const double gtmp = gv + gf * (-cs * c.a0 * c.a1 / (v * v));
+      ga1 = -gv + gf * (cs * c.a0 * tmp / (v * v)) + gtmp * e;
+      ga0 = gf * cs * tmp / v + gtmp * tmp * (-x / c.a2);
+      ga2 = gz * (-log(v)) + gtmp * tmp * x * c.a0 / (c.a2 * c.a2);
+      gx = gtmp * tmp * (-c.a0 / c.a2);
+      break;
+    }
+  }
+  gM += gm_direct + ga0 * c.da_dm[0] + ga1 * c.da_dm[1] + ga2 * c.da_dm[2];
+  gW += gw_direct + ga0 * c.da_dw[0] + ga1 * c.da_dw[1] + ga2 * c.da_dw[2];
+}
+
+struct CtEval {
+  double ct, st, f, tmp, e;
+  bool in, lo;
+};
+
+GPHS_HD CtEval ct_eval(int kind, double x, double b0, double b1) {
+  CtEval r{};
+  if (kind == CK_FLAT) {
+    r.tmp = 2 * (1 - x);
+    r.ct = 1 - r.tmp;
+    r.st = sqrt(r.tmp * (2 - r.tmp));
+    r.f = 1;
+    r.in = true;
+  } else {
+    r.lo = x < 0.5;
+    r.e = exp((r.lo ? 2 * x : 2 * (1 - x)) * b1);
+    r.tmp = b0 * r.e;
+    double c = r.lo ? (r.tmp - b0 - 1) : (-(r.tmp - b0) + 1);
+    r.in = c >= -1 && c <= 1;
+    double sq = sqrt(1 - c * c);
+    r.ct = r.in ? c : 1.0;
+    r.st = r.in ? sq : 0.0;
+    r.f = r.in ? r.tmp * b1 : 0.0;
+  }
+  return r;
+}
+
+// Adjoint of ct_eval: d/dx and d/d(b0, b1).
+GPHS_HD void ct_eval_adj(int kind, double x, double b0, double b1, const CtEval& r, double gct, double gst, double gf,
+                         double& gx, double& gb0, double& gb1) {
+  gx = 0; gb0 = 0; gb1 = 0;
+  if (kind == CK_FLAT) {
+    // ct = 2x - 1, st = sqrt(tmp (2 - tmp)), tmp = 2 (1 - x)
+    gx = 2 * gct;
+    if (r.st > 0) gx += gst * (-2 * (1 - r.tmp) / r.st);
+    return;
+  }
+  if (!r.in) return;
+  const double sg = r.lo ? 1.0 : -1.0;     // d ct / d tmp
+  const double u = r.lo ? x : 1 - x;
+  double gc = gct;
+  if (r.st > 0) gc += gst * (-r.ct / r.st);
+  const double gtmp = gc * sg + gf * b1;
+  gb0 = gc * (-sg) + gtmp * r.e;            // direct -b0 (lo) / +b0 (hi) in ct, tmp = b0 e
+  gb1 = gf * r.tmp + gtmp * r.tmp * 2 * u;
+  gx = gtmp * r.tmp * 2 * b1 * sg;          // du/dx = sg
+}
+
+// Composition M <- M L1 (phs_cpu.cpp:314-340), the same expression order as the forward kernel.
+GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double st, double cp, double sp) {
+  const double l01 = -bg * st, l03 = bg * ct;
+  const double l11 = ct * cp, l12 = -sp, l13 = st * cp;
+  const double l21 = ct * sp, l22 = cp, l23 = st * sp;
+  const double l31 = -g * st, l33 = g * ct;
+  for (int i = 0; i < 4; i++) {
+    const double r0 = M[4 * i], r1 = M[4 * i + 1], r2 = M[4 * i + 2], r3 = M[4 * i + 3];
+    M[4 * i + 0] = r0 * g + r3 * bg;
+    M[4 * i + 1] = r0 * l01 + r1 * l11 + r2 * l21 + r3 * l31;
+    M[4 * i + 2] = r1 * l12 + r2 * l22;
+    M[4 * i + 3] = r0 * l03 + r1 * l13 + r2 * l23 + r3 * l33;
+  }
+}
+
+// Where the per-event adjoint keeps its tape and stash.  `Store` provides
+//   double& M(int slot, int k)   boost after a branch with internal daughters (tape slot)
+//   double& in(int node, int k)  msq, m, p of a branch (k = 0, 1, 2)
+//   double& st(int level, int k) stash level: 16 matrix adjoints + 3 scalar adjoints
+
+// Forward pass of one event with tape recording, then the reverse sweep.
+//   xr: the event's n_x randoms; loss_mode 0: cotangents gp [n_out*4] (may be null) and gf
+//   are given; loss_mode 1: the loss is factor * sum_j (p_jx^2 + p_jy^2) / s and its
+//   cotangents are formed here.  Writes p_out [n_out*4] (may be null), *f_out, *loss_out
+//   (mode 1), gx [n_x] (zero for a non-ok event), and adds d/d(M, Gamma) of every branch to
+//   gpar[2 * node], gpar[2 * node + 1] (visit index; `GPar` is any indexable accumulator).
+//   Returns ok.
+template <class Store, class GPar>
+GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const double* xr, int loss_mode,
+                           const double* gp, double gf_in, double* p_out, double* f_out, double* loss_out,
+                           double* gx, GPar& gpar, Store& S) {
+  const double s = cd.s;
+  const int N = cd.n_nodes;
+  // ---------------- forward (phs_forward.hip, plus the tape)
+  double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  double cur_msq = s, cur_m = cd.sqrts, cur_p = 0, F = 1, S2 = 0;
+  bool ok = true;
+  int depth = 0;
+  for (int i = 0; i < N; i++) {
+    const NodeCmd& nc = prog[i];
+    S.in(i, 0) = cur_msq; S.in(i, 1) = cur_m; S.in(i, 2) = cur_p;
+    double msq[2], f[2], mk[2];
+    for (int c = 0; c < 2; c++) {
+      if (nc.k[c].leaf) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
+      else { MapEval r = msq_eval(nc.k[c], xr[nc.k[c].xi], s); msq[c] = r.msq; f[c] = r.f; mk[c] = sqrt(r.msq); }
+    }
+    const double m = cur_m;
+    const double lda = (cur_msq - msq[0] - msq[1]) * (cur_msq - msq[0] - msq[1]) - 4 * msq[0] * msq[1];
+    ok = ok && (lda > 0 && m > mk[0] + mk[1] && m <= nc.m_max);
+    const double sl = sqrt(lda), pk = sl / (2 * m);
+    F *= f[0] * f[1] * (sl / cur_msq);
+    const double bg = m > 0 ? cur_p / m : 0.0, g = sqrt(1 + bg * bg);
+    double sp, cp;
+    sincos(xr[nc.x_phi] * kTwoPi, &sp, &cp);
+    const CtEval ce = ct_eval(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1);
+    F *= ce.f;
+    compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
+    if (nc.tape_slot >= 0)
+      for (int q = 0; q < 16; q++) S.M(nc.tape_slot, q) = M[q];
+    for (int c = 0; c < 2; c++) {
+      if (!nc.k[c].leaf) continue;
+      const double pc = c == 0 ? pk : -pk;
+      const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : sqrt(nc.k[c].msq_leaf + pc * pc);
+      double q0 = M[0] * E + M[3] * pc, q1 = M[4] * E + M[7] * pc, q2 = M[8] * E + M[11] * pc,
+             q3 = M[12] * E + M[15] * pc;
+      if (cd.bg_lab != 0) {
+        const double b0 = cd.g_lab * q0 + cd.bg_lab * q3, b3 = cd.bg_lab * q0 + cd.g_lab * q3;
+        q0 = b0; q3 = b3;
+      }
+      S2 += q1 * q1 + q2 * q2;
+      if (p_out) {
+        double* o = p_out + 4 * nc.k[c].out;
+        o[0] = q0; o[1] = q1; o[2] = q2; o[3] = q3;
+      }
+    }
+    if (nc.push >= 0) {   // the deferred daughter's boost and kinematics
+      for (int q = 0; q < 16; q++) S.st(depth, q) = M[q];
+      S.st(depth, 16) = msq[nc.push]; S.st(depth, 17) = mk[nc.push]; S.st(depth, 18) = nc.push == 0 ? pk : -pk;
+      depth++;
+    }
+    if (nc.desc >= 0) {
+      cur_msq = msq[nc.desc]; cur_m = mk[nc.desc]; cur_p = nc.desc == 0 ? pk : -pk;
+    } else if (depth > 0) {
+      depth--;
+      for (int q = 0; q < 16; q++) M[q] = S.st(depth, q);
+      cur_msq = S.st(depth, 16); cur_m = S.st(depth, 17); cur_p = S.st(depth, 18);
+    }
+  }
+  *f_out = F;
+  for (int k = 0; k < cd.n_x; k++) gx[k] = 0;
+  if (loss_out) *loss_out = ok && loss_mode == 1 ? F * S2 / s : 0.0;
+  if (!ok) return false;
+
+  // ---------------- cotangents: of the factor here, of the momenta per leaf in the sweep
+  const double gF = loss_mode == 1 ? S2 / s : gf_in;
+  const double W = gF * F;   // adjoint of log(term) for every factor of the product
+
+  // ---------------- reverse sweep
+  double Gc[16] = {0};       // adjoint of the parent boost, from the branch just reversed
+  double gc_msq = 0, gc_m = 0, gc_p = 0;
+  for (int i = N - 1; i >= 0; i--) {
+    const NodeCmd& nc = prog[i];
+    double G[16] = {0};
+    double g_msq[2] = {0, 0}, g_m[2] = {0, 0}, g_pk = 0;
+    if (nc.desc >= 0) {
+      for (int q = 0; q < 16; q++) G[q] += Gc[q];
+      g_msq[nc.desc] += gc_msq; g_m[nc.desc] += gc_m; g_pk += nc.desc == 0 ? gc_p : -gc_p;
+    }
+    if (nc.push >= 0) {
+      depth--;
+      for (int q = 0; q < 16; q++) G[q] += S.st(depth, q);
+      g_msq[nc.push] += S.st(depth, 16); g_m[nc.push] += S.st(depth, 17);
+      g_pk += nc.push == 0 ? S.st(depth, 18) : -S.st(depth, 18);
+    }
+    // recompute the branch's local quantities
+    const double a = S.in(i, 0), m = S.in(i, 1), p0 = S.in(i, 2);
+    double P[16];
+    if (nc.parent < 0) { for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0; }
+    else { const int ps = prog[nc.parent].tape_slot; for (int q = 0; q < 16; q++) P[q] = S.M(ps, q); }
+    MapEval r[2];
+    double msq[2], f[2], mk[2], xk[2];
+    for (int c = 0; c < 2; c++) {
+      if (nc.k[c].leaf) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
+      else { xk[c] = xr[nc.k[c].xi]; r[c] = msq_eval(nc.k[c], xk[c], s); msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = sqrt(msq[c]); }
+    }
+    const double d = a - msq[0] - msq[1];
+    const double lda = d * d - 4 * msq[0] * msq[1];
+    const double sl = sqrt(lda), pk = sl / (2 * m);
+    const double bg = m > 0 ? p0 / m : 0.0, g = sqrt(1 + bg * bg);
+    const double xphi = xr[nc.x_phi], xct = xr[nc.x_phi + 1];
+    double sp, cp;
+    sincos(xphi * kTwoPi, &sp, &cp);
+    const CtEval ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
+    double Ma[16];
+    for (int q = 0; q < 16; q++) Ma[q] = P[q];
+    compose_l1(Ma, g, bg, ce.ct, ce.st, cp, sp);
+
+    // final-state daughters: q = Ma[:,0] E + Ma[:,3] pc
+    for (int c = 0; c < 2; c++) {
+      if (!nc.k[c].leaf) continue;
+      const double pc = c == 0 ? pk : -pk;
+      const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : sqrt(nc.k[c].msq_leaf + pc * pc);
+      // cotangent of the lab-frame momentum, moved back through the (linear) lab boost
+      double gq[4];
+      if (loss_mode == 1) {
+        const double q1 = Ma[4] * E + Ma[7] * pc, q2 = Ma[8] * E + Ma[11] * pc;
+        gq[0] = 0; gq[1] = F * 2 * q1 / s; gq[2] = F * 2 * q2 / s; gq[3] = 0;
+      } else if (gp) {
+        for (int mu = 0; mu < 4; mu++) gq[mu] = gp[4 * nc.k[c].out + mu];
+      } else {
+        gq[0] = gq[1] = gq[2] = gq[3] = 0;
+      }
+      if (cd.bg_lab != 0) {
+        const double a0 = gq[0], a3 = gq[3];
+        gq[0] = cd.g_lab * a0 + cd.bg_lab * a3;
+        gq[3] = cd.bg_lab * a0 + cd.g_lab * a3;
+      }
+      double gE = 0, gpc = 0;
+      for (int mu = 0; mu < 4; mu++) {
+        G[4 * mu + 0] += gq[mu] * E;
+        G[4 * mu + 3] += gq[mu] * pc;
+        gE += gq[mu] * Ma[4 * mu + 0];
+        gpc += gq[mu] * Ma[4 * mu + 3];
+      }
+      if (E > 0) gpc += gE * pc / E;
+      g_pk += c == 0 ? gpc : -gpc;
+    }
+    // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros only)
+    const double L[16] = {g, -bg * ce.st, 0, bg * ce.ct, 0, ce.ct * cp, -sp, ce.st * cp,
+                          0, ce.ct * sp, cp, ce.st * sp, bg, -g * ce.st, 0, g * ce.ct};
+    double gP[16];
+    for (int ii = 0; ii < 4; ii++)
+      for (int k = 0; k < 4; k++) {
+        double acc = 0;
+        for (int j = 0; j < 4; j++) acc += G[4 * ii + j] * L[4 * k + j];
+        gP[4 * ii + k] = acc;
+      }
+    auto gl = [&](int k, int j) {
+      double acc = 0;
+      for (int ii = 0; ii < 4; ii++) acc += P[4 * ii + k] * G[4 * ii + j];
+      return acc;
+    };
+    const double g00 = gl(0, 0), g01 = gl(0, 1), g03 = gl(0, 3), g11 = gl(1, 1), g12 = gl(1, 2), g13 = gl(1, 3);
+    const double g21 = gl(2, 1), g22 = gl(2, 2), g23 = gl(2, 3), g30 = gl(3, 0), g31 = gl(3, 1), g33 = gl(3, 3);
+    double gg = g00 - g31 * ce.st + g33 * ce.ct;
+    double gbg = -g01 * ce.st + g03 * ce.ct + g30;
+    const double gct = g03 * bg + g11 * cp + g21 * sp + g33 * g;
+    const double gst = -g01 * bg + g13 * cp + g23 * sp - g31 * g;
+    const double gcp = g11 * ce.ct + g13 * ce.st + g22;
+    const double gsp = -g12 + g21 * ce.ct + g23 * ce.st;
+    // phi = 2 pi x
+    gx[nc.x_phi] = (gsp * cp - gcp * sp) * kTwoPi;
+    // cos theta map of this branch (its own M through b0, b1)
+    const double gfct = ce.f != 0 ? W / ce.f : 0.0;
+    double gxc, gb0, gb1;
+    ct_eval_adj(nc.ct_kind, xct, nc.b0, nc.b1, ce, gct, gst, gfct, gxc, gb0, gb1);
+    gx[nc.x_phi + 1] = gxc;
+    gpar[2 * i] += gb0 * nc.db_dm[0] + gb1 * nc.db_dm[1];
+    // gamma = sqrt(1 + bg^2), bg = p / m
+    gbg += gg * bg / g;
+    double g_a = 0, g_mnode = 0, g_p0 = 0;
+    if (m > 0) { g_p0 += gbg / m; g_mnode -= gbg * p0 / (m * m); }
+    // pk = sl / (2 m)
+    double gsl = g_pk / (2 * m);
+    g_mnode -= g_pk * sl / (2 * m * m);
+    // factor term f0 f1 sl / a
+    double gfk[2] = {0, 0};
+    if (W != 0) {
+      for (int c = 0; c < 2; c++)
+        if (f[c] != 0) gfk[c] = W / f[c];
+      if (sl != 0) gsl += W / sl;
+      g_a -= W / a;
+    }
+    // lambda = d^2 - 4 msq0 msq1, d = a - msq0 - msq1
+    const double glda = sl > 0 ? gsl / (2 * sl) : 0.0;
+    const double gd = 2 * d * glda;
+    g_msq[0] += -4 * msq[1] * glda - gd;
+    g_msq[1] += -4 * msq[0] * glda - gd;
+    g_a += gd;
+    // internal daughters: m_k = sqrt(msq_k), then their msq maps
+    for (int c = 0; c < 2; c++) {
+      if (nc.k[c].leaf) continue;
+      if (mk[c] > 0) g_msq[c] += g_m[c] / (2 * mk[c]);
+      double gxk = 0, gM = 0, gW = 0;
+      msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      gx[nc.k[c].xi] = gxk;
+      gpar[2 * nc.k[c].node] += gM;
+      gpar[2 * nc.k[c].node + 1] += gW;
+    }
+    // hand the adjoints of this branch's inputs to its parent
+    for (int q = 0; q < 16; q++) Gc[q] = gP[q];
+    gc_msq = g_a; gc_m = g_mnode; gc_p = g_p0;
+    if (i >= 1 && prog[i - 1].desc < 0) {   // reached by a pop in the forward pass
+      for (int q = 0; q < 16; q++) S.st(depth, q) = Gc[q];
+      S.st(depth, 16) = gc_msq; S.st(depth, 17) = gc_m; S.st(depth, 18) = gc_p;
+      depth++;
+    }
+  }
+  return true;
+}
+
+}  // namespace gphs

@@ -1,0 +1,106 @@
+// Adjoint kernel: per event, the forward pass with a tape, then the reverse sweep
+// (phs_adjoint.h).  One workgroup = one wavefront = one tile of one channel; the tape, the
+// stash and the lane's (M, Gamma) accumulators live in LDS, lane-strided so every access is
+// conflict-free.  The parameter adjoints of a tile are reduced across the wavefront and
+// written as one partial per tile; a second kernel sums the partials of each channel in a
+// fixed order, so results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include "phs_adjoint.h"
+#include "phs_kernels.h"
+
+namespace gphs {
+
+namespace {
+
+struct LdsStore {
+  double* base;
+  int stride, off_in, off_st;
+  __device__ double& M(int slot, int k) { return base[(slot * 16 + k) * stride]; }
+  __device__ double& in(int node, int k) { return base[(off_in + node * 3 + k) * stride]; }
+  __device__ double& st(int level, int k) { return base[(off_st + level * 19 + k) * stride]; }
+};
+
+struct LdsAcc {
+  double* base;
+  int stride;
+  __device__ double& operator[](int i) { return base[i * stride]; }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end,
+                                                     const SegDesc* __restrict__ segs,
+                                                     const ChannelDesc* __restrict__ chans,
+                                                     const NodeCmd* __restrict__ nodes, int loss_mode,
+                                                     double* __restrict__ gpart, int part_stride, AdjLayout lay) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  LdsStore S{lds + lane, 64, lay.off_in, lay.off_st};
+  LdsAcc acc{lds + lay.off_acc * 64 + lane, 64};
+  for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
+    const TileDesc td = tiles[t];
+    const SegDesc sd = segs[td.seg];
+    const ChannelDesc cd = chans[sd.channel];
+    const NodeCmd* prog = nodes + cd.node0;
+    const int np = 2 * cd.n_nodes;
+    for (int v = 0; v < np; v++) acc[v] = 0;
+    for (int local = lane; local < td.count; local += 64) {
+      const int64_t e = td.e0 + local;
+      const double* xr = sd.x + e * cd.n_x;
+      const double* gp = sd.g_p ? sd.g_p + e * (int64_t)cd.n_out * 4 : nullptr;
+      const double gf = sd.g_f ? sd.g_f[e] : 0.0;
+      double* p_out = sd.p ? sd.p + e * (int64_t)cd.n_out * 4 : nullptr;
+      double F = 0, L = 0;
+      const bool ok = event_adjoint(cd, prog, xr, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc, S);
+      if (sd.factor) sd.factor[e] = F;
+      if (sd.volume) sd.volume[e] = cd.volume;
+      if (sd.ok) sd.ok[e] = ok ? 1 : 0;
+      if (sd.loss) sd.loss[e] = L;
+    }
+    for (int v = 0; v < np; v++) {
+      const double tot = wave_sum(acc[v]);
+      if (lane == 0) gpart[(int64_t)t * part_stride + v] = tot;
+    }
+  }
+}
+
+// gparam[2 * slot + w] = sum over the tiles of the slot's channel, in tile order (every slot is
+// written, channels without events get 0).
+__global__ void param_reduce_kernel(const double* __restrict__ gpart, int part_stride,
+                                    const int* __restrict__ chan_tiles, const int* __restrict__ chan_tile_off,
+                                    const ChannelDesc* __restrict__ chans, int n_chans, double* __restrict__ gparam) {
+  const int c = blockIdx.x;
+  if (c >= n_chans) return;
+  const int np = 2 * chans[c].n_nodes;
+  const int v = threadIdx.x;
+  if (v >= np) return;
+  double sum = 0;
+  for (int k = chan_tile_off[c]; k < chan_tile_off[c + 1]; k++) sum += gpart[(int64_t)chan_tiles[k] * part_stride + v];
+  gparam[2 * chans[c].param0 + v] = sum;
+}
+
+}  // namespace
+
+hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
+                          const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam, int grid,
+                          hipStream_t stream) {
+  const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
+  if (shmem > 65536) return hipErrorInvalidValue;
+  if (dp.n_tiles > 0) {
+    const int g = grid > 0 && grid < dp.n_tiles ? grid : dp.n_tiles;
+    hipLaunchKernelGGL(adjoint_kernel, dim3(g), dim3(64), shmem, stream, dp.tiles, 0, dp.n_tiles, dp.segs, dp.chans,
+                       dp.nodes, loss_mode, gpart, part_stride, lay);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(param_reduce_kernel, dim3(n_chans), dim3(64), 0, stream, gpart, part_stride, chan_tiles,
+                     chan_tile_off, dp.chans, n_chans, gparam);
+  return hipGetLastError();
+}
+
+}  // namespace gphs

@@ -62,6 +62,7 @@ PlanObj* find_plan(int h) {
 int drop_plan(int h) { return g_plans.erase(h) ? GPU_PHS_OK : GPU_PHS_E_HANDLE; }
 
 PlanObj::~PlanObj() {
+  if (adj_dev) (void)hipFree(adj_dev);
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -79,6 +80,8 @@ static void drop_graph(PlanObj& pl) {
 int plan_build(PlanObj& pl) {
   pl.chans.clear(); pl.nodes.clear(); pl.segs.clear(); pl.tiles.clear();
   pl.seg_tile_begin.clear(); pl.seg_tile_end.clear();
+  pl.chan_process.clear(); pl.chan_channel.clear();
+  if (pl.adj_dev) { (void)hipFree(pl.adj_dev); pl.adj_dev = nullptr; pl.adj_bytes = 0; }
   pl.n_banks = 0;
   std::map<std::pair<int, int>, int> chan_index;
   for (const auto& us : pl.user) {
@@ -101,6 +104,8 @@ int plan_build(PlanObj& pl) {
       cd.param0 = node0;
       ci = (int)pl.chans.size();
       pl.chans.push_back(cd);
+      pl.chan_process.push_back(us.process);
+      pl.chan_channel.push_back(us.channel);
       chan_index[key] = ci;
       pl.n_banks = std::max(pl.n_banks, (int)cd.n_banks);
     } else {
@@ -277,21 +282,24 @@ int gpu_phs_plan_destroy(int plan) {
   return ok_status();
 }
 
-int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order) {
+int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds_bank) {
   std::lock_guard<std::mutex> lk(api_mutex());
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
-  if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1)
-    return set_error(GPU_PHS_E_ARG, "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}");
+  if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1 ||
+      !(waves_per_eu == 0 || (waves_per_eu >= 3 && waves_per_eu <= 5)) || lds_bank < 0 || lds_bank > 1)
+    return set_error(GPU_PHS_E_ARG,
+                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4,5}, "
+                     "lds_bank in {0,1}");
   bool rebuild = block != pl->lc.block || ept != pl->lc.ept || order != pl->order;
+  if (grid != pl->lc.grid || waves_per_eu != pl->lc.waves_per_eu || lds_bank != pl->lc.lds_bank) drop_graph(*pl);
   pl->lc.block = block; pl->lc.ept = ept; pl->order = order;
-  if (grid != pl->lc.grid) drop_graph(*pl);
-  pl->lc.grid = grid;
+  pl->lc.grid = grid; pl->lc.waves_per_eu = waves_per_eu; pl->lc.lds_bank = lds_bank;
   if (rebuild) return plan_build(*pl);
   return ok_status();
 }
 
-int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order) {
+int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu, int* lds_bank) {
   std::lock_guard<std::mutex> lk(api_mutex());
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
@@ -299,6 +307,8 @@ int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* orde
   if (ept) *ept = pl->lc.ept;
   if (grid) *grid = pl->lc.grid;
   if (order) *order = pl->order;
+  if (waves_per_eu) *waves_per_eu = pl->lc.waves_per_eu;
+  if (lds_bank) *lds_bank = pl->lc.lds_bank;
   return ok_status();
 }
 
@@ -363,11 +373,16 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
   hipEvent_t a, b;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
     return set_error(GPU_PHS_E_HIP, "hipEventCreate");
-  auto time_cfg = [&](int block, int ept, int grid, int order, float* ms) -> int {
-    pl->lc.block = block; pl->lc.ept = ept; pl->lc.grid = grid; pl->order = order;
-    int rc = plan_build(*pl);
-    if (rc) return rc;
-    rc = plan_run(*pl, stream);   // warm-up
+  auto time_cfg = [&](const LaunchConfig& lc, int order, float* ms) -> int {
+    const bool rebuild = lc.block != pl->lc.block || lc.ept != pl->lc.ept || order != pl->order;
+    pl->lc = lc;
+    pl->order = order;
+    drop_graph(*pl);
+    if (rebuild) {
+      int rc = plan_build(*pl);
+      if (rc) return rc;
+    }
+    int rc = plan_run(*pl, stream);   // warm-up
     if (rc) return rc;
     std::vector<float> t;
     for (int r = 0; r < reps; r++) {
@@ -384,42 +399,62 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
     *ms = t[t.size() / 2];
     return GPU_PHS_OK;
   };
-  struct Cfg { int block, ept, grid, order; float ms; };
+  struct Cfg { LaunchConfig lc; int order; float ms; };
   std::vector<Cfg> tried;
-  Cfg best{pl->lc.block, pl->lc.ept, pl->lc.grid, pl->order, 1e30f};
-  auto consider = [&](int block, int ept, int grid, int order) -> int {
+  Cfg best{pl->lc, pl->order, 1e30f};
+  auto consider = [&](LaunchConfig lc, int order) -> int {
     float ms = 0;
-    int rc = time_cfg(block, ept, grid, order, &ms);
+    int rc = time_cfg(lc, order, &ms);
     if (rc) return rc;
-    tried.push_back({block, ept, grid, order, ms});
-    if (ms < best.ms) best = {block, ept, grid, order, ms};
+    tried.push_back({lc, order, ms});
+    if (ms < best.ms) best = {lc, order, ms};
     return GPU_PHS_OK;
   };
   int rc = GPU_PHS_OK;
-  // stage 1: workgroup size x events per thread, one workgroup per tile
+  const bool banks = pl->n_banks > 0;
+  // stage 1: register budget (occupancy target) and where deferred boosts wait
+  for (int wpe : {0, 3, 4})
+    for (int lds : {0, 1})
+      if (!rc && (lds == 0 || banks)) {
+        LaunchConfig lc = best.lc;
+        lc.waves_per_eu = wpe; lc.lds_bank = lds; lc.grid = 0;
+        rc = consider(lc, 0);
+      }
+  // stage 2: workgroup size x events per thread
+  const LaunchConfig base = best.lc;
   for (int block : {64, 128, 256})
     for (int ept : {1, 2, 4})
-      if (!rc) rc = consider(block, ept, 0, 0);
-  // stage 2: persistent grids and tile order around the best shape
-  const int b0 = best.block, e0 = best.ept;
-  for (int k : {1, 2, 4, 8})
-    if (!rc) rc = consider(b0, e0, n_cu * k * (256 / b0), 0);
-  for (int order : {1})
-    if (!rc) rc = consider(b0, e0, best.grid, order);
+      if (!rc) {
+        LaunchConfig lc = base;
+        lc.block = block; lc.ept = ept;
+        rc = consider(lc, 0);
+      }
+  // stage 3: persistent grids and tile order around the best shape
+  const LaunchConfig b2 = best.lc;
+  for (int k : {2, 4, 8})
+    if (!rc) {
+      LaunchConfig lc = b2;
+      lc.grid = n_cu * k * (256 / b2.block);
+      rc = consider(lc, 0);
+    }
+  if (!rc) rc = consider(best.lc, 1);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;
-  pl->lc.block = best.block; pl->lc.ept = best.ept; pl->lc.grid = best.grid; pl->order = best.order;
+  pl->lc = best.lc;
+  pl->order = best.order;
+  drop_graph(*pl);
   rc = plan_build(*pl);
   if (rc) return rc;
   if (report && report_len > 0) {
     std::string s;
     char buf[96];
-    snprintf(buf, sizeof buf, "best block=%d ept=%d grid=%d order=%d %.4f ms;", best.block, best.ept, best.grid,
-             best.order, best.ms);
+    snprintf(buf, sizeof buf, "best block=%d ept=%d grid=%d order=%d wpe=%d lds=%d %.4f ms;", best.lc.block,
+             best.lc.ept, best.lc.grid, best.order, best.lc.waves_per_eu, best.lc.lds_bank, best.ms);
     s += buf;
     for (const auto& c : tried) {
-      snprintf(buf, sizeof buf, " %d/%d/%d/%d:%.4f", c.block, c.ept, c.grid, c.order, c.ms);
+      snprintf(buf, sizeof buf, " %d/%d/%d/%d/%d/%d:%.4f", c.lc.block, c.lc.ept, c.lc.grid, c.order,
+               c.lc.waves_per_eu, c.lc.lds_bank, c.ms);
       s += buf;
     }
     snprintf(report, report_len, "%s", s.c_str());

@@ -20,36 +20,75 @@ namespace {
 template <int NBANK>
 struct Banks {
   double L[NBANK][16];
-  double msq[NBANK], p[NBANK];
+  double msq[NBANK], m[NBANK], p[NBANK];
 
-  __device__ __forceinline__ void push(int d, const double (&M)[16], double m2, double pp) {
+  __device__ __forceinline__ void push(int d, const double (&M)[16], double m2, double mm, double pp) {
 #pragma unroll
     for (int b = 0; b < NBANK; b++)
       if (b == d) {
 #pragma unroll
         for (int i = 0; i < 16; i++) L[b][i] = M[i];
-        msq[b] = m2; p[b] = pp;
+        msq[b] = m2; m[b] = mm; p[b] = pp;
       }
   }
-  __device__ __forceinline__ void pop(int d, double (&M)[16], double& m2, double& pp) const {
+  __device__ __forceinline__ void pop(int d, double (&M)[16], double& m2, double& mm, double& pp) const {
 #pragma unroll
     for (int b = 0; b < NBANK; b++)
       if (b == d) {
 #pragma unroll
         for (int i = 0; i < 16; i++) M[i] = L[b][i];
-        m2 = msq[b]; pp = p[b];
+        m2 = msq[b]; mm = m[b]; pp = p[b];
       }
   }
 };
 
 template <>
 struct Banks<0> {
-  __device__ __forceinline__ void push(int, const double (&)[16], double, double) {}
-  __device__ __forceinline__ void pop(int, double (&)[16], double&, double&) const {}
+  __device__ __forceinline__ void push(int, const double (&)[16], double, double, double) {}
+  __device__ __forceinline__ void pop(int, double (&)[16], double&, double&, double&) const {}
 };
 
-// One event of channel `cd` (program `prog`).
+// The same save bank in LDS: level d of lane t holds kBankWords doubles at
+// lds[(kBankWords d + i) * block + t] (consecutive lanes hit consecutive 8-byte words, so
+// every access is conflict-free).  Frees 38 VGPRs per level for occupancy at the price of 19
+// LDS stores and loads per deferral.
+constexpr int kBankWords = 19;
+struct LdsBanks {
+  double* base;
+  int stride;
+  __device__ __forceinline__ void push(int d, const double (&M)[16], double m2, double mm, double pp) {
+    double* b = base + kBankWords * d * stride;
+#pragma unroll
+    for (int i = 0; i < 16; i++) b[i * stride] = M[i];
+    b[16 * stride] = m2;
+    b[17 * stride] = mm;
+    b[18 * stride] = pp;
+  }
+  __device__ __forceinline__ void pop(int d, double (&M)[16], double& m2, double& mm, double& pp) const {
+    const double* b = base + kBankWords * d * stride;
+#pragma unroll
+    for (int i = 0; i < 16; i++) M[i] = b[i * stride];
+    m2 = b[16 * stride];
+    mm = b[17 * stride];
+    pp = b[18 * stride];
+  }
+};
+
+extern __shared__ double lds_banks[];
+
+template <int NBANK, bool LDS>
+struct BankSel { using type = Banks<NBANK>; };
 template <int NBANK>
+struct BankSel<NBANK, true> { using type = LdsBanks; };
+
+template <int NBANK, bool LDS>
+__device__ __forceinline__ typename BankSel<NBANK, LDS>::type make_banks() {
+  if constexpr (LDS) return LdsBanks{lds_banks + threadIdx.x, (int)blockDim.x};
+  else return Banks<NBANK>{};
+}
+
+// One event of channel `cd` (program `prog`).
+template <int NBANK, bool LDS>
 __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
                                               const SegDesc& sd, int64_t e, bool valid) {
   const gdouble* __restrict__ xr = as_global(sd.x) + e * cd.n_x;
@@ -58,25 +97,25 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
 
   // boost of the parent branch; the root's parent is the identity (phs_cpu.cpp:568-573)
   double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  Banks<NBANK> bank;
+  auto bank = make_banks<NBANK, LDS>();
   double cur_msq = s;    // root: msq = s (phs_cpu.cpp:195)
   double cur_p = 0.0;    // root: p_decay = 0 (never set by the reference)
   double factor = 1.0;
   bool ok = true;
   int depth = 0;
 
+  double cur_m = cd.sqrts;  // sqrt(msq) of the branch being visited, carried from its parent
   for (int i = 0; i < cd.n_nodes; ++i) {
     const NodeCmd& nc = prog[i];
-    // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547)
-    double msq0, f0, msq1, f1;
-    if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; }
-    else msq_map(nc.k[0], xr[nc.k[0].xi], s, msq0, f0);
-    if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; }
-    else msq_map(nc.k[1], xr[nc.k[1].xi], s, msq1, f1);
+    // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547) and masses
+    double msq0, f0, m1, msq1, f1, m2;
+    if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
+    else { msq_map(nc.k[0], xr[nc.k[0].xi], s, msq0, f0); m1 = sqrt(msq0); }
+    if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
+    else { msq_map(nc.k[1], xr[nc.k[1].xi], s, msq1, f1); m2 = sqrt(msq1); }
 
     // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
-    const double m = sqrt(cur_msq);
-    const double m1 = sqrt(msq0), m2 = sqrt(msq1);
+    const double m = cur_m;
     const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
     ok = ok && (lda > 0 && m > m1 + m2 && m <= nc.m_max);
     const double sl = sqrt(lda);
@@ -95,12 +134,12 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     compose(M, g, bg, ct, st, cp, sp);
 
     // final-state daughters: p = L (E, 0, 0, +-p) (phs_cpu.cpp:286-291), then the lab boost
-    // of the final leaves (phs.cu:1286-1301)
+    // of the final leaves (phs.cu:1286-1301); E = sqrt(0 + p^2) is |p| exactly
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       if (nc.k[c].leaf) {
         const double pc = c == 0 ? pk : -pk;
-        const double E = sqrt(nc.k[c].msq_leaf + pc * pc);
+        const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : sqrt(nc.k[c].msq_leaf + pc * pc);
         double q0 = M[0] * E + M[3] * pc;
         const double q1 = M[4] * E + M[7] * pc;
         const double q2 = M[8] * E + M[11] * pc;
@@ -119,14 +158,16 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
 
     // next branch: a daughter (matrix stays), or the deferred one from the save bank
     const double msq_push = nc.push == 0 ? msq0 : msq1;
+    const double m_push = nc.push == 0 ? m1 : m2;
     const double p_push = nc.push == 0 ? pk : -pk;
-    if (nc.push >= 0) { bank.push(depth, M, msq_push, p_push); depth++; }
+    if (nc.push >= 0) { bank.push(depth, M, msq_push, m_push, p_push); depth++; }
     if (nc.desc >= 0) {
       cur_msq = nc.desc == 0 ? msq0 : msq1;
+      cur_m = nc.desc == 0 ? m1 : m2;
       cur_p = nc.desc == 0 ? pk : -pk;
     } else if (depth > 0) {
       depth--;
-      bank.pop(depth, M, cur_msq, cur_p);
+      bank.pop(depth, M, cur_msq, cur_m, cur_p);
     }
   }
   if (valid) {
@@ -136,11 +177,12 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
   }
 }
 
-template <int NBANK>
-__global__ __launch_bounds__(256) void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end,
-                                                      const SegDesc* __restrict__ segs,
-                                                      const ChannelDesc* __restrict__ chans,
-                                                      const NodeCmd* __restrict__ nodes, int ept) {
+// WPE > 0 asks the register allocator for at least WPE waves per SIMD (fewer VGPRs, spills
+// allowed); the autotuner measures which occupancy target wins.
+template <int NBANK, bool LDS, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
+                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int ept) {
   for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
@@ -152,24 +194,49 @@ __global__ __launch_bounds__(256) void forward_kernel(const TileDesc* __restrict
       const int local = base + (int)threadIdx.x;
       const bool valid = local < td.count;
       const int64_t e = td.e0 + (valid ? local : td.count - 1);
-      forward_event<NBANK>(cd, prog, sd, e, valid);
+      forward_event<NBANK, LDS>(cd, prog, sd, e, valid);
     }
   }
 }
 
 }  // namespace
 
+namespace {
+using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int);
+
+template <bool LDS, int WPE>
+FwdKernel pick_banks(int n_banks) {
+  switch (n_banks) {
+    case 0: return forward_kernel<0, false, WPE>;
+    case 1: return forward_kernel<1, LDS, WPE>;
+    case 2: return forward_kernel<2, LDS, WPE>;
+    default: return forward_kernel<3, LDS, WPE>;
+  }
+}
+
+template <bool LDS>
+FwdKernel pick_wpe(int n_banks, int wpe) {
+  switch (wpe) {
+    case 3: return pick_banks<LDS, 3>(n_banks);
+    case 4: return pick_banks<LDS, 4>(n_banks);
+    case 5: return pick_banks<LDS, 5>(n_banks);
+    default: return pick_banks<LDS, 0>(n_banks);
+  }
+}
+}  // namespace
+
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream) {
   if (t_end <= t_begin) return hipSuccess;
+  if (dp.n_banks < 0 || dp.n_banks > 3 || lc.block < 64 || lc.block > 256) return hipErrorInvalidValue;
   const int n = t_end - t_begin;
   const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
-  const dim3 g(grid), b(lc.block);
-  switch (dp.n_banks) {
-    case 0: hipLaunchKernelGGL(forward_kernel<0>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
-    case 1: hipLaunchKernelGGL(forward_kernel<1>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
-    case 2: hipLaunchKernelGGL(forward_kernel<2>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
-    default: hipLaunchKernelGGL(forward_kernel<3>, g, b, 0, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans, dp.nodes, lc.ept); break;
-  }
+  // LDS banks only when they fit the 64 KiB a workgroup may allocate; else the VGPR banks
+  const size_t lds_bytes = (size_t)kBankWords * sizeof(double) * lc.block * dp.n_banks;
+  const bool lds = lc.lds_bank && dp.n_banks > 0 && lds_bytes <= 65536;
+  const size_t shmem = lds ? lds_bytes : 0;
+  FwdKernel k = lds ? pick_wpe<true>(dp.n_banks, lc.waves_per_eu) : pick_wpe<false>(dp.n_banks, lc.waves_per_eu);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,
+                     dp.nodes, lc.ept);
   return hipGetLastError();
 }
 

@@ -20,9 +20,24 @@ struct LaunchConfig {
   int block = 256;       // threads per workgroup (multiple of 64, <= 256)
   int ept = 1;           // events per thread per tile
   int grid = 0;          // workgroups; 0 = one per tile, else a persistent grid that strides
+  int waves_per_eu = 0;  // register-allocation occupancy target: 0 (compiler's choice), 3, 4, 5
+  int lds_bank = 0;      // 1: keep the save banks of deferred subtrees in LDS instead of VGPRs
 };
 
 // Forward pass over tiles [t_begin, t_end).
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream);
+
+// Per-lane LDS layout of the adjoint (in doubles, lane-strided): tape boosts [0, off_in),
+// branch inputs [off_in, off_st), stash [off_st, off_acc), (M, Gamma) accumulators
+// [off_acc, words).
+struct AdjLayout {
+  int off_in, off_st, off_acc, words;
+};
+
+// Adjoint over all tiles, then the per-channel reduction of the parameter partials into
+// gparam[2 * (chans[c].param0 + node) + {0: M, 1: Gamma}].
+hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
+                          const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam, int grid,
+                          hipStream_t stream);
 
 }  // namespace gphs

@@ -41,6 +41,17 @@ struct PlanObj {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t capture_stream = nullptr;
+  // adjoint: a copy of the tables with the gradient buffers, per-tile partials, tiles by channel
+  void* adj_dev = nullptr;
+  size_t adj_bytes = 0;
+  std::vector<SegDesc> adj_segs;
+  DevicePlan adj_dp;
+  AdjLayout adj_lay{};
+  int adj_part_stride = 0;
+  double* adj_part = nullptr;
+  int* adj_chan_tiles = nullptr;
+  int* adj_chan_off = nullptr;
+  std::vector<int> chan_process, chan_channel;   // plan channel -> (process handle, channel)
   ~PlanObj();
 };
 

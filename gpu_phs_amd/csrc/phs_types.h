@@ -33,12 +33,17 @@ struct ChildCmd {
   int32_t xi;        // msq random index (internal)
   int32_t out;       // final-state slot j = log2(bincode) (leaf)
   double msq_leaf;   // m_j^2 (leaf)
+  double m_leaf;     // sqrt(m_j^2), rounded as the device would (sqrt is correctly rounded)
   double a0, a1, a2; // msq map constants a[3] (internal), phs_cpu.cpp:452-511
   double m, w;       // map mass and width (internal)
   double msq_min;    // m_min^2, m_min = sum of leaf masses below the branch
   double msq_max;    // m_max^2, m_max = sqrt(s) - m_tot + m_min
   double f_lin;      // (msq_max - msq_min) / s
   double m_min;      // for the per-event sqrt(s) path
+  double da_dm[3];   // d a[k] / d M of this daughter's map (adjoint)
+  double da_dw[3];   // d a[k] / d Gamma
+  int32_t node;      // visit index of this daughter's NodeCmd (internal), -1 for a leaf
+  int32_t pad0;
 };
 
 struct NodeCmd {
@@ -47,12 +52,14 @@ struct NodeCmd {
   int32_t desc;      // daughter (0/1) visited next, -1: next branch comes from the save bank
   int32_t push;      // daughter (0/1) deferred to the save bank, -1: none
   int32_t map_id;    // reference map id of this branch (for the adjoint / per-event path)
-  int32_t pad0, pad1, pad2;
+  int32_t parent;    // visit index of the parent branch, -1 for the root
+  int32_t slot;      // which daughter (0/1) of the parent this branch is
+  int32_t tape_slot; // adjoint tape slot of the boost after this branch, -1: no internal daughter
   double m_max;      // kinematic bound of this branch's mass (root: sqrt(s))
   double b0, b1;     // cos-theta constants b[0], b[1], phs_cpu.cpp:514-527
   double m_min;      // sum of leaf masses below the branch
   double m;          // map mass of this branch (cos-theta constants per event)
-  double pad3;
+  double db_dm[2];   // d b[k] / d M (adjoint)
   ChildCmd k[2];
 };
 
@@ -63,7 +70,8 @@ struct ChannelDesc {
   int32_t n_x;
   int32_t param0;    // first (M, Gamma) accumulator slot of this channel (adjoint)
   int32_t n_banks;   // save-bank depth the program needs
-  int32_t pad0, pad1;
+  int32_t n_tape;    // adjoint tape slots the program needs
+  int32_t pad1;
   double sqrts, s;
   double volume;     // phase-space volume, evaluated in the reference's order
   double bg_lab;     // lab-frame boost beta*gamma = (E1 - E2) / (2 sqrt(E1 E2)), phs.cu:1289

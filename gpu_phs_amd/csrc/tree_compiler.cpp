@@ -38,47 +38,79 @@ bool ct_kind(int id, int32_t* kind) {
   }
 }
 
+// Value and derivatives with respect to (M, Gamma) of a map constant: lets the same code
+// give the constants and their Jacobians, which the adjoint kernel chains through.
+struct DualMW {
+  double v = 0, dm = 0, dw = 0;
+  DualMW() = default;
+  DualMW(double x) : v(x) {}
+  DualMW(double x, double a, double b) : v(x), dm(a), dw(b) {}
+};
+inline DualMW operator+(DualMW a, DualMW b) { return {a.v + b.v, a.dm + b.dm, a.dw + b.dw}; }
+inline DualMW operator-(DualMW a, DualMW b) { return {a.v - b.v, a.dm - b.dm, a.dw - b.dw}; }
+inline DualMW operator-(DualMW a) { return {-a.v, -a.dm, -a.dw}; }
+inline DualMW operator*(DualMW a, DualMW b) { return {a.v * b.v, a.dm * b.v + a.v * b.dm, a.dw * b.v + a.v * b.dw}; }
+inline DualMW operator/(DualMW a, DualMW b) {
+  return {a.v / b.v, (a.dm * b.v - a.v * b.dm) / (b.v * b.v), (a.dw * b.v - a.v * b.dw) / (b.v * b.v)};
+}
+inline DualMW d2chain(DualMW a, double f, double df) { return {f, df * a.dm, df * a.dw}; }
+inline DualMW atan(DualMW a) { return d2chain(a, std::atan(a.v), 1 / (1 + a.v * a.v)); }
+inline DualMW log(DualMW a) { return d2chain(a, std::log(a.v), 1 / a.v); }
+inline DualMW exp(DualMW a) { double e = std::exp(a.v); return d2chain(a, e, e); }
+inline DualMW sqrt(DualMW a) { double r = std::sqrt(a.v); return d2chain(a, r, 0.5 / r); }
+inline DualMW pow2(DualMW a) { return d2chain(a, std::pow(a.v, 2), 2 * a.v); }   // pow(a, 2) of the reference
+inline DualMW dmax(DualMW a, double b) { return a.v < b ? DualMW(b) : a; }           // std::max(a, b)
+inline double val(const DualMW& a) { return a.v; }
+
 // phs_cpu.cpp:452-531 (init_mapping_constants_cpu) for one branch, same expressions and
-// evaluation order.
-void map_constants(int id, double m, double w, double m_min, double m_tot, double sqrts, double a[3], double b[3]) {
+// evaluation order; the values are those of the reference, bit for bit, and dm / dw carry
+// their derivatives with respect to the map mass and width.
+void map_constants_d(int id, double m_, double w_, double m_min, double m_tot, double sqrts, DualMW a[3], DualMW b[3]) {
+  const DualMW m(m_, 1, 0), w(w_, 0, 1);
   double m_max = sqrts - m_tot + m_min;
   double msq_min = m_min * m_min, msq_max = m_max * m_max, s = sqrts * sqrts;
-  double msq0;
-  a[0] = a[1] = a[2] = 0;
+  DualMW msq0;
+  a[0] = a[1] = a[2] = DualMW(0);
   switch (id) {
     case MAP_NO:
       a[0] = 0; a[1] = msq_max - msq_min; a[2] = a[1] / s; break;
     case MAP_SCHANNEL:
       msq0 = m * m;
-      a[0] = std::atan((msq_min - msq0) / (m * w));
-      a[1] = std::atan((msq_max - msq0) / (m * w));
+      a[0] = atan((msq_min - msq0) / (m * w));
+      a[1] = atan((msq_max - msq0) / (m * w));
       a[2] = (a[1] - a[0]) * (m * w) / s; break;
     case MAP_RADIATION: case MAP_COLLINEAR: case MAP_INFRARED:
       msq0 = id == MAP_RADIATION ? w * w : m * m;
       a[0] = msq0;
-      a[1] = std::log((msq_max - msq_min) / msq0 + 1);
+      a[1] = log((msq_max - msq_min) / msq0 + 1);
       a[2] = a[1] / s; break;
     case MAP_TCHANNEL: case MAP_UCHANNEL:
       msq0 = m * m;
       a[0] = msq0;
-      a[1] = 2 * std::log((msq_max - msq_min) / (2 * msq0) + 1);
+      a[1] = 2 * log((msq_max - msq_min) / (2 * msq0) + 1);
       a[2] = a[1] / s; break;
     case MAP_STEP_E:
-      a[2] = std::max(2 * m * w / (msq_max - msq_min), 0.01);
-      a[1] = std::exp(-(m * m - msq_min) / (msq_max - msq_min) / a[2]);
-      a[0] = 1 - a[2] * std::log((1 + a[1] * std::exp(1 / a[2])) / (1 + a[1])); break;
+      a[2] = dmax(2 * m * w / (msq_max - msq_min), 0.01);
+      a[1] = exp(-(m * m - msq_min) / (msq_max - msq_min) / a[2]);
+      a[0] = 1 - a[2] * log((1 + a[1] * exp(1 / a[2])) / (1 + a[1])); break;
     case MAP_STEP_H:
       a[2] = (m * m - msq_min) / (msq_max - msq_min);
-      a[1] = std::max(std::pow(2 * m * w / (msq_max - msq_min), 2) / a[2], 0.000001);
-      a[0] = (1 + std::sqrt(1 + 4 * a[1] / (1 - a[2]))) / 2; break;
+      a[1] = dmax(pow2(2 * m * w / (msq_max - msq_min)) / a[2], 0.000001);
+      a[0] = (1 + sqrt(1 + 4 * a[1] / (1 - a[2]))) / 2; break;
     default: break;
   }
   switch (id) {
     case MAP_TCHANNEL: case MAP_UCHANNEL: case MAP_COLLINEAR:
-      b[0] = m * m / s; b[1] = std::log((b[0] + 1) / b[0]); b[2] = 0; break;
+      b[0] = m * m / s; b[1] = log((b[0] + 1) / b[0]); b[2] = 0; break;
     default:
-      b[0] = b[1] = b[2] = 0; break;
+      b[0] = b[1] = b[2] = DualMW(0); break;
   }
+}
+
+void map_constants(int id, double m, double w, double m_min, double m_tot, double sqrts, double a[3], double b[3]) {
+  DualMW ad[3], bd[3];
+  map_constants_d(id, m, w, m_min, m_tot, sqrts, ad, bd);
+  for (int k = 0; k < 3; k++) { a[k] = ad[k].v; b[k] = bd[k].v; }
 }
 
 }  // namespace
@@ -192,19 +224,23 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
         int j = ilog2(k + 1);
         double mj = sp.flv_masses[sp.n_in + j];
         cc.leaf = 1; cc.out = j; cc.msq_leaf = mj * mj;   // phs_cpu.cpp:544
+        cc.m_leaf = std::sqrt(cc.msq_leaf);
+        cc.node = -1;
         return "";
       }
       cc.leaf = 0;
+      cc.node = -1;
       if (!msq_kind(mid(k), &cc.kind)) {
         snprintf(msg, sizeof msg, "channel %d: branch %d has map id %d, which has no msq mapping", c + 1, k + 1, mid(k));
         return msg;
       }
-      double a[3], b[3];
+      DualMW a[3], b[3];
       double m = sp.map_masses[c * n_prt_out + k], w = sp.map_widths[c * n_prt_out + k];
-      map_constants(mid(k), m, w, ms[k], m_tot, sqrts, a, b);
+      map_constants_d(mid(k), m, w, ms[k], m_tot, sqrts, a, b);
       double m_max = sqrts - m_tot + ms[k];
       cc.xi = xi[k];
-      cc.a0 = a[0]; cc.a1 = a[1]; cc.a2 = a[2];
+      cc.a0 = a[0].v; cc.a1 = a[1].v; cc.a2 = a[2].v;
+      for (int q = 0; q < 3; q++) { cc.da_dm[q] = a[q].dm; cc.da_dw[q] = a[q].dw; }
       cc.m = m; cc.w = w;
       cc.m_min = ms[k];
       cc.msq_min = ms[k] * ms[k];
@@ -227,21 +263,24 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
     cd.m_tot = m_tot;
 
     // ---- visit order: descend into the smaller internal subtree, defer the other
-    int depth = 0, max_depth = 0;
-    std::function<std::string(int)> emit = [&](int b) -> std::string {
+    int depth = 0, max_depth = 0, n_tape = 0;
+    std::function<std::string(int, int, int)> emit = [&](int b, int parent, int slot) -> std::string {
       NodeCmd nd{};
       int k[2] = {D1[b] - 1, D2[b] - 1};
       if (!ct_kind(mid(b), &nd.ct_kind)) {
         snprintf(msg, sizeof msg, "channel %d: branch %d has map id %d, which has no cos-theta mapping", c + 1, b + 1, mid(b));
         return msg;
       }
-      double a[3], bb[3];
+      DualMW a[3], bb[3];
       double m = sp.map_masses[c * n_prt_out + b], w = sp.map_widths[c * n_prt_out + b];
-      map_constants(mid(b), m, w, ms[b], m_tot, sqrts, a, bb);
+      map_constants_d(mid(b), m, w, ms[b], m_tot, sqrts, a, bb);
       nd.x_phi = xphi[b];
       nd.map_id = mid(b);
+      nd.parent = parent;
+      nd.slot = slot;
       nd.m_max = (b == root) ? sqrts : sqrts - m_tot + ms[b];
-      nd.b0 = bb[0]; nd.b1 = bb[1];
+      nd.b0 = bb[0].v; nd.b1 = bb[1].v;
+      nd.db_dm[0] = bb[0].dm; nd.db_dm[1] = bb[1].dm;
       nd.m_min = ms[b];
       nd.m = m;
       for (int i = 0; i < 2; i++) {
@@ -258,23 +297,28 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
       } else if (in1) {
         nd.desc = 1;
       }
+      const int me = (int)out.nodes.size() - cd.node0;
+      nd.tape_slot = (in0 || in1) ? n_tape++ : -1;
       out.nodes.push_back(nd);
       out.node_branch.push_back(b);
       if (nd.push >= 0) { depth++; max_depth = std::max(max_depth, depth); }
       if (nd.desc >= 0) {
-        std::string err = emit(k[nd.desc]);
+        out.nodes[cd.node0 + me].k[nd.desc].node = (int)out.nodes.size() - cd.node0;
+        std::string err = emit(k[nd.desc], me, nd.desc);
         if (!err.empty()) return err;
       }
       if (nd.push >= 0) {
         depth--;
-        std::string err = emit(k[nd.push]);
+        out.nodes[cd.node0 + me].k[nd.push].node = (int)out.nodes.size() - cd.node0;
+        std::string err = emit(k[nd.push], me, nd.push);
         if (!err.empty()) return err;
       }
       return "";
     };
-    e = emit(root);
+    e = emit(root, -1, 0);
     if (!e.empty()) return e;
     cd.n_banks = max_depth;
+    cd.n_tape = n_tape;
     out.max_banks = std::max(out.max_banks, max_depth);
     if ((int)out.nodes.size() - cd.node0 != cd.n_nodes) return "internal error: node count";
     out.channels.push_back(cd);

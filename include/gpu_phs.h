@@ -132,10 +132,14 @@ typedef struct {
 int gpu_phs_plan_create(const gpu_phs_segment* segments, int n_segments);
 int gpu_phs_plan_destroy(int plan);
 /* Launch shape: workgroup size (64..256, multiple of 64), events per thread per tile
- * (1..16), persistent grid (0 = one workgroup per tile).  Changing block or ept rebuilds the
- * tile table.  order: 0 = segment order, 1 = tiles of the deepest trees first. */
-int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order);
-int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order);
+ * (1..16), persistent grid (0 = one workgroup per tile), tile order (0 = segment order,
+ * 1 = tiles of the deepest trees first), kernel variant: waves_per_eu = register budget for
+ * at least that many waves per SIMD (0 = compiler's choice, 3, 4, 5), lds_bank = 1 keeps
+ * the boosts of deferred subtrees in LDS instead of VGPRs.  Changing block, ept or order
+ * rebuilds the tile table. */
+int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds_bank);
+int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu,
+                            int* lds_bank);
 /* Forward pass of every segment in one kernel launch on `stream` (hipStream_t, 0 = null). */
 int gpu_phs_plan_run(int plan, void* stream);
 /* Forward pass with one launch per segment (the plain per-channel form). */
@@ -145,8 +149,30 @@ int gpu_phs_plan_run_graph(int plan, void* stream);
 /* Try launch shapes, time each with hipEvents (median of `reps`), keep the fastest.  Writes
  * a one-line summary to report (may be NULL).  Outputs are overwritten by the trial runs. */
 int gpu_phs_plan_autotune(int plan, void* stream, int reps, char* report, int report_len);
-/* Number of tiles and of workgroups of the single launch. */
+/* Number of tiles and segments of the plan, and the save-bank depth its trees need. */
 int gpu_phs_plan_info(int plan, int* n_tiles, int* n_segments, int* n_banks);
+
+/* Adjoint buffers of one segment (same order as the plan's segments). */
+typedef struct {
+  const double* g_p;      /* device [n_events][n_out][4] cotangents of the momenta, or NULL */
+  const double* g_f;      /* device [n_events] cotangents of the factor, or NULL */
+  double* g_x;            /* device [n_events][n_x]: d loss / d x (0 for events with ok = 0) */
+  double* loss;           /* device [n_events]: per-event loss (loss_mode 1), or NULL */
+} gpu_phs_adjoint_segment;
+
+/* Reverse-mode derivatives of every segment in one launch.  loss_mode 0: the loss is
+ * sum(g_p . p) + sum(g_f * factor) over ok events (a vector-Jacobian product); loss_mode 1:
+ * the loss is sum over ok events of factor * sum_j (p_jx^2 + p_jy^2) / s, and g_p, g_f are
+ * ignored.  Writes g_x, loss, the forward outputs when write_forward != 0, and gparam
+ * [n_params]: d loss / d M at 2 slot, d loss / d Gamma at 2 slot + 1, per (channel, branch)
+ * slot of gpu_phs_plan_param_map.  Trees up to 2 -> 6. */
+int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adjoint, int loss_mode, int write_forward,
+                         double* gparam, void* stream);
+/* 2 x the number of (channel, branch) parameter slots of a plan. */
+int gpu_phs_plan_n_params(int plan);
+/* For each slot: process handle, 0-based channel and bincode of the branch whose map mass and
+ * width it refers to.  Arrays hold at least n_params / 2 entries. */
+int gpu_phs_plan_param_map(int plan, int* process, int* channel, int* bincode, int n);
 
 #ifdef __cplusplus
 }

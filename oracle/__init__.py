@@ -76,8 +76,9 @@ def _tree_args(proc):
 
 
 def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
-    """Oracle forward pass: returns dict(p, factor, volume, ok, scale[, msq]); ``scale`` is the
-    componentwise forward-error scale |L1|...|L1||v| of every momentum component."""
+    """Oracle forward pass: returns dict(p, factor, volume, ok, scale, cond[, msq]); ``scale`` is
+    the componentwise forward-error scale |L1|...|L1||v| of every momentum component, ``cond``
+    the event's condition number (lambda near a threshold, sin theta of peaked maps)."""
     x = np.ascontiguousarray(x, np.float64)
     ch = np.ascontiguousarray(channels, np.int32)
     n = ch.shape[0]
@@ -88,16 +89,17 @@ def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
     ok = np.zeros(n, np.uint8)
     msq = np.zeros((n, proc.n_prt_out)) if want_msq else None
     scale = np.zeros((n, proc.n_out, 4))
+    cond = np.zeros(n)
     t = _tree_args(proc)
     e1 = None if E1 is None else np.ascontiguousarray(E1, np.float64)
     e2 = None if E2 is None else np.ascontiguousarray(E2, np.float64)
     rc = lib().oracle_forward(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
                               *[_ptr(a) for a in t], _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n),
                               _ptr(ch), _I(proc.n_x), _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok),
-                              _ptr(msq), _ptr(scale))
+                              _ptr(msq), _ptr(scale), _ptr(cond))
     if rc:
         raise ValueError(f"oracle_forward failed ({rc})")
-    out = dict(p=p, factor=f, volume=v, ok=ok.astype(bool), scale=scale)
+    out = dict(p=p, factor=f, volume=v, ok=ok.astype(bool), scale=scale, cond=cond)
     if want_msq:
         out["msq"] = msq
     return out

@@ -167,10 +167,18 @@ void init_constants(const Proc& P, MapPar<T>& M, int c, double sqrts) {
 }
 
 // phs_cpu.cpp:12-61 (mapping_msq_from_x_cpu).
+//
+// kappa (may be NULL) receives the condition number of the result: for every subtraction
+// a - b on the way, (|a| + |b|) / |a - b|, and for tan near +-pi/2 its own amplification.
 template <class T>
 void map_msq(int type, const T& x, double s, const T& m, const T& w, double msq_min, double msq_max,
-             const T* a, T* msq, T* factor) {
+             const T* a, T* msq, T* factor, double* kappa = nullptr) {
   T msq1, tmp, z;
+  double kap = 1;
+  auto cancel = [](double u, double v) {  // condition of u - v
+    double d = std::fabs(u - v);
+    return d > 0 ? (std::fabs(u) + std::fabs(v)) / d : 1e300;
+  };
   switch (type) {
     case MAP_NO:
       *msq = (1 - x) * msq_min + x * msq_max;
@@ -182,6 +190,12 @@ void map_msq(int type, const T& x, double s, const T& m, const T& w, double msq_
         tmp = tan(z);
         *msq = m * (m + w * tmp);
         *factor = a[2] * (1 + tmp * tmp);
+        {
+          double t = val(tmp), zz = std::fabs(val(z)) + std::fabs(val(a[0])) + std::fabs(val(a[1]));
+          double mm = std::fabs(val(m)), ww = std::fabs(val(w));
+          kap = std::max(kap, mm * ww * (1 + t * t) * zz / std::max(std::fabs(val(*msq)), 1e-300));
+          kap = std::max(kap, cancel(-mm * mm, ww * mm * t));
+        }
       } else {
         *msq = 0; *factor = 0;
       }
@@ -190,14 +204,17 @@ void map_msq(int type, const T& x, double s, const T& m, const T& w, double msq_
       msq1 = a[0] * exp(x * a[1]);
       *msq = msq1 - a[0] + msq_min;
       *factor = a[2] * msq1;
+      kap = (std::fabs(val(msq1)) + std::fabs(val(a[0])) + msq_min) / std::max(std::fabs(val(*msq)), 1e-300);
       break;
     case MAP_TCHANNEL: case MAP_UCHANNEL:
       if (val(x) < 0.5) {
         msq1 = a[0] * exp(x * a[1]);
         *msq = msq1 - a[0] + msq_min;
+        kap = (std::fabs(val(msq1)) + std::fabs(val(a[0])) + msq_min) / std::max(std::fabs(val(*msq)), 1e-300);
       } else {
         msq1 = a[0] * exp((1 - x) * a[1]);
         *msq = -(msq1 - a[0]) + msq_max;
+        kap = (std::fabs(val(msq1)) + std::fabs(val(a[0])) + msq_max) / std::max(std::fabs(val(*msq)), 1e-300);
       }
       *factor = a[2] * msq1;
       break;
@@ -205,14 +222,19 @@ void map_msq(int type, const T& x, double s, const T& m, const T& w, double msq_
       z = a[1] / (a[0] - x) - a[1] / a[0] + a[2] * x;
       *msq = z * msq_max + (1 - z) * msq_min;
       *factor = (a[1] / ((a[0] - x) * (a[0] - x)) + a[2]) * (msq_max - msq_min) / s;
+      kap = cancel(val(a[1] / (a[0] - x)), val(a[1] / a[0] - a[2] * x)) * std::fabs(val(z)) * (msq_max + msq_min) /
+            std::max(std::fabs(val(*msq)), 1e-300);
       break;
     case MAP_STEP_E:
       tmp = exp(-x * a[0] / a[2]) * (1 + a[1]);
       z = -a[2] * log(tmp - a[1]);
       *msq = z * msq_max + (1 - z) * msq_min;
       *factor = a[0] / (1 - a[1] / tmp) * (msq_max - msq_min) / s;
+      kap = cancel(val(tmp), val(a[1])) * std::max(1.0, std::fabs(val(a[2])) * (msq_max + msq_min) /
+                                                          std::max(std::fabs(val(*msq)), 1e-300));
       break;
   }
+  if (kappa) *kappa = kap;
 }
 
 // phs_cpu.cpp:112-146 (mapping_ct_from_x_cpu).
@@ -249,6 +271,7 @@ template <class T> struct Ev {
   std::vector<T> pd;        // p_decay per branch index
   std::vector<T> mom;       // 4 per branch index (prt[].p)
   std::vector<double> scale;  // 4 per branch index: forward-error scale of mom
+  double cond = 1;            // condition number of the event (see set_msq, set_angles)
 };
 
 // phs_cpu.cpp:174-234 (set_msq_cpu): post-order recursion, msq randoms consumed in
@@ -277,8 +300,10 @@ void set_msq(const Proc& P, const MapPar<T>& M, int c, int b, double m_tot, cons
     int ib = c * P.n_prt_out + b;
     double m_min = P.mass_sum[ib];
     m_max = sqrts - m_tot + m_min;
+    double kap = 1;
     map_msq(P.map_id[ib], x[*id_x], sqrts * sqrts, M.m[ib], M.w[ib], m_min * m_min, m_max * m_max,
-            &M.a[ib * 3], &E.p2[b], factor);
+            &M.a[ib * 3], &E.p2[b], factor, &kap);
+    E.cond = std::max(E.cond, kap);
     (*id_x)++;
     // The reference tests a local that is always 0 here (phs_cpu.cpp:202,210), so this
     // branch is always taken.
@@ -289,6 +314,13 @@ void set_msq(const Proc& P, const MapPar<T>& M, int c, int b, double m_tot, cons
     T this_msq = E.p2[b], msq1 = E.p2[k1], msq2 = E.p2[k2];
     T m1 = sqrt(msq1), m2 = sqrt(msq2), m = sqrt(this_msq);
     T lda = (this_msq - msq1 - msq2) * (this_msq - msq1 - msq2) - 4 * msq1 * msq2;
+    // Condition number of lambda: the terms it is formed from over lambda itself.  Near a
+    // threshold (lambda -> 0) an ulp of difference in any msq moves sqrt(lambda), the factor
+    // and the decay momenta by ~eps * cond.
+    if (val(lda) > 0) {
+      double t = std::fabs(val(this_msq)) + std::fabs(val(msq1)) + std::fabs(val(msq2));
+      E.cond = std::max(E.cond, t * t / val(lda));
+    }
     if (val(lda) > 0 && val(m) > val(m1 + m2) && val(m) <= m_max) {
       E.pd[k1] = sqrt(lda) / (2 * m);
       E.pd[k2] = -E.pd[k1];
@@ -328,6 +360,13 @@ void set_angles(const Proc& P, const MapPar<T>& M, int c, int b, const T* x, int
     T ct, st, f;
     map_ct(P.map_id[ib], xx, &M.b[ib * 3], &ct, &st, &f);
     *factor *= f;
+    // sin theta = sqrt(1 - ct^2) of the peaked map loses accuracy as 1 / sin theta, and
+    // ct = tmp - b0 - 1 cancels near ct = 0
+    if (val(st) > 0 && P.map_id[ib] != MAP_NO && val(f) != 1.0) {
+      E.cond = std::max(E.cond, 1.0 / std::fabs(val(st)));
+      double c = std::fabs(val(ct));
+      if (c > 0) E.cond = std::max(E.cond, (1 + std::fabs(val(M.b[ib * 3])) + 1) / c);
+    }
     T L1[4][4];
     L1[0][0] = gamma; L1[0][1] = -bg * st;    L1[0][2] = 0;  L1[0][3] = bg * ct;
     L1[1][0] = 0;     L1[1][1] = ct * cp;     L1[1][2] = -sp; L1[1][3] = st * cp;
@@ -375,9 +414,10 @@ bool build_proc(Proc& P, int n_in, int n_out, int n_channels, const int* d1, con
 // then the lab-frame boost of the final leaves, phs.cu:1286-1301, when E1 != E2.
 template <class T>
 bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg_lab, const T* x,
-               Ev<T>& E, T* factor, T* volume, T* p_out, double* msq_out, double* scale_out) {
+               Ev<T>& E, T* factor, T* volume, T* p_out, double* msq_out, double* scale_out, double* cond_out) {
   int n = P.n_prt;
   E.p2.assign(n, T(0)); E.pd.assign(n, T(0)); E.mom.assign(4 * n, T(0)); E.scale.assign(4 * n, 0.0);
+  E.cond = 1;
   // phs_cpu.cpp:541-547 (init_msq_cpu): leaf msq = flavour mass squared.
   for (int i = 0, p = 1; i < P.n_out; i++, p *= 2) E.p2[p - 1] = P.flv[P.n_in + i] * P.flv[P.n_in + i];
   bool ok = true;
@@ -411,6 +451,7 @@ bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg
       }
     }
   }
+  if (cond_out) *cond_out = E.cond;
   return ok;
 }
 
@@ -464,13 +505,14 @@ void oracle_map_ct(int type, double x, const double* b, double* ct, double* st, 
 //   channels [n_events] 0-based; x [n_events][n_x] event-major.
 //   Outputs: p [n_events][n_out][4], factors, volumes [n_events], oks [n_events] (0/1),
 //   msq [n_events][n_prt_out] (may be NULL), scale [n_events][n_out][4] the forward-error
-//   scale of every momentum component (may be NULL; see set_angles).  Non-ok events keep p = 0 and whatever factor
+//   scale of every momentum component (may be NULL; see set_angles), cond [n_events] the
+//   event's condition number (may be NULL; see set_msq).  Non-ok events keep p = 0 and whatever factor
 //   and volume the early return left, as the reference does.  Returns 0 on success.
 int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
                    const int* map_ids, const double* map_m, const double* map_w, const double* flv,
                    double sqrts, const double* E1, const double* E2, long n_events, const int* channels,
                    int n_x, const double* x, double* p, double* factors, double* volumes,
-                   unsigned char* oks, double* msq, double* scale) {
+                   unsigned char* oks, double* msq, double* scale, double* cond) {
   Proc P;
   if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
   if (n_x < 3 * n_out - 4) return 2;
@@ -499,7 +541,7 @@ int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int
     if (scale) std::memset(&scale[4 * n_out * i], 0, 4 * n_out * sizeof(double));
     bool ok = one_event<double>(P, M, c, s_ev, bg_lab, &x[(size_t)n_x * i], E, &f, &v, &p[4 * n_out * i],
                                 msq ? &msq[(size_t)P.n_prt_out * i] : nullptr,
-                                scale ? &scale[4 * n_out * i] : nullptr);
+                                scale ? &scale[4 * n_out * i] : nullptr, cond ? &cond[i] : nullptr);
     factors[i] = f; volumes[i] = v; oks[i] = ok ? 1 : 0;
   }
   return 0;
@@ -540,7 +582,7 @@ int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, cons
     }
     for (auto& q : pd) q = Dual(0.0);
     Dual f, v;
-    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr);
+    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr, nullptr);
     oks[i] = ok ? 1 : 0;
     factors[i] = ok ? f.v : 0;
     for (int j = 0; j < 4 * n_out; j++) {

@@ -1,0 +1,175 @@
+"""The adjoint kernel against the oracle's forward-mode dual numbers and central finite
+differences, and the torch.autograd wrapper."""
+import numpy as np
+import pytest
+
+import oracle
+from gpu_phs_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(proc, n, seed, lo=0.02):
+    import torch
+    import gpu_phs_amd as G
+    rng = np.random.default_rng(seed)
+    counts = np.full(proc.n_channels, n // proc.n_channels)
+    counts[: n - counts.sum()] += 1
+    ch = np.repeat(np.arange(proc.n_channels), counts).astype(np.int32)
+    x = lo + (1 - 2 * lo) * rng.random((n, proc.n_x))
+    ph = G.ProcessHandle(proc)
+    xt = torch.from_numpy(x).cuda()
+    bufs, segs = [], []
+    start = 0
+    for c, k in enumerate(counts):
+        b = G.EventBuffers.allocate(xt[start:start + k], proc.n_out)
+        bufs.append(b)
+        segs.append((ph, c, b))
+        start += k
+    plan = G.Plan(segs)
+    return rng, ch, x, ph, plan, bufs, counts
+
+
+def _dual_grads(proc, x, ch, gp=None, gf=None, loss_mode=1):
+    """d loss / d x from the oracle's dual numbers (K = n_x tangents)."""
+    n = len(ch)
+    K = proc.n_x
+    xt = np.zeros((n, K, K))
+    for k in range(K):
+        xt[:, k, k] = 1
+    d = oracle.forward_dual(proc, x, ch, xt, None, None, K)
+    ref = oracle.forward(proc, x, ch)
+    s = proc.sqrts ** 2
+    if loss_mode == 1:
+        S2 = (ref["p"][..., 1] ** 2 + ref["p"][..., 2] ** 2).sum(1)
+        dS = 2 * (ref["p"][..., 1, None] * d["p_tan"][..., 1, :] + ref["p"][..., 2, None] * d["p_tan"][..., 2, :]).sum(1)
+        g = d["factor_tan"] * (S2 / s)[:, None] + ref["factor"][:, None] * dS / s
+    else:
+        g = (gp[..., None] * d["p_tan"]).sum((1, 2)) + gf[:, None] * d["factor_tan"]
+    g[~ref["ok"]] = 0
+    return g, ref
+
+
+def _dual_param_grads(proc, x, ch, ok, slots, gp=None, gf=None, loss_mode=1):
+    """d loss / d (M, Gamma) of every (channel, bincode) slot from dual numbers."""
+    s = proc.sqrts ** 2
+    out = np.zeros((len(slots), 2))
+    ref = oracle.forward(proc, x, ch)
+    for i, (c, bc) in enumerate(slots):
+        sel = (ch == c) & ok
+        if not sel.any():
+            continue
+        for w in (0, 1):
+            mt = np.zeros((proc.n_channels, proc.n_prt_out, 1))
+            wt = np.zeros_like(mt)
+            (mt if w == 0 else wt)[c, bc - 1, 0] = 1
+            dd = oracle.forward_dual(proc, x[sel], ch[sel], None, mt, wt, 1)
+            if loss_mode == 1:
+                p = ref["p"][sel]
+                S2 = (p[..., 1] ** 2 + p[..., 2] ** 2).sum(1)
+                dS = 2 * (p[..., 1] * dd["p_tan"][..., 1, 0] + p[..., 2] * dd["p_tan"][..., 2, 0]).sum(1)
+                out[i, w] = (dd["factor_tan"][:, 0] * S2 / s + ref["factor"][sel] * dS / s).sum()
+            else:
+                out[i, w] = ((gp[sel] * dd["p_tan"][..., 0]).sum((1, 2)) + gf[sel] * dd["factor_tan"][:, 0]).sum()
+    return out
+
+
+@pytest.mark.parametrize("name", ["c1", "c0", "c2", "2to6"])
+@pytest.mark.parametrize("loss_mode", [1, 0])
+def test_adjoint_matches_dual_numbers(name, loss_mode):
+    import torch
+    proc = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6}[name]()
+    n = 3000 if name != "c2" else 6000
+    rng, ch, x, ph, plan, bufs, counts = _setup(proc, n, 17)
+    gp = rng.standard_normal((n, proc.n_out, 4)) if loss_mode == 0 else None
+    gf = rng.standard_normal(n) if loss_mode == 0 else None
+    gx = [torch.empty_like(b.x) for b in bufs]
+    loss = [torch.empty(b.x.shape[0], dtype=torch.float64, device="cuda") for b in bufs]
+    kw = {}
+    if loss_mode == 0:
+        gpt, gft = torch.from_numpy(gp).cuda(), torch.from_numpy(gf).cuda()
+        kw = dict(g_p=list(torch.split(gpt, list(counts))), g_f=list(torch.split(gft, list(counts))))
+    gparam = plan.adjoint(gx, loss=loss, loss_mode=loss_mode, **kw)
+    torch.cuda.synchronize()
+    g_gpu = torch.cat(gx).cpu().numpy()
+    g_ref, ref = _dual_grads(proc, x, ch, gp, gf, loss_mode)
+    ok = ref["ok"]
+    scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(g_gpu - g_ref) / scale
+    print(name, loss_mode, "max rel err", err[ok].max())
+    assert err[ok].max() < 1e-9
+    assert (g_gpu[~ok] == 0).all()
+    # forward outputs written by the adjoint kernel = the oracle's
+    okg = torch.cat([b.ok for b in bufs]).cpu().numpy().astype(bool)
+    assert np.array_equal(okg, ok)
+    if loss_mode == 1:
+        l_gpu = torch.cat(loss).cpu().numpy()
+        S2 = (ref["p"][..., 1] ** 2 + ref["p"][..., 2] ** 2).sum(1)
+        assert np.allclose(l_gpu[ok], (ref["factor"] * S2 / proc.sqrts ** 2)[ok], rtol=1e-12)
+    # parameter gradients
+    pr, pch, pbc = plan.param_map()
+    slots = list(zip(pch.tolist(), pbc.tolist()))
+    gpar_ref = _dual_param_grads(proc, x, ch, ok, slots, gp, gf, loss_mode)
+    gpar = gparam.cpu().numpy().reshape(-1, 2)[: len(slots)]
+    sc = np.abs(gpar_ref).max() + 1e-300
+    print(name, loss_mode, "param max rel err", np.abs(gpar - gpar_ref).max() / sc)
+    assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
+
+
+def test_adjoint_central_differences_c4():
+    """The C4 gradient on the c1 trees against central differences (h = 1e-6) of the loss."""
+    import torch
+    proc = W.process_c1()
+    n = 600
+    rng, ch, x, ph, plan, bufs, counts = _setup(proc, n, 5, lo=0.05)
+    gx = [torch.empty_like(b.x) for b in bufs]
+    plan.adjoint(gx, loss_mode=1)
+    g = torch.cat(gx).cpu().numpy()
+    s = proc.sqrts ** 2
+
+    def loss(xx):
+        r = oracle.forward(proc, xx, ch)
+        S2 = (r["p"][..., 1] ** 2 + r["p"][..., 2] ** 2).sum(1)
+        return np.where(r["ok"], r["factor"] * S2 / s, 0.0), r["ok"]
+
+    h = 1e-6
+    _, ok0 = loss(x)
+    for k in range(proc.n_x):
+        xp, xm = x.copy(), x.copy()
+        xp[:, k] += h
+        xm[:, k] -= h
+        lp, okp = loss(xp)
+        lm, okm = loss(xm)
+        fd = (lp - lm) / (2 * h)
+        good = ok0 & okp & okm
+        sc = np.abs(fd[good]).max()
+        assert np.allclose(g[good, k], fd[good], rtol=1e-4, atol=1e-6 * sc), k
+
+
+def test_autograd_function_and_dot_product():
+    import torch
+    from gpu_phs_amd.autograd import PhaseSpace
+    proc = W.process_c1()
+    n = 2000
+    counts = [700, 600, 700]
+    rng = np.random.default_rng(9)
+    x = torch.from_numpy(0.02 + 0.96 * rng.random((n, proc.n_x))).cuda().requires_grad_(True)
+    M = torch.from_numpy(proc.map_masses.copy()).requires_grad_(True)
+    Gm = torch.from_numpy(proc.map_widths.copy()).requires_grad_(True)
+    ps = PhaseSpace(proc, counts)
+    p, f, v, ok = ps(x, M, Gm)
+    gp = torch.from_numpy(rng.standard_normal((n, proc.n_out, 4))).cuda()
+    gf = torch.from_numpy(rng.standard_normal(n)).cuda()
+    okf = ok.double()
+    L = ((p * gp).sum((1, 2)) * okf).sum() + (f * gf * okf).sum()
+    L.backward()
+    ch = np.repeat(np.arange(3), counts).astype(np.int32)
+    g_ref, ref = _dual_grads(proc, x.detach().cpu().numpy(), ch, gp.cpu().numpy(), gf.cpu().numpy(), 0)
+    okn = ok.cpu().numpy()
+    err = np.abs(x.grad.cpu().numpy() - g_ref) / (np.abs(g_ref).max(1, keepdims=True) + 1e-300)
+    assert err[okn].max() < 1e-9
+    # BW mass gradient of channel 0, branch 3 = (12): against dual numbers
+    bw = [(0, 3), (0, 12), (1, 9), (1, 6), (2, 12)]
+    ref_par = _dual_param_grads(proc, x.detach().cpu().numpy(), ch, okn, bw, gp.cpu().numpy(), gf.cpu().numpy(), 0)
+    got = np.array([[M.grad[c, b - 1].item(), Gm.grad[c, b - 1].item()] for c, b in bw])
+    assert np.allclose(got, ref_par, rtol=1e-9, atol=1e-12 * np.abs(ref_par).max())

@@ -1,0 +1,144 @@
+"""The Whizard-facing C ABI on the GPU: the reference's call sequence (whizard_gpu_phs.cpp)
+with host arrays, checked against the oracle and against Whizard's own momenta."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import gpu_phs_amd as G
+import oracle
+from gpu_phs_amd import workloads as W
+from parity import assert_parity, compare
+from refio import read_ref
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(v))
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def setup_module_from(proc, block=256):
+    L = G.lib()
+    nc = proc.n_channels
+    L.c_whizard_set_particle_structure(_i(nc), _i(nc), _i(1), _i(proc.n_x), _i(proc.n_in), _i(proc.n_out))
+    flv = np.ascontiguousarray(proc.flv_masses)
+    L.c_whizard_set_flavors(_p(flv), _p(np.zeros_like(flv)))
+    L.c_whizard_init_mappings()
+    L.c_whizard_init_tree_structures()
+    keep = []
+    for c in range(nc):
+        arrs = [np.ascontiguousarray(a[c]) for a in (proc.map_ids, proc.map_masses, proc.map_widths,
+                                                     proc.d1, proc.d2, proc.has_children)]
+        arrs.append(np.zeros(proc.n_prt, np.int32))
+        keep.append(arrs)
+        L.c_whizard_fill_mapping(_i(c + 1), _p(arrs[0]), _p(arrs[1]), _p(arrs[2]))
+        L.c_whizard_fill_tree_structure(_i(c + 1), _p(arrs[3]), _p(arrs[4]), _p(arrs[5]), _p(arrs[6]))
+    L.c_whizard_set_threads(_i(block), _i(block), _i(block))
+    L.c_whizard_init_gpu_phs()
+    assert L.gpu_phs_last_error() == 0, L.gpu_phs_last_error_message()
+    return L
+
+
+def gen(L, proc, x, counts, E1, E2):
+    n = x.shape[0]
+    limits = (ctypes.c_int * len(counts))(*np.cumsum(counts).tolist())
+    L.c_whizard_init_channel_ids(_i(n), _i(len(counts)), limits)
+    assert L.gpu_phs_last_error() == 0
+    nc = proc.n_channels
+    e1 = np.full(n, E1)
+    e2 = np.full(n, E2)
+    factors = np.full(n * nc, -1.0)
+    volumes = np.zeros(n)
+    oks = np.zeros(n, np.bool_)
+    p = np.zeros((n, proc.n_out, 4))
+    x_out = np.full(n * nc * proc.n_x, -1.0)
+    flag = ctypes.c_bool(True)
+    x = np.ascontiguousarray(x)
+    L.c_whizard_gen_phs_from_x_gpu(_p(e1), _p(e2), ctypes.byref(flag), _i(n), _i(nc), _i(proc.n_x), _p(x),
+                                   _p(factors), _p(volumes), _p(oks), _p(p), _p(x_out))
+    assert L.gpu_phs_last_error() == 0, L.gpu_phs_last_error_message()
+    return p, factors.reshape(n, nc), volumes, oks, x_out.reshape(nc, n, proc.n_x)
+
+
+def test_whizard_abi_matches_oracle():
+    proc = W.process_c1()
+    L = setup_module_from(proc)
+    counts = [0, 700, 1300]          # an empty channel first, as Whizard's batches allow
+    n = sum(counts)
+    rng = np.random.default_rng(21)
+    x = rng.random((n, proc.n_x))
+    p, factors, volumes, oks, x_out = gen(L, proc, x, counts, 250.0, 250.0)
+    ch = np.repeat(np.arange(3), counts).astype(np.int32)
+    ref = oracle.forward(proc, x, ch)
+    own = factors[np.arange(n), ch]
+    r = compare(dict(p=p, factor=own, volume=volumes, ok=oks), ref)
+    print(r)
+    assert_parity(r)
+    others = factors.copy()
+    others[np.arange(n), ch] = 0
+    assert (others == 0).all()
+    for c in range(3):
+        sel = ch == c
+        assert np.array_equal(x_out[c][sel], x[sel]) and (x_out[c][~sel] == 0).all()
+    # the device pointer of the last call's momenta
+    addr = np.zeros(1, np.float64)
+    L.c_whizard_get_momentum_device_pointer(_p(addr))
+    assert addr.view(np.uint64)[0] != 0
+    L.c_whizard_finalize_gpu_phs()
+
+
+def test_whizard_abi_interior_empty_channel():
+    proc = W.process_c1()
+    L = setup_module_from(proc, block=64)
+    counts = [500, 0, 321]
+    n = sum(counts)
+    x = np.random.default_rng(5).random((n, proc.n_x))
+    p, factors, volumes, oks, _ = gen(L, proc, x, counts, 250.0, 250.0)
+    ch = np.repeat(np.arange(3), counts).astype(np.int32)
+    ref = oracle.forward(proc, x, ch)
+    r = compare(dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks), ref)
+    assert_parity(r)
+    L.c_whizard_finalize_gpu_phs()
+
+
+def test_whizard_abi_asymmetric_beams_lab_boost():
+    proc = W.process_c1()
+    L = setup_module_from(proc)
+    counts = [400, 400, 400]
+    n = sum(counts)
+    x = np.random.default_rng(6).random((n, proc.n_x))
+    E1, E2 = 300.0, 200.0
+    p, factors, volumes, oks, _ = gen(L, proc, x, counts, E1, E2)
+    ch = np.repeat(np.arange(3), counts).astype(np.int32)
+    ref = oracle.forward(proc, x, ch, E1=np.full(n, E1), E2=np.full(n, E2))
+    r = compare(dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks), ref)
+    assert_parity(r)
+    tot = p[oks].sum(axis=1)
+    assert np.allclose(tot[:, 0], E1 + E2) and np.allclose(tot[:, 3], E1 - E2, atol=1e-9)
+    L.c_whizard_finalize_gpu_phs()
+
+
+@pytest.mark.parametrize("name", ["ttH_head.ref", "bwbw_head.ref"])
+def test_reference_samples_on_gpu(name):
+    """Whizard's own momenta (9-decimal randoms) from the reference's sample files, through the
+    Whizard ABI on the GPU; trees with friends are left out (no CPU counterpart)."""
+    proc, ev = read_ref(os.path.join(GOLDEN, name))
+    L = setup_module_from(proc)
+    ch = ev["channel"]
+    order = np.argsort(ch, kind="stable")
+    counts = np.bincount(ch, minlength=proc.n_channels)
+    p, factors, volumes, oks, _ = gen(L, proc, ev["x"][order], counts, proc.sqrts / 2, proc.sqrts / 2)
+    sel = ev["hasfriends"][ch[order]] == 0
+    assert oks[sel].all()
+    pw = ev["p"][order][:, proc.n_in:, :]
+    dev = np.abs(p[sel] - pw[sel]).max() / proc.sqrts
+    assert dev < 1e-6, dev
+    assert np.allclose(volumes[sel], ev["volume"][order][sel], rtol=1e-12)
+    L.c_whizard_finalize_gpu_phs()
