@@ -79,6 +79,25 @@ def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None):
     return res, plan, segs, wl
 
 
+def run_adjoint(steps, warmup, torch, G, W):
+    """c4: forward + reverse mode on the c1 trees, 1e6 events, loss = sum ok * factor * sum pT^2 / s.
+    One adjoint launch produces the forward outputs and d loss / d (x, M, Gamma)."""
+    wl = W.workload("c4")
+    handles, segs = make_inputs(wl, torch, G)
+    plan = G.Plan(segs)
+    plan.autotune(reps=3)
+    gx = [torch.empty_like(b.x) for _, _, b in segs]
+    loss = [torch.empty(b.x.shape[0], dtype=torch.float64, device="cuda") for _, _, b in segs]
+    gparam = plan.adjoint(gx, loss=loss, loss_mode=1)
+    fw_wall, fw_ev = time_steps(plan.run_graph, steps, warmup, torch)
+    ad_wall, ad_ev = time_steps(lambda: plan.adjoint(gx, loss=loss, loss_mode=1, gparam=gparam), steps, warmup, torch)
+    n = wl.n_events
+    return dict(n_events=n, forward_ev_s=n * steps / fw_wall, fwd_adjoint_ev_s=n * steps / ad_wall,
+                forward_ms=fw_ev / steps * 1e3, fwd_adjoint_ms=ad_ev / steps * 1e3,
+                fwd_adjoint_over_forward=ad_wall / fw_wall,
+                loss=float(sum(l.sum().item() for l in loss)))
+
+
 def cpu_baseline(wl_name, W):
     """The reference's own phs_cpu.cpp (oracle/_ref build) on one host core, over a bounded
     sample of the workload's events (repeated until ~10 s of CPU work)."""
@@ -90,7 +109,7 @@ def cpu_baseline(wl_name, W):
     x = np.random.default_rng(wl.seed).random((ch.shape[0], proc.n_x))
     kind = "reference" if oracle.ref_available() else "port"
     total_t, total_n, reps = 0.0, 0, 0
-    while total_t < 10.0 and reps < 5:
+    while total_t < 10.0 and reps < 40:
         if kind == "reference":
             sec, _ = oracle.ref_time(proc, x, ch)
         else:
@@ -165,6 +184,12 @@ def main():
             log(f"workload {name}: {r['batched_ev_s']:.4g} ev/s batched, {r['plain_ev_s']:.4g} plain")
             del p2, s2
             torch.cuda.empty_cache()
+    if not args.no_extra:
+        log("workload c4 (forward + adjoint): setup")
+        extra["c4"] = run_adjoint(max(5, args.steps), 2, torch, G, W)
+        log(f"workload c4: {extra['c4']['fwd_adjoint_ev_s']:.4g} ev/s fwd+adjoint, "
+            f"{extra['c4']['fwd_adjoint_over_forward']:.2f}x forward")
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -134,15 +134,19 @@ def test_adjoint_central_differences_c4():
 
     h = 1e-6
     _, ok0 = loss(x)
+    fds, goods = [], []
     for k in range(proc.n_x):
         xp, xm = x.copy(), x.copy()
         xp[:, k] += h
         xm[:, k] -= h
         lp, okp = loss(xp)
         lm, okm = loss(xm)
-        fd = (lp - lm) / (2 * h)
-        good = ok0 & okp & okm
-        sc = np.abs(fd[good]).max()
+        fds.append((lp - lm) / (2 * h))
+        goods.append(ok0 & okp & okm)
+    # one absolute scale for all randoms: the root's phi leaves sum pT^2 unchanged, so its
+    # derivative is zero and its difference quotient is rounding noise (~eps * loss / h)
+    sc = max(np.abs(fd[good]).max() for fd, good in zip(fds, goods))
+    for k, (fd, good) in enumerate(zip(fds, goods)):
         assert np.allclose(g[good, k], fd[good], rtol=1e-4, atol=1e-6 * sc), k
 
 
