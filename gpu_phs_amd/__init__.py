@@ -169,7 +169,7 @@ class Plan:
     alive and keep their storage while the plan is used.
     """
 
-    def __init__(self, segments, block=256, ept=1, grid=0, order=0, waves_per_eu=0, lds_bank=0):
+    def __init__(self, segments, **config):
         arr = (_Segment * len(segments))()
         self._keep = []
         for i, (ph, ch, buf) in enumerate(segments):
@@ -182,11 +182,19 @@ class Plan:
                               buf.factor.data_ptr(), buf.volume.data_ptr(), buf.ok.data_ptr())
             self._keep.append((ph, buf))
         self.handle = _check(lib().gpu_phs_plan_create(arr, len(segments)), "plan_create")
-        self.set_config(block, ept, grid, order, waves_per_eu, lds_bank)
+        if config:
+            self.set_config(**config)
 
-    def set_config(self, block=256, ept=1, grid=0, order=0, waves_per_eu=0, lds_bank=0):
-        _check(lib().gpu_phs_plan_set_config(self.handle, block, ept, grid, order, waves_per_eu, lds_bank),
-               "plan_set_config")
+    def set_config(self, **config):
+        """Launch configuration: block, ept, grid, order, waves_per_eu, lds_bank; keys left out
+        keep their current values (the library's defaults for a new plan)."""
+        unknown = set(config) - {"block", "ept", "grid", "order", "waves_per_eu", "lds_bank"}
+        if unknown:
+            raise TypeError(f"unknown launch parameters {sorted(unknown)}")
+        c = self.config()
+        c.update(config)
+        _check(lib().gpu_phs_plan_set_config(self.handle, c["block"], c["ept"], c["grid"], c["order"],
+                                             c["waves_per_eu"], c["lds_bank"]), "plan_set_config")
 
     def config(self):
         v = [ctypes.c_int() for _ in range(6)]

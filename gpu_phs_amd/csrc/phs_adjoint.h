@@ -11,6 +11,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "phs_math.h"
 #include "phs_types.h"
 
 namespace gphs {
@@ -37,14 +38,14 @@ GPHS_HD MapEval msq_eval(const ChildCmd& c, double x, double s) {
       break;
     case MK_BREIT_WIGNER: {
       double z = (1 - x) * c.a0 + x * c.a1;
-      r.t = tan(z);
+      r.t = math::tan_radians(z);
       r.in = (-kPi / 2 < z) && (z < kPi / 2);
       r.msq = r.in ? c.m * (c.m + c.w * r.t) : 0.0;
       r.f = r.in ? c.a2 * (1 + r.t * r.t) : 0.0;
       break;
     }
     case MK_EXP: {
-      r.t = exp(x * c.a1);
+      r.t = math::exp_bounded(x * c.a1);
       double m1 = c.a0 * r.t;
       r.msq = m1 - c.a0 + c.msq_min;
       r.f = c.a2 * m1;
@@ -53,7 +54,7 @@ GPHS_HD MapEval msq_eval(const ChildCmd& c, double x, double s) {
     case MK_TU: {
       bool lo = x < 0.5;
       r.aux = lo ? 1.0 : -1.0;
-      r.t = exp((lo ? x : (1 - x)) * c.a1);
+      r.t = math::exp_bounded((lo ? x : (1 - x)) * c.a1);
       double m1 = c.a0 * r.t;
       r.msq = lo ? (m1 - c.a0 + c.msq_min) : (-(m1 - c.a0) + c.msq_max);
       r.f = c.a2 * m1;
@@ -67,9 +68,9 @@ GPHS_HD MapEval msq_eval(const ChildCmd& c, double x, double s) {
       break;
     }
     default: {  // MK_STEP_E
-      r.aux = exp(-x * c.a0 / c.a2);
+      r.aux = math::exp_bounded(-x * c.a0 / c.a2);
       r.t = r.aux * (1 + c.a1);
-      double z = -c.a2 * log(r.t - c.a1);
+      double z = -c.a2 * math::log_bounded(r.t - c.a1);
       r.msq = z * c.msq_max + (1 - z) * c.msq_min;
       r.f = c.a0 / (1 - c.a1 / r.t) * (c.msq_max - c.msq_min) / s;
       break;
@@ -137,7 +138,7 @@ GPHS_HD void msq_eval_adj(const ChildCmd& c, double x, double s, const MapEval& 
       const double gtmp = gv + gf * (-cs * c.a0 * c.a1 / (v * v));
       ga1 = -gv + gf * (cs * c.a0 * tmp / (v * v)) + gtmp * e;
       ga0 = gf * cs * tmp / v + gtmp * tmp * (-x / c.a2);
-      ga2 = gz * (-log(v)) + gtmp * tmp * x * c.a0 / (c.a2 * c.a2);
+      ga2 = gz * (-math::log_bounded(v)) + gtmp * tmp * x * c.a0 / (c.a2 * c.a2);
       gx = gtmp * tmp * (-c.a0 / c.a2);
       break;
     }
@@ -161,7 +162,7 @@ GPHS_HD CtEval ct_eval(int kind, double x, double b0, double b1) {
     r.in = true;
   } else {
     r.lo = x < 0.5;
-    r.e = exp((r.lo ? 2 * x : 2 * (1 - x)) * b1);
+    r.e = math::exp_bounded((r.lo ? 2 * x : 2 * (1 - x)) * b1);
     r.tmp = b0 * r.e;
     double c = r.lo ? (r.tmp - b0 - 1) : (-(r.tmp - b0) + 1);
     r.in = c >= -1 && c <= 1;
@@ -247,7 +248,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     F *= f[0] * f[1] * (sl / cur_msq);
     const double bg = m > 0 ? cur_p / m : 0.0, g = sqrt(1 + bg * bg);
     double sp, cp;
-    sincos(xr[nc.x_phi] * kTwoPi, &sp, &cp);
+    math::sincos_turns(xr[nc.x_phi], sp, cp);
     const CtEval ce = ct_eval(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1);
     F *= ce.f;
     compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
@@ -325,7 +326,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     const double bg = m > 0 ? p0 / m : 0.0, g = sqrt(1 + bg * bg);
     const double xphi = xr[nc.x_phi], xct = xr[nc.x_phi + 1];
     double sp, cp;
-    sincos(xphi * kTwoPi, &sp, &cp);
+    math::sincos_turns(xphi, sp, cp);
     const CtEval ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
     double Ma[16];
     for (int q = 0; q < 16; q++) Ma[q] = P[q];

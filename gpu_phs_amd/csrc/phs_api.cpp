@@ -115,6 +115,8 @@ int plan_build(PlanObj& pl) {
     sd.channel = ci;
     sd.n_events = us.n_events;
     sd.x = us.x; sd.p = us.p; sd.factor = us.factors; sd.volume = us.volumes; sd.ok = us.oks;
+    sd.align16 = ((reinterpret_cast<uintptr_t>(us.x) % 16) == 0 ? 1 : 0) |
+                 ((reinterpret_cast<uintptr_t>(us.p) % 16) == 0 ? 2 : 0);
     pl.segs.push_back(sd);
   }
   if (pl.n_banks > 3) return set_error(GPU_PHS_E_TREE, "tree needs more than 3 save banks");
@@ -174,6 +176,11 @@ int plan_build(PlanObj& pl) {
   pl.dp.nodes = (const NodeCmd*)(base + o_nodes);
   pl.dp.n_tiles = (int)pl.tiles.size();
   pl.dp.n_banks = pl.n_banks;
+  pl.dp.nx_max = pl.dp.nout_max = 0;
+  for (const auto& c : pl.chans) {
+    pl.dp.nx_max = std::max(pl.dp.nx_max, (int)c.n_x);
+    pl.dp.nout_max = std::max(pl.dp.nout_max, (int)c.n_out);
+  }
   return ok_status();
 }
 
@@ -271,6 +278,12 @@ int gpu_phs_plan_create(const gpu_phs_segment* segments, int n_segments) {
   pl->user.assign(segments, segments + n_segments);
   rc = plan_build(*pl);
   if (rc) return rc;
+  // default workgroup: the largest whose staging tables fit the LDS of a workgroup
+  while (pl->lc.block > 64 && forward_lds_bytes(pl->dp, pl->lc) > forward_lds_limit()) {
+    pl->lc.block -= 64;
+    rc = plan_build(*pl);
+    if (rc) return rc;
+  }
   int h = register_plan(std::move(pl));
   ok_status();
   return h;
@@ -291,6 +304,11 @@ int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, i
     return set_error(GPU_PHS_E_ARG,
                      "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4,5}, "
                      "lds_bank in {0,1}");
+  LaunchConfig want = pl->lc;
+  want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds_bank = lds_bank;
+  if (forward_lds_bytes(pl->dp, want) > forward_lds_limit())
+    return set_error(GPU_PHS_E_ARG, "the staging tables of this block size (and LDS save banks) exceed the LDS a "
+                                    "workgroup may use on this device; choose a smaller block");
   bool rebuild = block != pl->lc.block || ept != pl->lc.ept || order != pl->order;
   if (grid != pl->lc.grid || waves_per_eu != pl->lc.waves_per_eu || lds_bank != pl->lc.lds_bank) drop_graph(*pl);
   pl->lc.block = block; pl->lc.ept = ept; pl->order = order;
@@ -403,6 +421,7 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
   std::vector<Cfg> tried;
   Cfg best{pl->lc, pl->order, 1e30f};
   auto consider = [&](LaunchConfig lc, int order) -> int {
+    if (forward_lds_bytes(pl->dp, lc) > forward_lds_limit()) return GPU_PHS_OK;   // does not fit a workgroup
     float ms = 0;
     int rc = time_cfg(lc, order, &ms);
     if (rc) return rc;

@@ -2,13 +2,15 @@
 //
 // The arithmetic follows the reference's serial algorithm expression by expression
 // (phs_cpu.cpp:12-61 msq maps, 112-146 cos-theta maps, 174-234 decay, 283-345 angles and
-// boosts) so that, with contraction off, results differ from the CPU only where the device
-// math library rounds a transcendental differently.  Where the reference computes a value
-// and then branches on x (T/U-channel maps, peaked cos-theta map), the kernel evaluates one
-// transcendental on a selected argument and selects the result, so lanes never diverge.
+// boosts) so that, with contraction off, results differ from the CPU only where a
+// transcendental (sin, cos, tan, exp, log: phs_math.h) rounds differently.  Where the
+// reference computes a value and then branches on x (T/U-channel maps, peaked cos-theta map),
+// the kernel evaluates one transcendental on a selected argument and selects the result, so
+// lanes never diverge.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "phs_math.h"
 #include "phs_types.h"
 
 namespace gphs {
@@ -22,7 +24,6 @@ __device__ __forceinline__ const gdouble* as_global(const double* p) { return (c
 __device__ __forceinline__ guint8* as_global(uint8_t* p) { return (guint8*)p; }
 
 #define GPHS_PI 3.14159265358979323846
-#define GPHS_TWOPI 6.28318530717958647693
 
 // msq of an internal daughter and its mapping factor (phs_cpu.cpp:12-61).
 __device__ __forceinline__ void msq_map(const ChildCmd& c, double x, double s, double& msq, double& f) {
@@ -33,21 +34,21 @@ __device__ __forceinline__ void msq_map(const ChildCmd& c, double x, double s, d
       break;
     case MK_BREIT_WIGNER: {
       double z = (1 - x) * c.a0 + x * c.a1;
-      double t = tan(z);
+      double t = math::tan_radians(z);
       bool in = (-GPHS_PI / 2 < z) && (z < GPHS_PI / 2);
       msq = in ? c.m * (c.m + c.w * t) : 0.0;
       f = in ? c.a2 * (1 + t * t) : 0.0;
       break;
     }
     case MK_EXP: {
-      double m1 = c.a0 * exp(x * c.a1);
+      double m1 = c.a0 * math::exp_bounded(x * c.a1);
       msq = m1 - c.a0 + c.msq_min;
       f = c.a2 * m1;
       break;
     }
     case MK_TU: {
       bool lo = x < 0.5;
-      double m1 = c.a0 * exp((lo ? x : (1 - x)) * c.a1);
+      double m1 = c.a0 * math::exp_bounded((lo ? x : (1 - x)) * c.a1);
       msq = lo ? (m1 - c.a0 + c.msq_min) : (-(m1 - c.a0) + c.msq_max);
       f = c.a2 * m1;
       break;
@@ -59,8 +60,8 @@ __device__ __forceinline__ void msq_map(const ChildCmd& c, double x, double s, d
       break;
     }
     default: {  // MK_STEP_E
-      double tmp = exp(-x * c.a0 / c.a2) * (1 + c.a1);
-      double z = -c.a2 * log(tmp - c.a1);
+      double tmp = math::exp_bounded(-x * c.a0 / c.a2) * (1 + c.a1);
+      double z = -c.a2 * math::log_bounded(tmp - c.a1);
       msq = z * c.msq_max + (1 - z) * c.msq_min;
       f = c.a0 / (1 - c.a1 / tmp) * (c.msq_max - c.msq_min) / s;
       break;
@@ -77,7 +78,7 @@ __device__ __forceinline__ void ct_map(int kind, double x, double b0, double b1,
     f = 1;
   } else {
     bool lo = x < 0.5;
-    double tmp = b0 * exp((lo ? 2 * x : 2 * (1 - x)) * b1);
+    double tmp = b0 * math::exp_bounded((lo ? 2 * x : 2 * (1 - x)) * b1);
     double c = lo ? (tmp - b0 - 1) : (-(tmp - b0) + 1);
     bool in = c >= -1 && c <= 1;
     double sq = sqrt(1 - c * c);
@@ -101,6 +102,19 @@ __device__ __forceinline__ void compose(double (&M)[16], double g, double bg, do
     M[4 * i + 0] = r0 * g + r3 * bg;
     M[4 * i + 1] = r0 * l01 + r1 * l11 + r2 * l21 + r3 * l31;
     M[4 * i + 2] = r1 * l12 + r2 * l22;
+    M[4 * i + 3] = r0 * l03 + r1 * l13 + r2 * l23 + r3 * l33;
+  }
+}
+
+// Columns 0 and 3 of M * L1 only: all that the momenta of two final-state daughters need
+// (p = L (E, 0, 0, +-p)); columns 1 and 2 are left stale, so M must not be used further.
+__device__ __forceinline__ void compose_cols03(double (&M)[16], double g, double bg, double ct, double st, double cp,
+                                               double sp) {
+  const double l03 = bg * ct, l13 = st * cp, l23 = st * sp, l33 = g * ct;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double r0 = M[4 * i], r1 = M[4 * i + 1], r2 = M[4 * i + 2], r3 = M[4 * i + 3];
+    M[4 * i + 0] = r0 * g + r3 * bg;
     M[4 * i + 3] = r0 * l03 + r1 * l13 + r2 * l23 + r3 * l33;
   }
 }

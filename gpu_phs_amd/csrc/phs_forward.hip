@@ -8,7 +8,17 @@
 // selection is a wave-uniform branch instead of the reference's per-lane function pointers
 // (mappings_gpu.hpp:312-363).  The boost matrix of the branch being visited stays in
 // registers; the matrix of a deferred subtree waits in one of NBANK save banks.
+//
+// HBM traffic is staged through LDS so that every global access is a contiguous run of the
+// workgroup's events: the tile's randoms ([n][n_x], event-major) are read with consecutive
+// lanes on consecutive 16-byte words and scattered into an LDS table x[k][event]; each lane
+// reads its randoms from that table (consecutive lanes, consecutive words: conflict-free) and
+// writes its momenta into a table p[4 j + mu][event]; the workgroup then writes the tile's
+// momenta ([n][n_out][4]) as one contiguous run.  Rows are padded to block + 1 doubles so
+// that the scatter and gather of the cooperative phases spread over the LDS banks.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "phs_device.h"
 #include "phs_kernels.h"
@@ -49,7 +59,7 @@ struct Banks<0> {
 };
 
 // The same save bank in LDS: level d of lane t holds kBankWords doubles at
-// lds[(kBankWords d + i) * block + t] (consecutive lanes hit consecutive 8-byte words, so
+// base[(kBankWords d + i) * block + t] (consecutive lanes hit consecutive 8-byte words, so
 // every access is conflict-free).  Frees 38 VGPRs per level for occupancy at the price of 19
 // LDS stores and loads per deferral.
 constexpr int kBankWords = 19;
@@ -74,7 +84,11 @@ struct LdsBanks {
   }
 };
 
-extern __shared__ double lds_banks[];
+extern __shared__ double lds[];
+
+// 16-byte words of global memory (two doubles)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) dvec2 gdouble2;
 
 template <int NBANK, bool LDS>
 struct BankSel { using type = Banks<NBANK>; };
@@ -82,22 +96,22 @@ template <int NBANK>
 struct BankSel<NBANK, true> { using type = LdsBanks; };
 
 template <int NBANK, bool LDS>
-__device__ __forceinline__ typename BankSel<NBANK, LDS>::type make_banks() {
-  if constexpr (LDS) return LdsBanks{lds_banks + threadIdx.x, (int)blockDim.x};
+__device__ __forceinline__ typename BankSel<NBANK, LDS>::type make_banks(double* bank_base) {
+  if constexpr (LDS) return LdsBanks{bank_base + threadIdx.x, (int)blockDim.x};
   else return Banks<NBANK>{};
 }
 
-// One event of channel `cd` (program `prog`).
+// One event of channel `cd` (program `prog`).  xl: this lane's column of the staged randoms
+// (word k at xl[k * S]); pl: its column of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
 template <int NBANK, bool LDS>
 __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
-                                              const SegDesc& sd, int64_t e, bool valid) {
-  const gdouble* __restrict__ xr = as_global(sd.x) + e * cd.n_x;
-  gdouble* __restrict__ pr = as_global(sd.p) + e * (int64_t)cd.n_out * 4;
+                                              const double* __restrict__ xl, double* __restrict__ pl, int S,
+                                              double* bank_base, double& factor_out, bool& ok_out) {
   const double s = cd.s;
 
   // boost of the parent branch; the root's parent is the identity (phs_cpu.cpp:568-573)
   double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  auto bank = make_banks<NBANK, LDS>();
+  auto bank = make_banks<NBANK, LDS>(bank_base);
   double cur_msq = s;    // root: msq = s (phs_cpu.cpp:195)
   double cur_p = 0.0;    // root: p_decay = 0 (never set by the reference)
   double factor = 1.0;
@@ -110,9 +124,9 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547) and masses
     double msq0, f0, m1, msq1, f1, m2;
     if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
-    else { msq_map(nc.k[0], xr[nc.k[0].xi], s, msq0, f0); m1 = sqrt(msq0); }
+    else { msq_map(nc.k[0], xl[nc.k[0].xi * S], s, msq0, f0); m1 = sqrt(msq0); }
     if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
-    else { msq_map(nc.k[1], xr[nc.k[1].xi], s, msq1, f1); m2 = sqrt(msq1); }
+    else { msq_map(nc.k[1], xl[nc.k[1].xi * S], s, msq1, f1); m2 = sqrt(msq1); }
 
     // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
     const double m = cur_m;
@@ -125,13 +139,17 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     // angles and the boost of this branch (phs_cpu.cpp:299-340)
     const double bg = m > 0 ? cur_p / m : 0.0;
     const double g = sqrt(1 + bg * bg);
-    const double phi = xr[nc.x_phi] * GPHS_TWOPI;
     double sp, cp;
-    sincos(phi, &sp, &cp);
+    math::sincos_turns(xl[nc.x_phi * S], sp, cp);  // phi = 2 pi x
     double ct, st, fct;
-    ct_map(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1, ct, st, fct);
+    ct_map(nc.ct_kind, xl[(nc.x_phi + 1) * S], nc.b0, nc.b1, ct, st, fct);
     factor *= fct;
-    compose(M, g, bg, ct, st, cp, sp);
+    // the root's parent boost is the identity (phs_cpu.cpp:568-573): M = L1; after a branch
+    // with two final-state daughters the walk pops a saved matrix, so only the columns the
+    // daughters' momenta use are formed
+    if (i == 0) set_l1(M, g, bg, ct, st, cp, sp);
+    else if (nc.k[0].leaf && nc.k[1].leaf) compose_cols03(M, g, bg, ct, st, cp, sp);
+    else compose(M, g, bg, ct, st, cp, sp);
 
     // final-state daughters: p = L (E, 0, 0, +-p) (phs_cpu.cpp:286-291), then the lab boost
     // of the final leaves (phs.cu:1286-1301); E = sqrt(0 + p^2) is |p| exactly
@@ -149,10 +167,8 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
           const double b3 = cd.bg_lab * q0 + cd.g_lab * q3;
           q0 = b0; q3 = b3;
         }
-        if (valid) {
-          gdouble* o = pr + 4 * nc.k[c].out;
-          o[0] = q0; o[1] = q1; o[2] = q2; o[3] = q3;
-        }
+        double* o = pl + 4 * nc.k[c].out * S;
+        o[0] = q0; o[S] = q1; o[2 * S] = q2; o[3 * S] = q3;
       }
     }
 
@@ -170,31 +186,101 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
       bank.pop(depth, M, cur_msq, cur_m, cur_p);
     }
   }
-  if (valid) {
-    as_global(sd.factor)[e] = factor;
-    as_global(sd.volume)[e] = cd.volume;
-    as_global(sd.ok)[e] = ok ? 1 : 0;
+  factor_out = factor;
+  ok_out = ok;
+}
+
+// Randoms of `cnt` events (src = [cnt][n_x], event-major) into xs[k * S + event].
+__device__ __forceinline__ void stage_randoms(const double* __restrict__ src, int cnt, const ChannelDesc& cd,
+                                              bool aligned, double* __restrict__ xs, int S) {
+  const int n_x = cd.n_x;
+  const int n = cnt * n_x;
+  if (aligned && !(n_x & 1)) {
+    // 16-byte words; n_x even, so both doubles of a word belong to one event
+    const gdouble2* s2 = (const gdouble2*)src;
+    for (int w = (int)threadIdx.x; w < n / 2; w += (int)blockDim.x) {
+      const dvec2 v = s2[w];
+      const int i = 2 * w;
+      const int ev = (int)__umulhi((unsigned)i, cd.div_nx);
+      const int k = i - ev * n_x;
+      xs[k * S + ev] = v.x;
+      xs[(k + 1) * S + ev] = v.y;
+    }
+  } else {
+    const gdouble* s1 = as_global(src);
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
+      const int ev = (int)__umulhi((unsigned)i, cd.div_nx);
+      xs[(i - ev * n_x) * S + ev] = s1[i];
+    }
+  }
+}
+
+// Momenta of `cnt` events from ps[(4 j + mu) * S + event] to dst = [cnt][n_out][4].
+__device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int S, int cnt, const ChannelDesc& cd,
+                                              bool aligned, double* __restrict__ dst) {
+  const int w4 = 4 * cd.n_out;
+  const int n = cnt * w4;
+  if (aligned) {
+    // 16-byte words; 4 n_out is even, so both doubles of a word belong to one event
+    gdouble2* d2 = (gdouble2*)dst;
+    for (int w = (int)threadIdx.x; w < n / 2; w += (int)blockDim.x) {
+      const int i = 2 * w;
+      const int ev = (int)__umulhi((unsigned)i, cd.div_p);
+      const int c = i - ev * w4;
+      dvec2 v;
+      v.x = ps[c * S + ev];
+      v.y = ps[(c + 1) * S + ev];
+      d2[w] = v;
+    }
+  } else {
+    gdouble* d1 = as_global(dst);
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
+      const int ev = (int)__umulhi((unsigned)i, cd.div_p);
+      d1[i] = ps[(i - ev * w4) * S + ev];
+    }
   }
 }
 
 // WPE > 0 asks the register allocator for at least WPE waves per SIMD (fewer VGPRs, spills
 // allowed); the autotuner measures which occupancy target wins.
+//
+// LDS per workgroup (S = blockDim.x + 1 doubles per row): randoms nx_max rows, momenta
+// 4 nout_max rows, then the save banks when LDS is set.  A tile of `count` events is done in
+// sub-tiles of blockDim.x events.
 template <int NBANK, bool LDS, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
-                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int ept) {
+                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
+                    int nout_max) {
+  const int B = (int)blockDim.x;
+  const int S = B + 1;
+  double* xs = lds;
+  double* ps = lds + nx_max * S;
+  double* bank_base = ps + 4 * nout_max * S;
+  const int lane = (int)threadIdx.x;
   for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
     const ChannelDesc cd = chans[sd.channel];
     const NodeCmd* prog = nodes + cd.node0;
-    for (int k = 0; k < ept; ++k) {
-      const int base = k * (int)blockDim.x;
-      if (base >= td.count) break;
-      const int local = base + (int)threadIdx.x;
-      const bool valid = local < td.count;
-      const int64_t e = td.e0 + (valid ? local : td.count - 1);
-      forward_event<NBANK, LDS>(cd, prog, sd, e, valid);
+    for (int base = 0; base < td.count; base += B) {
+      const int cnt = min(B, td.count - base);
+      const int64_t e0 = td.e0 + base;
+      __syncthreads();   // the previous sub-tile's tables have been read
+      stage_randoms(sd.x + e0 * cd.n_x, cnt, cd, sd.align16 & 1, xs, S);
+      __syncthreads();
+      const bool valid = lane < cnt;
+      double factor;
+      bool ok;
+      // lanes past the end repeat the last event and store nothing
+      forward_event<NBANK, LDS>(cd, prog, xs + (valid ? lane : cnt - 1), ps + lane, S, bank_base, factor, ok);
+      if (valid) {
+        as_global(sd.factor)[e0 + lane] = factor;
+        as_global(sd.volume)[e0 + lane] = cd.volume;
+        as_global(sd.ok)[e0 + lane] = ok ? 1 : 0;
+      }
+      __syncthreads();
+      store_momenta(ps, S, cnt, cd, sd.align16 & 2, sd.p + e0 * (int64_t)cd.n_out * 4);
     }
   }
 }
@@ -202,7 +288,7 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
 }  // namespace
 
 namespace {
-using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int);
+using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int, int);
 
 template <bool LDS, int WPE>
 FwdKernel pick_banks(int n_banks) {
@@ -223,20 +309,40 @@ FwdKernel pick_wpe(int n_banks, int wpe) {
     default: return pick_banks<LDS, 0>(n_banks);
   }
 }
+
+bool use_lds_bank(const DevicePlan& dp, const LaunchConfig& lc) { return lc.lds_bank && dp.n_banks > 0; }
 }  // namespace
+
+size_t forward_lds_bytes(const DevicePlan& dp, const LaunchConfig& lc) {
+  const size_t rows = (size_t)dp.nx_max + 4 * (size_t)dp.nout_max;
+  size_t bytes = rows * (size_t)(lc.block + 1) * sizeof(double);
+  if (use_lds_bank(dp, lc)) bytes += (size_t)kBankWords * sizeof(double) * lc.block * dp.n_banks;
+  return bytes;
+}
+
+size_t forward_lds_limit() {
+  static size_t limit[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!limit[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v < 65536) v = 65536;
+    limit[dev] = (size_t)std::min(v, 160 * 1024);
+  }
+  return limit[dev];
+}
 
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream) {
   if (t_end <= t_begin) return hipSuccess;
-  if (dp.n_banks < 0 || dp.n_banks > 3 || lc.block < 64 || lc.block > 256) return hipErrorInvalidValue;
+  if (dp.n_banks < 0 || dp.n_banks > 3 || lc.block < 64 || lc.block > 256 || lc.block % 64) return hipErrorInvalidValue;
+  const size_t shmem = forward_lds_bytes(dp, lc);
+  if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
   const int n = t_end - t_begin;
   const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
-  // LDS banks only when they fit the 64 KiB a workgroup may allocate; else the VGPR banks
-  const size_t lds_bytes = (size_t)kBankWords * sizeof(double) * lc.block * dp.n_banks;
-  const bool lds = lc.lds_bank && dp.n_banks > 0 && lds_bytes <= 65536;
-  const size_t shmem = lds ? lds_bytes : 0;
-  FwdKernel k = lds ? pick_wpe<true>(dp.n_banks, lc.waves_per_eu) : pick_wpe<false>(dp.n_banks, lc.waves_per_eu);
+  const bool lds_bank = use_lds_bank(dp, lc);
+  FwdKernel k = lds_bank ? pick_wpe<true>(dp.n_banks, lc.waves_per_eu) : pick_wpe<false>(dp.n_banks, lc.waves_per_eu);
   hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,
-                     dp.nodes, lc.ept);
+                     dp.nodes, dp.nx_max, dp.nout_max);
   return hipGetLastError();
 }
 

@@ -14,6 +14,8 @@ struct DevicePlan {
   const NodeCmd* nodes = nullptr;
   int n_tiles = 0;
   int n_banks = 0;       // save-bank depth the deepest program needs (0..3)
+  int nx_max = 0;        // largest n_x of the plan's channels
+  int nout_max = 0;      // largest n_out of the plan's channels
 };
 
 struct LaunchConfig {
@@ -23,6 +25,12 @@ struct LaunchConfig {
   int waves_per_eu = 0;  // register-allocation occupancy target: 0 (compiler's choice), 3, 4, 5
   int lds_bank = 0;      // 1: keep the save banks of deferred subtrees in LDS instead of VGPRs
 };
+
+// LDS bytes per workgroup of the forward kernel under `lc`: the staged randoms and momenta of
+// `lc.block` events, plus the save banks when they are kept in LDS.
+size_t forward_lds_bytes(const DevicePlan& dp, const LaunchConfig& lc);
+// LDS a workgroup may allocate on the current device.
+size_t forward_lds_limit();
 
 // Forward pass over tiles [t_begin, t_end).
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream);

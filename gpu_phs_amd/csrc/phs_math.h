@@ -1,0 +1,175 @@
+// Bounded-range elementary functions for the kernels: sin and cos of 2 pi x, tan, exp, log.
+//
+// The maps need sin and cos of 2*pi*x for a random x in [0, 1] (the azimuth, phs_cpu.cpp:301-307),
+// tan of an angle inside (-pi/2, pi/2) (the Breit-Wigner map, phs_cpu.cpp:25-28), and exp and
+// log of moderate arguments (phs_cpu.cpp:29-59, 128-140).  The device library's routines accept
+// every double and cost 40-190 VALU instructions each; worse, inside the kernels' branch loop
+// the compiler hoists their polynomial coefficients out of the loop into vector registers, some
+// sixty of them for the whole set, which halves the number of resident waves.  Here every
+// coefficient goes through pin(), which makes the compiler materialise it in a scalar register
+// pair right where it is used (two scalar moves, issued beside the vector work) instead.
+//
+// Methods (all with explicit fused multiply-adds, so they do not depend on contraction):
+//   * sin/cos: reduction to |theta| <= pi/4 -- exact for turns (t = 4x, r = t - rint(t));
+//     for radians, Cody-Waite subtraction of n * pi/2 with pi/2 split into two 33-bit parts
+//     and a 53-bit tail (W. J. Cody, W. Waite, Software Manual for the Elementary Functions,
+//     1980), exact in its first two steps for |n| < 2^20 -- then the Taylor series cut after
+//     theta^17/17! and theta^18/18! (first dropped terms at pi/4: 1.1e-19 and 3.3e-21);
+//   * exp: x = k ln 2 + r with |r| <= ln2 / 2 (ln 2 as a double-double), the Taylor series of
+//     e^r to r^13/13! (first dropped term 4.2e-18), then scaling by 2^k;
+//   * log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m - 1)/(m + 1),
+//     |s| <= 0.1716, series to s^21/21 (first dropped term 2.3e-17 relative), plus e ln 2.
+// The tests check them against 300-bit arithmetic: within 2 units in the last place (sin,
+// cos, exp, log) and 3 (tan).  tan(-0) is +0.  The header has no HIP dependency, so the same
+// code is compiled for the host in those tests.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GPHS_MATH_FN __host__ __device__ __forceinline__
+#else
+#define GPHS_MATH_FN inline
+#endif
+
+namespace gphs {
+namespace math {
+
+// 1/n! as the correctly rounded doubles of exact quotients (every n! here is below 2^53).
+constexpr double kS3 = -1.0 / 6.0;
+constexpr double kS5 = 1.0 / 120.0;
+constexpr double kS7 = -1.0 / 5040.0;
+constexpr double kS9 = 1.0 / 362880.0;
+constexpr double kS11 = -1.0 / 39916800.0;
+constexpr double kS13 = 1.0 / 6227020800.0;
+constexpr double kS15 = -1.0 / 1307674368000.0;
+constexpr double kS17 = 1.0 / 355687428096000.0;
+constexpr double kC4 = 1.0 / 24.0;
+constexpr double kC6 = -1.0 / 720.0;
+constexpr double kC8 = 1.0 / 40320.0;
+constexpr double kC10 = -1.0 / 3628800.0;
+constexpr double kC12 = 1.0 / 479001600.0;
+constexpr double kC14 = -1.0 / 87178291200.0;
+constexpr double kC16 = 1.0 / 20922789888000.0;
+constexpr double kC18 = -1.0 / 6402373705728000.0;
+
+// pi/2 = kPio2Hi + kPio2Lo to 106 bits.
+constexpr double kPio2Hi = 0x1.921fb54442d18p+0;
+constexpr double kPio2Lo = 0x1.1a62633145c07p-54;
+// pi/2 = kCw1 + kCw2 + kCw3: kCw1 and kCw2 hold 33 significant bits, so n * kCw1 and n * kCw2
+// are exact for |n| < 2^20.
+constexpr double kCw1 = 0x1.921fb544p+0;
+constexpr double kCw2 = 0x1.0b4611a6p-34;
+constexpr double kCw3 = 0x1.3198a2e037073p-69;
+constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
+// ln 2 = kLn2Hi + kLn2Lo to 106 bits; log2(e); sqrt(1/2).
+constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
+constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
+constexpr double kLog2e = 0x1.71547652b82fep+0;
+constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;
+
+// A constant materialised in a scalar register pair at its point of use (see the file comment).
+GPHS_MATH_FN double pin(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+s"(c));
+#endif
+  return c;
+}
+
+// sin and cos of |t| <= pi/4.
+GPHS_MATH_FN void sincos_reduced(double t, double& s, double& c) {
+  const double t2 = t * t;
+  double ps = pin(kS17);
+  ps = __builtin_fma(ps, t2, pin(kS15));
+  ps = __builtin_fma(ps, t2, pin(kS13));
+  ps = __builtin_fma(ps, t2, pin(kS11));
+  ps = __builtin_fma(ps, t2, pin(kS9));
+  ps = __builtin_fma(ps, t2, pin(kS7));
+  ps = __builtin_fma(ps, t2, pin(kS5));
+  ps = __builtin_fma(ps, t2, pin(kS3));
+  s = __builtin_fma(t * t2, ps, t);
+  double pc = pin(kC18);
+  pc = __builtin_fma(pc, t2, pin(kC16));
+  pc = __builtin_fma(pc, t2, pin(kC14));
+  pc = __builtin_fma(pc, t2, pin(kC12));
+  pc = __builtin_fma(pc, t2, pin(kC10));
+  pc = __builtin_fma(pc, t2, pin(kC8));
+  pc = __builtin_fma(pc, t2, pin(kC6));
+  pc = __builtin_fma(pc, t2, pin(kC4));
+  c = __builtin_fma(t2 * t2, pc, __builtin_fma(t2, -0.5, 1.0));
+}
+
+// s = sin(2 pi x), c = cos(2 pi x) for |x| < 2^28 (a full turn per unit of x).
+GPHS_MATH_FN void sincos_turns(double x, double& s, double& c) {
+  const double t = 4.0 * x;
+  const double q = __builtin_rint(t);
+  const double r = t - q;  // exact, |r| <= 1/2
+  double s0, c0;
+  sincos_reduced(__builtin_fma(r, pin(kPio2Hi), r * pin(kPio2Lo)), s0, c0);
+  const int n = (int)q;  // quadrant; n & 3 is right for negative n too
+  const double a = (n & 1) ? c0 : s0;
+  const double b = (n & 1) ? s0 : c0;
+  s = (n & 2) ? -a : a;
+  c = ((n + 1) & 2) ? -b : b;
+}
+
+// tan(z) for |z| < 2^20 * pi/2; the kernels call it for |z| < pi/2.
+GPHS_MATH_FN double tan_radians(double z) {
+  const double q = __builtin_rint(z * pin(kTwoOverPi));
+  const double r = ((z - q * pin(kCw1)) - q * pin(kCw2)) - q * pin(kCw3);
+  double s0, c0;
+  sincos_reduced(r, s0, c0);
+  const bool odd = ((int)q) & 1;
+  return (odd ? -c0 : s0) / (odd ? s0 : c0);
+}
+
+// e^x; overflows to +inf above ~709.8 and underflows to 0 below ~-745 like exp().
+GPHS_MATH_FN double exp_bounded(double x) {
+  const double xc = x > 709.8 ? 709.8 : (x < -745.2 ? -745.2 : x);   // keeps k in int range
+  const double k = __builtin_rint(xc * pin(kLog2e));
+  const double r = __builtin_fma(-k, pin(kLn2Lo), __builtin_fma(-k, pin(kLn2Hi), xc));
+  double p = pin(1.0 / 6227020800.0);
+  p = __builtin_fma(p, r, pin(1.0 / 479001600.0));
+  p = __builtin_fma(p, r, pin(1.0 / 39916800.0));
+  p = __builtin_fma(p, r, pin(1.0 / 3628800.0));
+  p = __builtin_fma(p, r, pin(1.0 / 362880.0));
+  p = __builtin_fma(p, r, pin(1.0 / 40320.0));
+  p = __builtin_fma(p, r, pin(1.0 / 5040.0));
+  p = __builtin_fma(p, r, pin(1.0 / 720.0));
+  p = __builtin_fma(p, r, pin(1.0 / 120.0));
+  p = __builtin_fma(p, r, pin(1.0 / 24.0));
+  p = __builtin_fma(p, r, pin(1.0 / 6.0));
+  p = __builtin_fma(p, r, 0.5);
+  p = __builtin_fma(p, r * r, r) + 1.0;   // 1 + r + r^2 (1/2 + r/6 + ...)
+  // 2^k in two steps, so that 2^k itself need not be representable near the ends of the range
+  const int ki = (int)k;
+  const double y = __builtin_ldexp(__builtin_ldexp(p, ki / 2), ki - ki / 2);
+  return x != x ? x : y;
+}
+
+// Natural logarithm; log(0) = -inf, log(x < 0) = NaN, log(+inf) = +inf, like log().
+GPHS_MATH_FN double log_bounded(double x) {
+  int e;
+  double m = __builtin_frexp(x, &e);   // x = m 2^e, m in [1/2, 1)
+  if (m < pin(kSqrtHalf)) { m = 2 * m; e -= 1; }
+  const double f = m - 1.0;             // exact
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double p = pin(1.0 / 21.0);
+  p = __builtin_fma(p, z, pin(1.0 / 19.0));
+  p = __builtin_fma(p, z, pin(1.0 / 17.0));
+  p = __builtin_fma(p, z, pin(1.0 / 15.0));
+  p = __builtin_fma(p, z, pin(1.0 / 13.0));
+  p = __builtin_fma(p, z, pin(1.0 / 11.0));
+  p = __builtin_fma(p, z, pin(1.0 / 9.0));
+  p = __builtin_fma(p, z, pin(1.0 / 7.0));
+  p = __builtin_fma(p, z, pin(1.0 / 5.0));
+  p = __builtin_fma(p, z, pin(1.0 / 3.0));
+  const double lm = __builtin_fma(2 * s * z, p, 2 * s);     // 2 atanh(s)
+  const double de = (double)e;
+  const double y = __builtin_fma(de, pin(kLn2Hi), __builtin_fma(de, pin(kLn2Lo), lm));
+  if (!(x > 0)) return x == 0 ? -__builtin_inf() : __builtin_nan("");
+  return x == __builtin_inf() ? x : y;
+}
+
+}  // namespace math
+}  // namespace gphs

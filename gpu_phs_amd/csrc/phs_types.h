@@ -71,6 +71,8 @@ struct ChannelDesc {
   int32_t param0;    // first (M, Gamma) accumulator slot of this channel (adjoint)
   int32_t n_banks;   // save-bank depth the program needs
   int32_t n_tape;    // adjoint tape slots the program needs
+  uint32_t div_nx;   // floor(2^32 / n_x) + 1: i / n_x = umulhi(i, div_nx) for i < 2^32 / n_x
+  uint32_t div_p;    // floor(2^32 / (4 n_out)) + 1, likewise for the momenta of an event
   int32_t pad1;
   double sqrts, s;
   double volume;     // phase-space volume, evaluated in the reference's order
@@ -83,7 +85,7 @@ struct ChannelDesc {
 // the reference's interface: x [n][n_x], p [n][n_out][4]).
 struct SegDesc {
   int32_t channel;   // index into the plan's channel table
-  int32_t pad;
+  int32_t align16;   // bit 0: x is 16-byte aligned, bit 1: p is 16-byte aligned
   int64_t n_events;
   const double* x;
   double* p;

@@ -254,6 +254,8 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
     cd.n_nodes = sp.n_out - 1;
     cd.n_out = sp.n_out;
     cd.n_x = sp.n_x();
+    cd.div_nx = (uint32_t)(((uint64_t)1 << 32) / (uint64_t)cd.n_x + 1);
+    cd.div_p = (uint32_t)(((uint64_t)1 << 32) / (uint64_t)(4 * cd.n_out) + 1);
     cd.param0 = cd.node0;
     cd.sqrts = sqrts;
     cd.s = sqrts * sqrts;

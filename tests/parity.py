@@ -30,12 +30,13 @@ EPS = 2.0 ** -52
 
 def threshold_margin(proc, msq, channels, sqrts):
     """Smallest relative distance of each event to one of the kinematic cuts of the oracle
-    (phs_cpu.cpp:225-226): lambda / msq^2, (m - m1 - m2) / m and (m_max - m) / m_max over the
-    internal branches, from the oracle's msq of every branch."""
+    (phs_cpu.cpp:225-226): lambda / msq^2, (m - m1 - m2) / m and (m_max - m) / m_max, from the
+    oracle's msq of every branch.  The branches are visited in the oracle's post-order up to
+    and including the first one whose cut fails: the oracle returns there (phs_cpu.cpp:186-193)
+    and leaves the msq of later branches at 0, so only the cuts it actually evaluated count."""
     n = msq.shape[0]
     out = np.full(n, np.inf)
-    n_prt, root = proc.n_prt, proc.n_prt_out - 1
-    mtot_all = np.zeros(proc.n_channels)
+    root = proc.n_prt_out - 1
     import oracle
     ms = oracle.mass_sums(proc)
     for c in range(proc.n_channels):
@@ -43,9 +44,18 @@ def threshold_margin(proc, msq, channels, sqrts):
         if sel.size == 0:
             continue
         m2 = msq[sel]
-        for b in range(proc.n_prt_out):
-            if not proc.has_children[c, b]:
-                continue
+        order = []
+
+        def post(b):
+            for k in (proc.d1[c, b] - 1, proc.d2[c, b] - 1):
+                if proc.has_children[c, k]:
+                    post(k)
+            order.append(b)
+
+        post(root)
+        alive = np.ones(sel.size, dtype=bool)
+        marg = np.full(sel.size, np.inf)
+        for b in order:
             k1, k2 = proc.d1[c, b] - 1, proc.d2[c, b] - 1
             b1, b2 = m2[:, k1], m2[:, k2]
             a = np.full_like(b1, sqrts * sqrts) if b == root else m2[:, b]
@@ -53,10 +63,14 @@ def threshold_margin(proc, msq, channels, sqrts):
             with np.errstate(all="ignore"):
                 m, m1_, m2_ = np.sqrt(a), np.sqrt(b1), np.sqrt(b2)
                 mmax = sqrts if b == root else sqrts - ms[c, root] + ms[c, b]
-                r = np.minimum.reduce([np.abs(lda) / np.maximum(a * a, 1e-300),
-                                       np.abs(m - m1_ - m2_) / np.maximum(m, 1e-300),
-                                       np.abs(mmax - m) / mmax])
-            out[sel] = np.minimum(out[sel], np.nan_to_num(r, nan=0.0))
+                cuts = [np.abs(lda) / np.maximum(a * a, 1e-300), np.abs(m - m1_ - m2_) / np.maximum(m, 1e-300)]
+                if b != root:   # the root sits at m = m_max = sqrt(s) by construction
+                    cuts.append(np.abs(mmax - m) / mmax)
+                r = np.nan_to_num(np.minimum.reduce(cuts), nan=0.0)
+                passed = (lda > 0) & (m > m1_ + m2_) & (m <= mmax)
+            marg = np.where(alive, np.minimum(marg, r), marg)
+            alive &= passed
+        out[sel] = marg
     return out
 
 

@@ -1,0 +1,129 @@
+"""The kernels' bounded-range sin/cos/tan/exp/log (gpu_phs_amd/csrc/phs_math.h), compiled for the host,
+against 300-bit arithmetic (mpmath).  The kernels use the same code, so the error bounds
+checked here are the ones the GPU parity tests rely on."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "gpu_phs_amd", "csrc")
+mp = pytest.importorskip("mpmath")
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("math") / "libmath_host.so")
+    # -mfma: the fused multiply-adds become instructions, as on the GPU (without it they are
+    # libm calls with the same, correctly rounded, result)
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-I", CSRC,
+                           os.path.join(HERE, "math_host.cpp"), "-o", out])
+    L = ctypes.CDLL(out)
+    P = ctypes.POINTER(ctypes.c_double)
+    L.host_sincos_turns.argtypes = [P, ctypes.c_long, P, P]
+    L.host_tan_radians.argtypes = [P, ctypes.c_long, P]
+    L.host_exp.argtypes = [P, ctypes.c_long, P]
+    L.host_log.argtypes = [P, ctypes.c_long, P]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ulps(got, exact):
+    """|got - exact| in units of the last place of the exact value (exact as mpf)."""
+    out = np.empty(len(got))
+    for i, (g, e) in enumerate(zip(got, exact)):
+        ef = float(e)
+        if ef == 0:
+            out[i] = 0.0 if g == 0 else np.inf
+            continue
+        out[i] = float(abs(mp.mpf(g) - e)) / np.spacing(abs(ef))
+    return out
+
+
+def _turn_points():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.random(3000),
+                        # quadrant edges and their neighbours, where reduction errors would show
+                        np.array([k / 8 for k in range(9)]),
+                        np.nextafter(np.array([k / 8 for k in range(1, 9)]), 0),
+                        np.nextafter(np.array([k / 8 for k in range(0, 8)]), 1),
+                        np.array([1e-300, 5e-324, 1e-17, 0.5 - 1e-17, 1 - 2 ** -53]),
+                        -rng.random(200), 1 + 3 * rng.random(200)])
+    return x
+
+
+def test_sincos_turns_within_two_ulps(lib):
+    mp.mp.prec = 300
+    x = _turn_points()
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib.host_sincos_turns(_p(x), len(x), _p(s), _p(c))
+    # sin(2 pi x) and cos(2 pi x) are exactly 0 or +-1 where 4x is an integer
+    quarter = {0: (0, 1), 1: (1, 0), 2: (0, -1), 3: (-1, 0)}
+    es = [mp.mpf(quarter[int(4 * v) % 4][0]) if 4 * v == int(4 * v) else mp.sin(2 * mp.pi * mp.mpf(v)) for v in x]
+    ec = [mp.mpf(quarter[int(4 * v) % 4][1]) if 4 * v == int(4 * v) else mp.cos(2 * mp.pi * mp.mpf(v)) for v in x]
+    us, uc = _ulps(s, es), _ulps(c, ec)
+    print("max ulps sin", us.max(), x[us.argmax()], "cos", uc.max(), x[uc.argmax()])
+    # exact zeros at the quadrant edges
+    for k in range(5):
+        xs = np.array([k / 4])
+        lib.host_sincos_turns(_p(xs), 1, _p(s[:1]), _p(c[:1]))
+        assert (s[0], c[0]) == [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0), (0.0, 1.0)][k]
+    assert us.max() <= 2.0 and uc.max() <= 2.0
+
+
+def test_tan_radians_within_three_ulps(lib):
+    mp.mp.prec = 300
+    rng = np.random.default_rng(2)
+    half_pi = np.pi / 2
+    z = np.concatenate([(2 * rng.random(3000) - 1) * half_pi,
+                        (2 * rng.random(300) - 1) * 1e-3,
+                        np.array([half_pi, -half_pi, np.nextafter(half_pi, 0), np.pi / 4, -np.pi / 4,
+                                  np.nextafter(np.pi / 4, 1), 1e-300, 0.0, -0.0, 1.2, -1.5]),
+                        (2 * rng.random(300) - 1) * 20])
+    t = np.empty_like(z)
+    lib.host_tan_radians(_p(z), len(z), _p(t))
+    u = _ulps(t, [mp.tan(mp.mpf(v)) for v in z])
+    print("max ulps tan", u.max(), z[u.argmax()])
+    assert u.max() <= 3.0
+
+
+def test_exp_within_two_ulps(lib):
+    mp.mp.prec = 300
+    rng = np.random.default_rng(3)
+    x = np.concatenate([(2 * rng.random(2000) - 1) * 40, (2 * rng.random(500) - 1) * 700,
+                        (2 * rng.random(300) - 1) * 1e-6,
+                        np.array([0.0, -0.0, 1e-300, 709.7, -744.0, -708.5, 0.5 * np.log(2), -0.5 * np.log(2)])])
+    y = np.empty_like(x)
+    lib.host_exp(_p(x), len(x), _p(y))
+    u = _ulps(y, [mp.exp(mp.mpf(v)) for v in x])
+    print("max ulps exp", u.max(), x[u.argmax()])
+    assert u.max() <= 2.0
+    # the ends of the range and special values
+    sp = np.array([710.0, 1e308, -746.0, -1e308, np.inf, -np.inf, np.nan])
+    ys = np.empty_like(sp)
+    lib.host_exp(_p(sp), len(sp), _p(ys))
+    assert ys[0] == np.inf and ys[1] == np.inf and ys[2] == 0 and ys[3] == 0
+    assert ys[4] == np.inf and ys[5] == 0 and np.isnan(ys[6])
+
+
+def test_log_within_two_ulps(lib):
+    mp.mp.prec = 300
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.random(1500) * 4, 10.0 ** (rng.random(1000) * 600 - 300),
+                        1 + (2 * rng.random(300) - 1) * 1e-8,
+                        np.array([1.0, 2.0, 0.5, np.sqrt(0.5), np.nextafter(np.sqrt(0.5), 1), 5e-324, 1.7e308])])
+    y = np.empty_like(x)
+    lib.host_log(_p(x), len(x), _p(y))
+    u = _ulps(y, [mp.log(mp.mpf(v)) for v in x])
+    print("max ulps log", u.max(), x[u.argmax()])
+    assert u.max() <= 2.0
+    sp = np.array([0.0, -1.0, np.inf, np.nan])
+    ys = np.empty_like(sp)
+    lib.host_log(_p(sp), len(sp), _p(ys))
+    assert ys[0] == -np.inf and np.isnan(ys[1]) and ys[2] == np.inf and np.isnan(ys[3])
