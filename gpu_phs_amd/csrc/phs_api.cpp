@@ -300,9 +300,9 @@ int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, i
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
   if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1 ||
-      !(waves_per_eu == 0 || (waves_per_eu >= 3 && waves_per_eu <= 5)) || lds_bank < 0 || lds_bank > 1)
+      !(waves_per_eu == 0 || waves_per_eu == 3 || waves_per_eu == 4) || lds_bank < 0 || lds_bank > 1)
     return set_error(GPU_PHS_E_ARG,
-                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4,5}, "
+                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4}, "
                      "lds_bank in {0,1}");
   LaunchConfig want = pl->lc;
   want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds_bank = lds_bank;

@@ -9,13 +9,16 @@
 // (mappings_gpu.hpp:312-363).  The boost matrix of the branch being visited stays in
 // registers; the matrix of a deferred subtree waits in one of NBANK save banks.
 //
-// HBM traffic is staged through LDS so that every global access is a contiguous run of the
-// workgroup's events: the tile's randoms ([n][n_x], event-major) are read with consecutive
-// lanes on consecutive 16-byte words and scattered into an LDS table x[k][event]; each lane
-// reads its randoms from that table (consecutive lanes, consecutive words: conflict-free) and
-// writes its momenta into a table p[4 j + mu][event]; the workgroup then writes the tile's
-// momenta ([n][n_out][4]) as one contiguous run.  Rows are padded to block + 1 doubles so
-// that the scatter and gather of the cooperative phases spread over the LDS banks.
+// Each lane loads its event's randoms (one row of [n][n_x]) into registers when the event
+// starts, so the branch loop never waits on memory; the row is indexed with the program's
+// wave-uniform random indices (register indexing by a scalar).  The momenta, the bulk of the
+// traffic, go through LDS: each lane writes its event's four-vectors into a table
+// p[4 j + mu][event] (consecutive lanes, consecutive words: conflict-free), and the workgroup
+// then writes the sub-tile's momenta ([n][n_out][4]) as one contiguous run of 16-byte words,
+// so the stores leave the CU as whole cache lines instead of 64 scattered 16-byte pieces.
+// Rows are padded to block + 1 doubles so that the gather of the store phase spreads over the
+// LDS banks.  Channels with more than 16 randoms (n_out > 6) read them from memory as they
+// are used instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -101,13 +104,24 @@ __device__ __forceinline__ typename BankSel<NBANK, LDS>::type make_banks(double*
   else return Banks<NBANK>{};
 }
 
-// One event of channel `cd` (program `prog`).  xl: this lane's column of the staged randoms
-// (word k at xl[k * S]); pl: its column of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
-template <int NBANK, bool LDS>
+// Randoms of one event: with XREG the row is held in registers (n_x <= kXReg) and indexed by
+// the program's wave-uniform indices, else it is read from memory at each use.
+constexpr int kXReg = 16;
+
+// One event of channel `cd` (program `prog`).  xg: the event's randoms; pl: this lane's column
+// of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
+template <int NBANK, bool LDS, bool XREG>
 __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
-                                              const double* __restrict__ xl, double* __restrict__ pl, int S,
+                                              const double* __restrict__ xg, double* __restrict__ pl, int S,
                                               double* bank_base, double& factor_out, bool& ok_out) {
   const double s = cd.s;
+  const gdouble* __restrict__ xrow = as_global(xg);
+  double xv[XREG ? kXReg : 1];
+  if constexpr (XREG) {
+#pragma unroll
+    for (int k = 0; k < kXReg; k++) xv[k] = k < cd.n_x ? xrow[k] : 0.0;
+  }
+#define GPHS_X(k) (XREG ? xv[XREG ? (k) : 0] : xrow[k])
 
   // boost of the parent branch; the root's parent is the identity (phs_cpu.cpp:568-573)
   double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
@@ -124,9 +138,9 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547) and masses
     double msq0, f0, m1, msq1, f1, m2;
     if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
-    else { msq_map(nc.k[0], xl[nc.k[0].xi * S], s, msq0, f0); m1 = sqrt(msq0); }
+    else { msq_map(nc.k[0], GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = sqrt(msq0); }
     if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
-    else { msq_map(nc.k[1], xl[nc.k[1].xi * S], s, msq1, f1); m2 = sqrt(msq1); }
+    else { msq_map(nc.k[1], GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = sqrt(msq1); }
 
     // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
     const double m = cur_m;
@@ -140,9 +154,9 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     const double bg = m > 0 ? cur_p / m : 0.0;
     const double g = sqrt(1 + bg * bg);
     double sp, cp;
-    math::sincos_turns(xl[nc.x_phi * S], sp, cp);  // phi = 2 pi x
+    math::sincos_turns(GPHS_X(nc.x_phi), sp, cp);  // phi = 2 pi x
     double ct, st, fct;
-    ct_map(nc.ct_kind, xl[(nc.x_phi + 1) * S], nc.b0, nc.b1, ct, st, fct);
+    ct_map(nc.ct_kind, GPHS_X(nc.x_phi + 1), nc.b0, nc.b1, ct, st, fct);
     factor *= fct;
     // the root's parent boost is the identity (phs_cpu.cpp:568-573): M = L1; after a branch
     // with two final-state daughters the walk pops a saved matrix, so only the columns the
@@ -188,31 +202,7 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
   }
   factor_out = factor;
   ok_out = ok;
-}
-
-// Randoms of `cnt` events (src = [cnt][n_x], event-major) into xs[k * S + event].
-__device__ __forceinline__ void stage_randoms(const double* __restrict__ src, int cnt, const ChannelDesc& cd,
-                                              bool aligned, double* __restrict__ xs, int S) {
-  const int n_x = cd.n_x;
-  const int n = cnt * n_x;
-  if (aligned && !(n_x & 1)) {
-    // 16-byte words; n_x even, so both doubles of a word belong to one event
-    const gdouble2* s2 = (const gdouble2*)src;
-    for (int w = (int)threadIdx.x; w < n / 2; w += (int)blockDim.x) {
-      const dvec2 v = s2[w];
-      const int i = 2 * w;
-      const int ev = (int)__umulhi((unsigned)i, cd.div_nx);
-      const int k = i - ev * n_x;
-      xs[k * S + ev] = v.x;
-      xs[(k + 1) * S + ev] = v.y;
-    }
-  } else {
-    const gdouble* s1 = as_global(src);
-    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
-      const int ev = (int)__umulhi((unsigned)i, cd.div_nx);
-      xs[(i - ev * n_x) * S + ev] = s1[i];
-    }
-  }
+#undef GPHS_X
 }
 
 // Momenta of `cnt` events from ps[(4 j + mu) * S + event] to dst = [cnt][n_out][4].
@@ -244,18 +234,16 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
 // WPE > 0 asks the register allocator for at least WPE waves per SIMD (fewer VGPRs, spills
 // allowed); the autotuner measures which occupancy target wins.
 //
-// LDS per workgroup (S = blockDim.x + 1 doubles per row): randoms nx_max rows, momenta
-// 4 nout_max rows, then the save banks when LDS is set.  A tile of `count` events is done in
-// sub-tiles of blockDim.x events.
-template <int NBANK, bool LDS, int WPE>
+// LDS per workgroup (S = blockDim.x + 1 doubles per row): momenta 4 nout_max rows, then the
+// save banks when LDS is set.  A tile of `count` events is done in sub-tiles of blockDim.x
+// events.
+template <int NBANK, bool LDS, int WPE, bool XREG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
-                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
-                    int nout_max) {
+                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nout_max) {
   const int B = (int)blockDim.x;
   const int S = B + 1;
-  double* xs = lds;
-  double* ps = lds + nx_max * S;
+  double* ps = lds;
   double* bank_base = ps + 4 * nout_max * S;
   const int lane = (int)threadIdx.x;
   for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
@@ -266,14 +254,13 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
     for (int base = 0; base < td.count; base += B) {
       const int cnt = min(B, td.count - base);
       const int64_t e0 = td.e0 + base;
-      __syncthreads();   // the previous sub-tile's tables have been read
-      stage_randoms(sd.x + e0 * cd.n_x, cnt, cd, sd.align16 & 1, xs, S);
-      __syncthreads();
       const bool valid = lane < cnt;
       double factor;
       bool ok;
+      __syncthreads();   // the previous sub-tile's momentum table has been read
       // lanes past the end repeat the last event and store nothing
-      forward_event<NBANK, LDS>(cd, prog, xs + (valid ? lane : cnt - 1), ps + lane, S, bank_base, factor, ok);
+      forward_event<NBANK, LDS, XREG>(cd, prog, sd.x + (e0 + (valid ? lane : cnt - 1)) * cd.n_x, ps + lane, S,
+                                      bank_base, factor, ok);
       if (valid) {
         as_global(sd.factor)[e0 + lane] = factor;
         as_global(sd.volume)[e0 + lane] = cd.volume;
@@ -288,33 +275,37 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
 }  // namespace
 
 namespace {
-using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int, int);
+using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int);
 
-template <bool LDS, int WPE>
+template <bool LDS, int WPE, bool XREG>
 FwdKernel pick_banks(int n_banks) {
   switch (n_banks) {
-    case 0: return forward_kernel<0, false, WPE>;
-    case 1: return forward_kernel<1, LDS, WPE>;
-    case 2: return forward_kernel<2, LDS, WPE>;
-    default: return forward_kernel<3, LDS, WPE>;
+    case 0: return forward_kernel<0, false, WPE, XREG>;
+    case 1: return forward_kernel<1, LDS, WPE, XREG>;
+    case 2: return forward_kernel<2, LDS, WPE, XREG>;
+    default: return forward_kernel<3, LDS, WPE, XREG>;
   }
 }
 
-template <bool LDS>
+template <bool LDS, bool XREG>
 FwdKernel pick_wpe(int n_banks, int wpe) {
   switch (wpe) {
-    case 3: return pick_banks<LDS, 3>(n_banks);
-    case 4: return pick_banks<LDS, 4>(n_banks);
-    case 5: return pick_banks<LDS, 5>(n_banks);
-    default: return pick_banks<LDS, 0>(n_banks);
+    case 3: return pick_banks<LDS, 3, XREG>(n_banks);
+    case 4: return pick_banks<LDS, 4, XREG>(n_banks);
+    default: return pick_banks<LDS, 0, XREG>(n_banks);
   }
+}
+
+template <bool XREG>
+FwdKernel pick(const DevicePlan& dp, const LaunchConfig& lc, bool lds_bank) {
+  return lds_bank ? pick_wpe<true, XREG>(dp.n_banks, lc.waves_per_eu) : pick_wpe<false, XREG>(dp.n_banks, lc.waves_per_eu);
 }
 
 bool use_lds_bank(const DevicePlan& dp, const LaunchConfig& lc) { return lc.lds_bank && dp.n_banks > 0; }
 }  // namespace
 
 size_t forward_lds_bytes(const DevicePlan& dp, const LaunchConfig& lc) {
-  const size_t rows = (size_t)dp.nx_max + 4 * (size_t)dp.nout_max;
+  const size_t rows = 4 * (size_t)dp.nout_max;
   size_t bytes = rows * (size_t)(lc.block + 1) * sizeof(double);
   if (use_lds_bank(dp, lc)) bytes += (size_t)kBankWords * sizeof(double) * lc.block * dp.n_banks;
   return bytes;
@@ -340,9 +331,9 @@ hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const La
   const int n = t_end - t_begin;
   const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
   const bool lds_bank = use_lds_bank(dp, lc);
-  FwdKernel k = lds_bank ? pick_wpe<true>(dp.n_banks, lc.waves_per_eu) : pick_wpe<false>(dp.n_banks, lc.waves_per_eu);
+  FwdKernel k = dp.nx_max <= kXReg ? pick<true>(dp, lc, lds_bank) : pick<false>(dp, lc, lds_bank);
   hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,
-                     dp.nodes, dp.nx_max, dp.nout_max);
+                     dp.nodes, dp.nout_max);
   return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ struct LaunchConfig {
   int block = 256;       // threads per workgroup (multiple of 64, <= 256)
   int ept = 1;           // events per thread per tile
   int grid = 0;          // workgroups; 0 = one per tile, else a persistent grid that strides
-  int waves_per_eu = 0;  // register-allocation occupancy target: 0 (compiler's choice), 3, 4, 5
+  int waves_per_eu = 0;  // register-allocation occupancy target: 0 (compiler's choice), 3, 4
   int lds_bank = 0;      // 1: keep the save banks of deferred subtrees in LDS instead of VGPRs
 };
 

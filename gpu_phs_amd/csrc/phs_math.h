@@ -6,8 +6,8 @@
 // every double and cost 40-190 VALU instructions each; worse, inside the kernels' branch loop
 // the compiler hoists their polynomial coefficients out of the loop into vector registers, some
 // sixty of them for the whole set, which halves the number of resident waves.  Here every
-// coefficient goes through pin(), which makes the compiler materialise it in a scalar register
-// pair right where it is used (two scalar moves, issued beside the vector work) instead.
+// coefficient goes through GPHS_PIN, which materialises it in a scalar register pair right
+// where it is used (two scalar moves, issued beside the vector work) instead.
 //
 // Methods (all with explicit fused multiply-adds, so they do not depend on contraction):
 //   * sin/cos: reduction to |theta| <= pi/4 -- exact for turns (t = 4x, r = t - rint(t));
@@ -23,6 +23,8 @@
 // cos, exp, log) and 3 (tan).  tan(-0) is +0.  The header has no HIP dependency, so the same
 // code is compiled for the host in those tests.
 #pragma once
+
+#include <stdint.h>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -67,34 +69,43 @@ constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
 constexpr double kLog2e = 0x1.71547652b82fep+0;
 constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;
 
-// A constant materialised in a scalar register pair at its point of use (see the file comment).
-GPHS_MATH_FN double pin(double c) {
+// A constant materialised in a scalar register pair at its point of use (see the file comment):
+// two scalar moves inside volatile asm, which the compiler neither hoists out of loops nor
+// places before `after` has been computed.
+template <uint64_t B>
+GPHS_MATH_FN double pin_bits(double after) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  asm volatile("" : "+s"(c));
+  uint32_t lo, hi;
+  asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"((int32_t)(uint32_t)B), "v"(after));
+  asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"((int32_t)(uint32_t)(B >> 32)), "v"(after));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+#else
+  (void)after;
+  return __builtin_bit_cast(double, B);
 #endif
-  return c;
 }
+#define GPHS_PIN(c, after) ::gphs::math::pin_bits<__builtin_bit_cast(uint64_t, (double)(c))>(after)
 
 // sin and cos of |t| <= pi/4.
 GPHS_MATH_FN void sincos_reduced(double t, double& s, double& c) {
   const double t2 = t * t;
-  double ps = pin(kS17);
-  ps = __builtin_fma(ps, t2, pin(kS15));
-  ps = __builtin_fma(ps, t2, pin(kS13));
-  ps = __builtin_fma(ps, t2, pin(kS11));
-  ps = __builtin_fma(ps, t2, pin(kS9));
-  ps = __builtin_fma(ps, t2, pin(kS7));
-  ps = __builtin_fma(ps, t2, pin(kS5));
-  ps = __builtin_fma(ps, t2, pin(kS3));
+  double ps = GPHS_PIN(kS17, t2);
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS15, ps));
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS13, ps));
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS11, ps));
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS9, ps));
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS7, ps));
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS5, ps));
+  ps = __builtin_fma(ps, t2, GPHS_PIN(kS3, ps));
   s = __builtin_fma(t * t2, ps, t);
-  double pc = pin(kC18);
-  pc = __builtin_fma(pc, t2, pin(kC16));
-  pc = __builtin_fma(pc, t2, pin(kC14));
-  pc = __builtin_fma(pc, t2, pin(kC12));
-  pc = __builtin_fma(pc, t2, pin(kC10));
-  pc = __builtin_fma(pc, t2, pin(kC8));
-  pc = __builtin_fma(pc, t2, pin(kC6));
-  pc = __builtin_fma(pc, t2, pin(kC4));
+  double pc = GPHS_PIN(kC18, t2);
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC16, pc));
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC14, pc));
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC12, pc));
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC10, pc));
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC8, pc));
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC6, pc));
+  pc = __builtin_fma(pc, t2, GPHS_PIN(kC4, pc));
   c = __builtin_fma(t2 * t2, pc, __builtin_fma(t2, -0.5, 1.0));
 }
 
@@ -104,7 +115,7 @@ GPHS_MATH_FN void sincos_turns(double x, double& s, double& c) {
   const double q = __builtin_rint(t);
   const double r = t - q;  // exact, |r| <= 1/2
   double s0, c0;
-  sincos_reduced(__builtin_fma(r, pin(kPio2Hi), r * pin(kPio2Lo)), s0, c0);
+  sincos_reduced(__builtin_fma(r, GPHS_PIN(kPio2Hi, r), r * GPHS_PIN(kPio2Lo, r)), s0, c0);
   const int n = (int)q;  // quadrant; n & 3 is right for negative n too
   const double a = (n & 1) ? c0 : s0;
   const double b = (n & 1) ? s0 : c0;
@@ -114,8 +125,8 @@ GPHS_MATH_FN void sincos_turns(double x, double& s, double& c) {
 
 // tan(z) for |z| < 2^20 * pi/2; the kernels call it for |z| < pi/2.
 GPHS_MATH_FN double tan_radians(double z) {
-  const double q = __builtin_rint(z * pin(kTwoOverPi));
-  const double r = ((z - q * pin(kCw1)) - q * pin(kCw2)) - q * pin(kCw3);
+  const double q = __builtin_rint(z * GPHS_PIN(kTwoOverPi, z));
+  const double r = ((z - q * GPHS_PIN(kCw1, q)) - q * GPHS_PIN(kCw2, q)) - q * GPHS_PIN(kCw3, q);
   double s0, c0;
   sincos_reduced(r, s0, c0);
   const bool odd = ((int)q) & 1;
@@ -125,19 +136,19 @@ GPHS_MATH_FN double tan_radians(double z) {
 // e^x; overflows to +inf above ~709.8 and underflows to 0 below ~-745 like exp().
 GPHS_MATH_FN double exp_bounded(double x) {
   const double xc = x > 709.8 ? 709.8 : (x < -745.2 ? -745.2 : x);   // keeps k in int range
-  const double k = __builtin_rint(xc * pin(kLog2e));
-  const double r = __builtin_fma(-k, pin(kLn2Lo), __builtin_fma(-k, pin(kLn2Hi), xc));
-  double p = pin(1.0 / 6227020800.0);
-  p = __builtin_fma(p, r, pin(1.0 / 479001600.0));
-  p = __builtin_fma(p, r, pin(1.0 / 39916800.0));
-  p = __builtin_fma(p, r, pin(1.0 / 3628800.0));
-  p = __builtin_fma(p, r, pin(1.0 / 362880.0));
-  p = __builtin_fma(p, r, pin(1.0 / 40320.0));
-  p = __builtin_fma(p, r, pin(1.0 / 5040.0));
-  p = __builtin_fma(p, r, pin(1.0 / 720.0));
-  p = __builtin_fma(p, r, pin(1.0 / 120.0));
-  p = __builtin_fma(p, r, pin(1.0 / 24.0));
-  p = __builtin_fma(p, r, pin(1.0 / 6.0));
+  const double k = __builtin_rint(xc * GPHS_PIN(kLog2e, xc));
+  const double r = __builtin_fma(-k, GPHS_PIN(kLn2Lo, k), __builtin_fma(-k, GPHS_PIN(kLn2Hi, k), xc));
+  double p = GPHS_PIN(1.0 / 6227020800.0, r);
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 479001600.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 39916800.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 3628800.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 362880.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 40320.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 5040.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 720.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 120.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 24.0, p));
+  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 6.0, p));
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r * r, r) + 1.0;   // 1 + r + r^2 (1/2 + r/6 + ...)
   // 2^k in two steps, so that 2^k itself need not be representable near the ends of the range
@@ -150,23 +161,23 @@ GPHS_MATH_FN double exp_bounded(double x) {
 GPHS_MATH_FN double log_bounded(double x) {
   int e;
   double m = __builtin_frexp(x, &e);   // x = m 2^e, m in [1/2, 1)
-  if (m < pin(kSqrtHalf)) { m = 2 * m; e -= 1; }
+  if (m < GPHS_PIN(kSqrtHalf, m)) { m = 2 * m; e -= 1; }
   const double f = m - 1.0;             // exact
   const double s = f / (2.0 + f);
   const double z = s * s;
-  double p = pin(1.0 / 21.0);
-  p = __builtin_fma(p, z, pin(1.0 / 19.0));
-  p = __builtin_fma(p, z, pin(1.0 / 17.0));
-  p = __builtin_fma(p, z, pin(1.0 / 15.0));
-  p = __builtin_fma(p, z, pin(1.0 / 13.0));
-  p = __builtin_fma(p, z, pin(1.0 / 11.0));
-  p = __builtin_fma(p, z, pin(1.0 / 9.0));
-  p = __builtin_fma(p, z, pin(1.0 / 7.0));
-  p = __builtin_fma(p, z, pin(1.0 / 5.0));
-  p = __builtin_fma(p, z, pin(1.0 / 3.0));
+  double p = GPHS_PIN(1.0 / 21.0, z);
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 19.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 17.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 15.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 13.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 11.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 9.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 7.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 5.0, p));
+  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 3.0, p));
   const double lm = __builtin_fma(2 * s * z, p, 2 * s);     // 2 atanh(s)
   const double de = (double)e;
-  const double y = __builtin_fma(de, pin(kLn2Hi), __builtin_fma(de, pin(kLn2Lo), lm));
+  const double y = __builtin_fma(de, GPHS_PIN(kLn2Hi, lm), __builtin_fma(de, GPHS_PIN(kLn2Lo, lm), lm));
   if (!(x > 0)) return x == 0 ? -__builtin_inf() : __builtin_nan("");
   return x == __builtin_inf() ? x : y;
 }

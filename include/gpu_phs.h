@@ -134,9 +134,10 @@ int gpu_phs_plan_destroy(int plan);
 /* Launch shape: workgroup size (64..256, multiple of 64), events per thread per tile
  * (1..16), persistent grid (0 = one workgroup per tile), tile order (0 = segment order,
  * 1 = tiles of the deepest trees first), kernel variant: waves_per_eu = register budget for
- * at least that many waves per SIMD (0 = compiler's choice, 3, 4, 5), lds_bank = 1 keeps
+ * at least that many waves per SIMD (0 = compiler's choice, 3, 4), lds_bank = 1 keeps
  * the boosts of deferred subtrees in LDS instead of VGPRs.  Changing block, ept or order
- * rebuilds the tile table. */
+ * rebuilds the tile table.  A block whose LDS tables (4 n_out rows of block + 1 doubles, plus
+ * the LDS save banks) exceed the device's per-workgroup LDS is rejected. */
 int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds_bank);
 int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu,
                             int* lds_bank);
