@@ -74,14 +74,14 @@ __device__ __forceinline__ void ct_map(int kind, double x, double b0, double b1,
   if (kind == CK_FLAT) {
     double tmp = 2 * (1 - x);
     ct = 1 - tmp;
-    st = sqrt(tmp * (2 - tmp));
+    st = math::sqrt_phys(tmp * (2 - tmp));
     f = 1;
   } else {
     bool lo = x < 0.5;
     double tmp = b0 * math::exp_bounded((lo ? 2 * x : 2 * (1 - x)) * b1);
     double c = lo ? (tmp - b0 - 1) : (-(tmp - b0) + 1);
     bool in = c >= -1 && c <= 1;
-    double sq = sqrt(1 - c * c);
+    double sq = math::sqrt_phys(1 - c * c);
     ct = in ? c : 1.0;
     st = in ? sq : 0.0;
     f = in ? tmp * b1 : 0.0;

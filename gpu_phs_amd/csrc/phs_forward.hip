@@ -119,7 +119,7 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
   double xv[XREG ? kXReg : 1];
   if constexpr (XREG) {
 #pragma unroll
-    for (int k = 0; k < kXReg; k++) xv[k] = k < cd.n_x ? xrow[k] : 0.0;
+    for (int k = 0; k < kXReg; k++) xv[k] = xrow[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
   }
 #define GPHS_X(k) (XREG ? xv[XREG ? (k) : 0] : xrow[k])
 
@@ -138,21 +138,21 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547) and masses
     double msq0, f0, m1, msq1, f1, m2;
     if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
-    else { msq_map(nc.k[0], GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = sqrt(msq0); }
+    else { msq_map(nc.k[0], GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = math::sqrt_phys(msq0); }
     if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
-    else { msq_map(nc.k[1], GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = sqrt(msq1); }
+    else { msq_map(nc.k[1], GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = math::sqrt_phys(msq1); }
 
     // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
     const double m = cur_m;
     const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
     ok = ok && (lda > 0 && m > m1 + m2 && m <= nc.m_max);
-    const double sl = sqrt(lda);
+    const double sl = math::sqrt_phys(lda);
     const double pk = sl / (2 * m);
     factor *= f0 * f1 * (sl / cur_msq);
 
     // angles and the boost of this branch (phs_cpu.cpp:299-340)
     const double bg = m > 0 ? cur_p / m : 0.0;
-    const double g = sqrt(1 + bg * bg);
+    const double g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
     math::sincos_turns(GPHS_X(nc.x_phi), sp, cp);  // phi = 2 pi x
     double ct, st, fct;
@@ -171,7 +171,7 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     for (int c = 0; c < 2; c++) {
       if (nc.k[c].leaf) {
         const double pc = c == 0 ? pk : -pk;
-        const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : sqrt(nc.k[c].msq_leaf + pc * pc);
+        const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
         double q0 = M[0] * E + M[3] * pc;
         const double q1 = M[4] * E + M[7] * pc;
         const double q2 = M[8] * E + M[11] * pc;

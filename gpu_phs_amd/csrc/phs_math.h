@@ -86,26 +86,60 @@ GPHS_MATH_FN double pin_bits(double after) {
 }
 #define GPHS_PIN(c, after) ::gphs::math::pin_bits<__builtin_bit_cast(uint64_t, (double)(c))>(after)
 
+// a * b + c with c in a scalar register pair: one v_fma_f64.  (Left to itself the compiler
+// picks v_fmac_f64, whose addend must be the destination vector register, and copies the
+// scalar constant into vector registers first: two extra moves per Horner step.)
+GPHS_MATH_FN double fma_sc(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
+#else
+  return __builtin_fma(a, b, c);
+#endif
+}
+
+// sqrt(x): the reciprocal square root estimate refined by one Goldschmidt step and two Newton
+// corrections -- the compiler's own sequence for a double sqrt without its rescaling of inputs
+// below 2^-767, which the kernels' arguments (squared masses and momenta of physical size,
+// or exactly 0) never reach; below 2^-1022 the result is not accurate.  sqrt(0) = 0,
+// sqrt(+inf) = +inf, sqrt(x < 0) = NaN.
+GPHS_MATH_FN double sqrt_phys(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return (x == 0 || x == __builtin_inf()) ? x : g;
+#else
+  return __builtin_sqrt(x);
+#endif
+}
+
 // sin and cos of |t| <= pi/4.
 GPHS_MATH_FN void sincos_reduced(double t, double& s, double& c) {
   const double t2 = t * t;
-  double ps = GPHS_PIN(kS17, t2);
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS15, ps));
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS13, ps));
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS11, ps));
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS9, ps));
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS7, ps));
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS5, ps));
-  ps = __builtin_fma(ps, t2, GPHS_PIN(kS3, ps));
+  // the first step as a product and a sum, so that both constants can stay scalar
+  double ps = t2 * GPHS_PIN(kS17, t2) + GPHS_PIN(kS15, t2);
+  ps = fma_sc(ps, t2, GPHS_PIN(kS13, ps));
+  ps = fma_sc(ps, t2, GPHS_PIN(kS11, ps));
+  ps = fma_sc(ps, t2, GPHS_PIN(kS9, ps));
+  ps = fma_sc(ps, t2, GPHS_PIN(kS7, ps));
+  ps = fma_sc(ps, t2, GPHS_PIN(kS5, ps));
+  ps = fma_sc(ps, t2, GPHS_PIN(kS3, ps));
   s = __builtin_fma(t * t2, ps, t);
-  double pc = GPHS_PIN(kC18, t2);
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC16, pc));
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC14, pc));
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC12, pc));
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC10, pc));
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC8, pc));
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC6, pc));
-  pc = __builtin_fma(pc, t2, GPHS_PIN(kC4, pc));
+  double pc = t2 * GPHS_PIN(kC18, t2) + GPHS_PIN(kC16, t2);
+  pc = fma_sc(pc, t2, GPHS_PIN(kC14, pc));
+  pc = fma_sc(pc, t2, GPHS_PIN(kC12, pc));
+  pc = fma_sc(pc, t2, GPHS_PIN(kC10, pc));
+  pc = fma_sc(pc, t2, GPHS_PIN(kC8, pc));
+  pc = fma_sc(pc, t2, GPHS_PIN(kC6, pc));
+  pc = fma_sc(pc, t2, GPHS_PIN(kC4, pc));
   c = __builtin_fma(t2 * t2, pc, __builtin_fma(t2, -0.5, 1.0));
 }
 
@@ -138,17 +172,16 @@ GPHS_MATH_FN double exp_bounded(double x) {
   const double xc = x > 709.8 ? 709.8 : (x < -745.2 ? -745.2 : x);   // keeps k in int range
   const double k = __builtin_rint(xc * GPHS_PIN(kLog2e, xc));
   const double r = __builtin_fma(-k, GPHS_PIN(kLn2Lo, k), __builtin_fma(-k, GPHS_PIN(kLn2Hi, k), xc));
-  double p = GPHS_PIN(1.0 / 6227020800.0, r);
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 479001600.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 39916800.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 3628800.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 362880.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 40320.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 5040.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 720.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 120.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 24.0, p));
-  p = __builtin_fma(p, r, GPHS_PIN(1.0 / 6.0, p));
+  double p = r * GPHS_PIN(1.0 / 6227020800.0, r) + GPHS_PIN(1.0 / 479001600.0, r);
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 39916800.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 3628800.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 362880.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 40320.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 5040.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 720.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 120.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 24.0, p));
+  p = fma_sc(p, r, GPHS_PIN(1.0 / 6.0, p));
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r * r, r) + 1.0;   // 1 + r + r^2 (1/2 + r/6 + ...)
   // 2^k in two steps, so that 2^k itself need not be representable near the ends of the range
@@ -165,16 +198,15 @@ GPHS_MATH_FN double log_bounded(double x) {
   const double f = m - 1.0;             // exact
   const double s = f / (2.0 + f);
   const double z = s * s;
-  double p = GPHS_PIN(1.0 / 21.0, z);
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 19.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 17.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 15.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 13.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 11.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 9.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 7.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 5.0, p));
-  p = __builtin_fma(p, z, GPHS_PIN(1.0 / 3.0, p));
+  double p = z * GPHS_PIN(1.0 / 21.0, z) + GPHS_PIN(1.0 / 19.0, z);
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 17.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 15.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 13.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 11.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 9.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 7.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 5.0, p));
+  p = fma_sc(p, z, GPHS_PIN(1.0 / 3.0, p));
   const double lm = __builtin_fma(2 * s * z, p, 2 * s);     // 2 atanh(s)
   const double de = (double)e;
   const double y = __builtin_fma(de, GPHS_PIN(kLn2Hi, lm), __builtin_fma(de, GPHS_PIN(kLn2Lo, lm), lm));
