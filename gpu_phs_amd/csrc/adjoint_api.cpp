@@ -24,11 +24,9 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   lay.words = lay.off_acc + 2 * max_nodes;
   if ((size_t)lay.words * 64 * sizeof(double) > 65536)
     return set_error(GPU_PHS_E_TREE, "adjoint: trees too deep for the LDS tape (supported up to 2 -> 6)");
-  pl.adj_lay = lay;
-  pl.adj_part_stride = 2 * max_nodes;
-  pl.adj_segs = pl.segs;
+  std::vector<SegDesc> segs = pl.segs;
   for (int s = 0; s < n_seg; s++) {
-    SegDesc& sd = pl.adj_segs[s];
+    SegDesc& sd = segs[s];
     if (pl.segs[s].n_events > 0 && !adj[s].g_x) return set_error(GPU_PHS_E_ARG, "adjoint: g_x must not be NULL");
     sd.g_p = adj[s].g_p;
     sd.g_f = adj[s].g_f;
@@ -36,6 +34,16 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
     sd.loss = adj[s].loss;
     if (!write_forward) { sd.p = nullptr; sd.factor = nullptr; sd.volume = nullptr; sd.ok = nullptr; }
   }
+  // the same buffers as the previous call: the device tables are current
+  if (pl.adj_valid && segs.size() == pl.adj_segs.size() &&
+      std::memcmp(segs.data(), pl.adj_segs.data(), segs.size() * sizeof(SegDesc)) == 0)
+    return GPU_PHS_OK;
+  pl.adj_valid = false;
+  pl.adj_lay = lay;
+  pl.adj_part_stride = 2 * max_nodes;
+  pl.adj_segs = std::move(segs);
+  // one wavefront per 64 events of a tile
+  pl.adj_sub = std::max(1, (pl.lc.block * pl.lc.ept + 63) / 64);
   // tiles of every channel, in tile order
   std::vector<int> off(n_ch + 1, 0), list(n_t);
   for (const auto& t : pl.tiles) off[pl.segs[t.seg].channel + 1]++;
@@ -44,7 +52,8 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   for (int t = 0; t < n_t; t++) list[fill[pl.segs[pl.tiles[t].seg].channel]++] = t;
 
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t b_segs = al(n_seg * sizeof(SegDesc)), b_part = al((size_t)std::max(n_t, 1) * pl.adj_part_stride * sizeof(double));
+  const size_t b_segs = al(n_seg * sizeof(SegDesc));
+  const size_t b_part = al((size_t)std::max(n_t, 1) * pl.adj_sub * pl.adj_part_stride * sizeof(double));
   const size_t b_list = al(std::max(n_t, 1) * sizeof(int)), b_off = al((n_ch + 1) * sizeof(int));
   const size_t total = b_segs + b_part + b_list + b_off;
   if (total > pl.adj_bytes) {
@@ -67,6 +76,7 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   pl.adj_part = (double*)(base + b_segs);
   pl.adj_chan_tiles = (int*)(base + b_segs + b_part);
   pl.adj_chan_off = (int*)(base + b_segs + b_part + b_list);
+  pl.adj_valid = true;
   return GPU_PHS_OK;
 }
 
@@ -116,7 +126,7 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
       return set_error(GPU_PHS_E_ARG, "adjoint: the built-in loss assumes symmetric beams");
   int rc = adjoint_prepare(*pl, adj, write_forward);
   if (rc) return rc;
-  hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, loss_mode, pl->adj_part, pl->adj_part_stride,
+  hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, loss_mode, pl->adj_part, pl->adj_part_stride, pl->adj_sub,
                                 pl->adj_chan_tiles, pl->adj_chan_off, (int)pl->chans.size(), gparam, 0,
                                 (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "adjoint launch");

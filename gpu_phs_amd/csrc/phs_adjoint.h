@@ -362,16 +362,17 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
       if (E > 0) gpc += gE * pc / E;
       g_pk += c == 0 ? gpc : -gpc;
     }
-    // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros only)
-    const double L[16] = {g, -bg * ce.st, 0, bg * ce.ct, 0, ce.ct * cp, -sp, ce.st * cp,
-                          0, ce.ct * sp, cp, ce.st * sp, bg, -g * ce.st, 0, g * ce.ct};
+    // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros of L1 only)
+    const double l01 = -bg * ce.st, l03 = bg * ce.ct, l11 = ce.ct * cp, l12 = -sp, l13 = ce.st * cp;
+    const double l21 = ce.ct * sp, l22 = cp, l23 = ce.st * sp, l31 = -g * ce.st, l33 = g * ce.ct;
     double gP[16];
-    for (int ii = 0; ii < 4; ii++)
-      for (int k = 0; k < 4; k++) {
-        double acc = 0;
-        for (int j = 0; j < 4; j++) acc += G[4 * ii + j] * L[4 * k + j];
-        gP[4 * ii + k] = acc;
-      }
+    for (int ii = 0; ii < 4; ii++) {
+      const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
+      gP[4 * ii + 0] = G0 * g + G1 * l01 + G3 * l03;
+      gP[4 * ii + 1] = G1 * l11 + G2 * l12 + G3 * l13;
+      gP[4 * ii + 2] = G1 * l21 + G2 * l22 + G3 * l23;
+      gP[4 * ii + 3] = G0 * bg + G1 * l31 + G3 * l33;
+    }
     auto gl = [&](int k, int j) {
       double acc = 0;
       for (int ii = 0; ii < 4; ii++) acc += P[4 * ii + k] * G[4 * ii + j];

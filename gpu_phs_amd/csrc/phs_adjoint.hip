@@ -33,7 +33,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end,
+__global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict__ tiles, int n_units, int n_sub,
                                                      const SegDesc* __restrict__ segs,
                                                      const ChannelDesc* __restrict__ chans,
                                                      const NodeCmd* __restrict__ nodes, int loss_mode,
@@ -42,14 +42,17 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
   const int lane = threadIdx.x;
   LdsStore S{lds + lane, 64, lay.off_in, lay.off_st};
   LdsAcc acc{lds + lay.off_acc * 64 + lane, 64};
-  for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
+  // unit u = sub-tile (u % n_sub) of tile (u / n_sub): events [64 sub, 64 sub + 64) of the tile
+  for (int u = (int)blockIdx.x; u < n_units; u += (int)gridDim.x) {
+    const int t = u / n_sub, sub = u - t * n_sub;
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
     const ChannelDesc cd = chans[sd.channel];
     const NodeCmd* prog = nodes + cd.node0;
     const int np = 2 * cd.n_nodes;
     for (int v = 0; v < np; v++) acc[v] = 0;
-    for (int local = lane; local < td.count; local += 64) {
+    const int end = min(td.count, 64 * (sub + 1));
+    for (int local = 64 * sub + lane; local < end; local += 64) {
       const int64_t e = td.e0 + local;
       const double* xr = sd.x + e * cd.n_x;
       const double* gp = sd.g_p ? sd.g_p + e * (int64_t)cd.n_out * 4 : nullptr;
@@ -64,42 +67,62 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
     }
     for (int v = 0; v < np; v++) {
       const double tot = wave_sum(acc[v]);
-      if (lane == 0) gpart[(int64_t)t * part_stride + v] = tot;
+      if (lane == 0) gpart[(int64_t)u * part_stride + v] = tot;
     }
   }
 }
 
-// gparam[2 * slot + w] = sum over the tiles of the slot's channel, in tile order (every slot is
-// written, channels without events get 0).
-__global__ void param_reduce_kernel(const double* __restrict__ gpart, int part_stride,
-                                    const int* __restrict__ chan_tiles, const int* __restrict__ chan_tile_off,
-                                    const ChannelDesc* __restrict__ chans, int n_chans, double* __restrict__ gparam) {
+// gparam[2 * slot + w] = sum over the sub-tiles of the tiles of the slot's channel (every slot
+// is written, channels without events get 0).  One workgroup per channel: thread t adds the
+// units t, t + 256, ... and the 256 partial sums are combined by a fixed tree, so the result
+// does not depend on scheduling.
+constexpr int kReduceThreads = 256;
+__global__ __launch_bounds__(kReduceThreads) void param_reduce_kernel(
+    const double* __restrict__ gpart, int part_stride, int n_sub, const int* __restrict__ chan_tiles,
+    const int* __restrict__ chan_tile_off, const ChannelDesc* __restrict__ chans, int n_chans,
+    double* __restrict__ gparam) {
+  __shared__ double red[kReduceThreads];
   const int c = blockIdx.x;
   if (c >= n_chans) return;
   const int np = 2 * chans[c].n_nodes;
-  const int v = threadIdx.x;
-  if (v >= np) return;
-  double sum = 0;
-  for (int k = chan_tile_off[c]; k < chan_tile_off[c + 1]; k++) sum += gpart[(int64_t)chan_tiles[k] * part_stride + v];
-  gparam[2 * chans[c].param0 + v] = sum;
+  const int t0 = chan_tile_off[c];
+  const int n_units = (chan_tile_off[c + 1] - t0) * n_sub;
+  const int tid = threadIdx.x;
+  for (int v = 0; v < np; v++) {
+    double sum = 0;
+    for (int k = tid; k < n_units; k += kReduceThreads) {
+      const int tile = chan_tiles[t0 + k / n_sub];
+      sum += gpart[((int64_t)tile * n_sub + k % n_sub) * part_stride + v];
+    }
+    red[tid] = sum;
+    __syncthreads();
+    for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) gparam[2 * chans[c].param0 + v] = red[0];
+    __syncthreads();
+  }
 }
 
 }  // namespace
 
 hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
-                          const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam, int grid,
-                          hipStream_t stream) {
+                          int n_sub, const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam,
+                          int grid, hipStream_t stream) {
   const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
-  if (shmem > 65536) return hipErrorInvalidValue;
-  if (dp.n_tiles > 0) {
-    const int g = grid > 0 && grid < dp.n_tiles ? grid : dp.n_tiles;
-    hipLaunchKernelGGL(adjoint_kernel, dim3(g), dim3(64), shmem, stream, dp.tiles, 0, dp.n_tiles, dp.segs, dp.chans,
-                       dp.nodes, loss_mode, gpart, part_stride, lay);
+  if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
+  const int64_t units = (int64_t)dp.n_tiles * n_sub;
+  if (units > 0x7fffffff) return hipErrorInvalidValue;
+  if (units > 0) {
+    const int g = grid > 0 && grid < units ? grid : (int)units;
+    hipLaunchKernelGGL(adjoint_kernel, dim3(g), dim3(64), shmem, stream, dp.tiles, (int)units, n_sub, dp.segs,
+                       dp.chans, dp.nodes, loss_mode, gpart, part_stride, lay);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(param_reduce_kernel, dim3(n_chans), dim3(64), 0, stream, gpart, part_stride, chan_tiles,
-                     chan_tile_off, dp.chans, n_chans, gparam);
+  hipLaunchKernelGGL(param_reduce_kernel, dim3(n_chans), dim3(kReduceThreads), 0, stream, gpart, part_stride, n_sub,
+                     chan_tiles, chan_tile_off, dp.chans, n_chans, gparam);
   return hipGetLastError();
 }
 

@@ -82,6 +82,7 @@ int plan_build(PlanObj& pl) {
   pl.seg_tile_begin.clear(); pl.seg_tile_end.clear();
   pl.chan_process.clear(); pl.chan_channel.clear();
   if (pl.adj_dev) { (void)hipFree(pl.adj_dev); pl.adj_dev = nullptr; pl.adj_bytes = 0; }
+  pl.adj_valid = false;
   pl.n_banks = 0;
   std::map<std::pair<int, int>, int> chan_index;
   for (const auto& us : pl.user) {
