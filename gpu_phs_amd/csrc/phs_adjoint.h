@@ -157,7 +157,7 @@ GPHS_HD CtEval ct_eval(int kind, double x, double b0, double b1) {
   if (kind == CK_FLAT) {
     r.tmp = 2 * (1 - x);
     r.ct = 1 - r.tmp;
-    r.st = sqrt(r.tmp * (2 - r.tmp));
+    r.st = math::sqrt_phys(r.tmp * (2 - r.tmp));
     r.f = 1;
     r.in = true;
   } else {
@@ -166,7 +166,7 @@ GPHS_HD CtEval ct_eval(int kind, double x, double b0, double b1) {
     r.tmp = b0 * r.e;
     double c = r.lo ? (r.tmp - b0 - 1) : (-(r.tmp - b0) + 1);
     r.in = c >= -1 && c <= 1;
-    double sq = sqrt(1 - c * c);
+    double sq = math::sqrt_phys(1 - c * c);
     r.ct = r.in ? c : 1.0;
     r.st = r.in ? sq : 0.0;
     r.f = r.in ? r.tmp * b1 : 0.0;
@@ -201,6 +201,7 @@ GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double 
   const double l11 = ct * cp, l12 = -sp, l13 = st * cp;
   const double l21 = ct * sp, l22 = cp, l23 = st * sp;
   const double l31 = -g * st, l33 = g * ct;
+#pragma unroll
   for (int i = 0; i < 4; i++) {
     const double r0 = M[4 * i], r1 = M[4 * i + 1], r2 = M[4 * i + 2], r3 = M[4 * i + 3];
     M[4 * i + 0] = r0 * g + r3 * bg;
@@ -216,14 +217,14 @@ GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double 
 //   double& st(int level, int k) stash level: 16 matrix adjoints + 3 scalar adjoints
 
 // Forward pass of one event with tape recording, then the reverse sweep.
-//   xr: the event's n_x randoms; loss_mode 0: cotangents gp [n_out*4] (may be null) and gf
+//   xr: the event's n_x randoms (anything indexable); loss_mode 0: cotangents gp [n_out*4] (may be null) and gf
 //   are given; loss_mode 1: the loss is factor * sum_j (p_jx^2 + p_jy^2) / s and its
 //   cotangents are formed here.  Writes p_out [n_out*4] (may be null), *f_out, *loss_out
 //   (mode 1), gx [n_x] (zero for a non-ok event), and adds d/d(M, Gamma) of every branch to
 //   gpar[2 * node], gpar[2 * node + 1] (visit index; `GPar` is any indexable accumulator).
 //   Returns ok.
-template <class Store, class GPar>
-GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const double* xr, int loss_mode,
+template <class Store, class GPar, class XR>
+GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR& xr, int loss_mode,
                            const double* gp, double gf_in, double* p_out, double* f_out, double* loss_out,
                            double* gx, GPar& gpar, Store& S) {
   const double s = cd.s;
@@ -237,27 +238,30 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     const NodeCmd& nc = prog[i];
     S.in(i, 0) = cur_msq; S.in(i, 1) = cur_m; S.in(i, 2) = cur_p;
     double msq[2], f[2], mk[2];
+#pragma unroll
     for (int c = 0; c < 2; c++) {
       if (nc.k[c].leaf) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
-      else { MapEval r = msq_eval(nc.k[c], xr[nc.k[c].xi], s); msq[c] = r.msq; f[c] = r.f; mk[c] = sqrt(r.msq); }
+      else { MapEval r = msq_eval(nc.k[c], xr[nc.k[c].xi], s); msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq); }
     }
     const double m = cur_m;
     const double lda = (cur_msq - msq[0] - msq[1]) * (cur_msq - msq[0] - msq[1]) - 4 * msq[0] * msq[1];
     ok = ok && (lda > 0 && m > mk[0] + mk[1] && m <= nc.m_max);
-    const double sl = sqrt(lda), pk = sl / (2 * m);
+    const double sl = math::sqrt_phys(lda), pk = sl / (2 * m);
     F *= f[0] * f[1] * (sl / cur_msq);
-    const double bg = m > 0 ? cur_p / m : 0.0, g = sqrt(1 + bg * bg);
+    const double bg = m > 0 ? cur_p / m : 0.0, g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
     math::sincos_turns(xr[nc.x_phi], sp, cp);
     const CtEval ce = ct_eval(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1);
     F *= ce.f;
     compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
     if (nc.tape_slot >= 0)
+#pragma unroll
       for (int q = 0; q < 16; q++) S.M(nc.tape_slot, q) = M[q];
+#pragma unroll
     for (int c = 0; c < 2; c++) {
       if (!nc.k[c].leaf) continue;
       const double pc = c == 0 ? pk : -pk;
-      const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : sqrt(nc.k[c].msq_leaf + pc * pc);
+      const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
       double q0 = M[0] * E + M[3] * pc, q1 = M[4] * E + M[7] * pc, q2 = M[8] * E + M[11] * pc,
              q3 = M[12] * E + M[15] * pc;
       if (cd.bg_lab != 0) {
@@ -271,6 +275,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
       }
     }
     if (nc.push >= 0) {   // the deferred daughter's boost and kinematics
+#pragma unroll
       for (int q = 0; q < 16; q++) S.st(depth, q) = M[q];
       S.st(depth, 16) = msq[nc.push]; S.st(depth, 17) = mk[nc.push]; S.st(depth, 18) = nc.push == 0 ? pk : -pk;
       depth++;
@@ -279,6 +284,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
       cur_msq = msq[nc.desc]; cur_m = mk[nc.desc]; cur_p = nc.desc == 0 ? pk : -pk;
     } else if (depth > 0) {
       depth--;
+#pragma unroll
       for (int q = 0; q < 16; q++) M[q] = S.st(depth, q);
       cur_msq = S.st(depth, 16); cur_m = S.st(depth, 17); cur_p = S.st(depth, 18);
     }
@@ -300,11 +306,13 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     double G[16] = {0};
     double g_msq[2] = {0, 0}, g_m[2] = {0, 0}, g_pk = 0;
     if (nc.desc >= 0) {
+#pragma unroll
       for (int q = 0; q < 16; q++) G[q] += Gc[q];
       g_msq[nc.desc] += gc_msq; g_m[nc.desc] += gc_m; g_pk += nc.desc == 0 ? gc_p : -gc_p;
     }
     if (nc.push >= 0) {
       depth--;
+#pragma unroll
       for (int q = 0; q < 16; q++) G[q] += S.st(depth, q);
       g_msq[nc.push] += S.st(depth, 16); g_m[nc.push] += S.st(depth, 17);
       g_pk += nc.push == 0 ? S.st(depth, 18) : -S.st(depth, 18);
@@ -312,37 +320,47 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     // recompute the branch's local quantities
     const double a = S.in(i, 0), m = S.in(i, 1), p0 = S.in(i, 2);
     double P[16];
-    if (nc.parent < 0) { for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0; }
-    else { const int ps = prog[nc.parent].tape_slot; for (int q = 0; q < 16; q++) P[q] = S.M(ps, q); }
+    if (nc.parent < 0) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    } else {
+      const int ps = prog[nc.parent].tape_slot;
+#pragma unroll
+      for (int q = 0; q < 16; q++) P[q] = S.M(ps, q);
+    }
     MapEval r[2];
     double msq[2], f[2], mk[2], xk[2];
+#pragma unroll
     for (int c = 0; c < 2; c++) {
       if (nc.k[c].leaf) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
-      else { xk[c] = xr[nc.k[c].xi]; r[c] = msq_eval(nc.k[c], xk[c], s); msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = sqrt(msq[c]); }
+      else { xk[c] = xr[nc.k[c].xi]; r[c] = msq_eval(nc.k[c], xk[c], s); msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]); }
     }
     const double d = a - msq[0] - msq[1];
     const double lda = d * d - 4 * msq[0] * msq[1];
-    const double sl = sqrt(lda), pk = sl / (2 * m);
-    const double bg = m > 0 ? p0 / m : 0.0, g = sqrt(1 + bg * bg);
+    const double sl = math::sqrt_phys(lda), pk = sl / (2 * m);
+    const double bg = m > 0 ? p0 / m : 0.0, g = math::sqrt_phys(1 + bg * bg);
     const double xphi = xr[nc.x_phi], xct = xr[nc.x_phi + 1];
     double sp, cp;
     math::sincos_turns(xphi, sp, cp);
     const CtEval ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
     double Ma[16];
+#pragma unroll
     for (int q = 0; q < 16; q++) Ma[q] = P[q];
     compose_l1(Ma, g, bg, ce.ct, ce.st, cp, sp);
 
     // final-state daughters: q = Ma[:,0] E + Ma[:,3] pc
+#pragma unroll
     for (int c = 0; c < 2; c++) {
       if (!nc.k[c].leaf) continue;
       const double pc = c == 0 ? pk : -pk;
-      const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : sqrt(nc.k[c].msq_leaf + pc * pc);
+      const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
       // cotangent of the lab-frame momentum, moved back through the (linear) lab boost
       double gq[4];
       if (loss_mode == 1) {
         const double q1 = Ma[4] * E + Ma[7] * pc, q2 = Ma[8] * E + Ma[11] * pc;
         gq[0] = 0; gq[1] = F * 2 * q1 / s; gq[2] = F * 2 * q2 / s; gq[3] = 0;
       } else if (gp) {
+#pragma unroll
         for (int mu = 0; mu < 4; mu++) gq[mu] = gp[4 * nc.k[c].out + mu];
       } else {
         gq[0] = gq[1] = gq[2] = gq[3] = 0;
@@ -353,6 +371,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
         gq[3] = cd.bg_lab * a0 + cd.g_lab * a3;
       }
       double gE = 0, gpc = 0;
+#pragma unroll
       for (int mu = 0; mu < 4; mu++) {
         G[4 * mu + 0] += gq[mu] * E;
         G[4 * mu + 3] += gq[mu] * pc;
@@ -366,6 +385,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     const double l01 = -bg * ce.st, l03 = bg * ce.ct, l11 = ce.ct * cp, l12 = -sp, l13 = ce.st * cp;
     const double l21 = ce.ct * sp, l22 = cp, l23 = ce.st * sp, l31 = -g * ce.st, l33 = g * ce.ct;
     double gP[16];
+#pragma unroll
     for (int ii = 0; ii < 4; ii++) {
       const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
       gP[4 * ii + 0] = G0 * g + G1 * l01 + G3 * l03;
@@ -375,6 +395,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     }
     auto gl = [&](int k, int j) {
       double acc = 0;
+#pragma unroll
       for (int ii = 0; ii < 4; ii++) acc += P[4 * ii + k] * G[4 * ii + j];
       return acc;
     };
@@ -404,6 +425,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     // factor term f0 f1 sl / a
     double gfk[2] = {0, 0};
     if (W != 0) {
+#pragma unroll
       for (int c = 0; c < 2; c++)
         if (f[c] != 0) gfk[c] = W / f[c];
       if (sl != 0) gsl += W / sl;
@@ -416,6 +438,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
     g_msq[1] += -4 * msq[0] * glda - gd;
     g_a += gd;
     // internal daughters: m_k = sqrt(msq_k), then their msq maps
+#pragma unroll
     for (int c = 0; c < 2; c++) {
       if (nc.k[c].leaf) continue;
       if (mk[c] > 0) g_msq[c] += g_m[c] / (2 * mk[c]);
@@ -426,9 +449,11 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const dou
       gpar[2 * nc.k[c].node + 1] += gW;
     }
     // hand the adjoints of this branch's inputs to its parent
+#pragma unroll
     for (int q = 0; q < 16; q++) Gc[q] = gP[q];
     gc_msq = g_a; gc_m = g_mnode; gc_p = g_p0;
     if (i >= 1 && prog[i - 1].desc < 0) {   // reached by a pop in the forward pass
+#pragma unroll
       for (int q = 0; q < 16; q++) S.st(depth, q) = Gc[q];
       S.st(depth, 16) = gc_msq; S.st(depth, 17) = gc_m; S.st(depth, 18) = gc_p;
       depth++;

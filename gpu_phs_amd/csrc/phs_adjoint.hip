@@ -33,6 +33,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+constexpr int kAdjXReg = 8;
+
+// XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
+template <int XN>
 __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict__ tiles, int n_units, int n_sub,
                                                      const SegDesc* __restrict__ segs,
                                                      const ChannelDesc* __restrict__ chans,
@@ -54,12 +58,20 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
     const int end = min(td.count, 64 * (sub + 1));
     for (int local = 64 * sub + lane; local < end; local += 64) {
       const int64_t e = td.e0 + local;
-      const double* xr = sd.x + e * cd.n_x;
+      const double* xg = sd.x + e * cd.n_x;
       const double* gp = sd.g_p ? sd.g_p + e * (int64_t)cd.n_out * 4 : nullptr;
       const double gf = sd.g_f ? sd.g_f[e] : 0.0;
       double* p_out = sd.p ? sd.p + e * (int64_t)cd.n_out * 4 : nullptr;
       double F = 0, L = 0;
-      const bool ok = event_adjoint(cd, prog, xr, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc, S);
+      bool ok;
+      if constexpr (XN > 0) {
+        double xv[XN];
+#pragma unroll
+        for (int k = 0; k < XN; k++) xv[k] = xg[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
+        ok = event_adjoint(cd, prog, xv, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc, S);
+      } else {
+        ok = event_adjoint(cd, prog, xg, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc, S);
+      }
       if (sd.factor) sd.factor[e] = F;
       if (sd.volume) sd.volume[e] = cd.volume;
       if (sd.ok) sd.ok[e] = ok ? 1 : 0;
@@ -116,8 +128,9 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_m
   if (units > 0x7fffffff) return hipErrorInvalidValue;
   if (units > 0) {
     const int g = grid > 0 && grid < units ? grid : (int)units;
-    hipLaunchKernelGGL(adjoint_kernel, dim3(g), dim3(64), shmem, stream, dp.tiles, (int)units, n_sub, dp.segs,
-                       dp.chans, dp.nodes, loss_mode, gpart, part_stride, lay);
+    auto k = dp.nx_max <= kAdjXReg ? adjoint_kernel<kAdjXReg> : adjoint_kernel<0>;
+    hipLaunchKernelGGL(k, dim3(g), dim3(64), shmem, stream, dp.tiles, (int)units, n_sub, dp.segs, dp.chans, dp.nodes,
+                       loss_mode, gpart, part_stride, lay);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
