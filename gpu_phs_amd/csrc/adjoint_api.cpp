@@ -11,19 +11,18 @@ namespace {
 // table carrying the gradient buffers, the per-tile partials and the tiles of each channel.
 int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_forward) {
   const int n_seg = (int)pl.segs.size(), n_ch = (int)pl.chans.size(), n_t = pl.dp.n_tiles;
-  int max_tape = 0, max_nodes = 0, max_banks = 0;
+  int max_nodes = 0, max_banks = 0;
   for (const auto& c : pl.chans) {
-    max_tape = std::max(max_tape, (int)c.n_tape);
     max_nodes = std::max(max_nodes, (int)c.n_nodes);
     max_banks = std::max(max_banks, (int)c.n_banks);
   }
   AdjLayout lay;
-  lay.off_in = 16 * max_tape;
-  lay.off_st = lay.off_in + 3 * max_nodes;
+  lay.off_in = 6 * max_nodes;
+  lay.off_st = lay.off_in + 2 * max_nodes;
   lay.off_acc = lay.off_st + 19 * max_banks;
   lay.words = lay.off_acc + 2 * max_nodes;
   if ((size_t)lay.words * 64 * sizeof(double) > 65536)
-    return set_error(GPU_PHS_E_TREE, "adjoint: trees too deep for the LDS tape (supported up to 2 -> 6)");
+    return set_error(GPU_PHS_E_TREE, "adjoint: trees too large for the LDS tape");
   std::vector<SegDesc> segs = pl.segs;
   for (int s = 0; s < n_seg; s++) {
     SegDesc& sd = segs[s];

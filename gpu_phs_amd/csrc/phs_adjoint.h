@@ -2,10 +2,13 @@
 //
 // No counterpart in the reference.  The forward pass is the one of phs_forward.hip (the
 // reference's phs_cpu.cpp algorithm, visit order of the compiled program); it records a
-// small tape per branch (the boost matrix after the branch and the branch's incoming msq, m,
-// p_decay).  The reverse sweep visits the branches backwards and mirrors the forward's save
-// bank: the adjoint of a deferred subtree waits in a stash until its parent is reached.
-// Local quantities (maps, lambda, angles) are recomputed from the tape instead of stored.
+// small tape per branch: the six numbers of its boost L1 = B_z(bg) R_z(phi) R_y(theta)
+// (gamma, bg, cos theta, sin theta, cos phi, sin phi) and the branch's incoming msq and
+// p_decay.  A branch's parent boost is the product of the L1 of its ancestors, re-formed from
+// the tape where it is needed (a deferred subtree in the forward pass, every branch in the
+// reverse sweep), so no 4x4 matrix is stored per branch.  The reverse sweep visits the
+// branches backwards; the adjoint of a deferred subtree waits in a stash until its parent is
+// reached.  The maps and lambda are recomputed from the randoms instead of stored.
 //
 // Everything here is __host__ __device__ so the same code can be checked on the CPU.
 #pragma once
@@ -212,9 +215,24 @@ GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double 
 }
 
 // Where the per-event adjoint keeps its tape and stash.  `Store` provides
-//   double& M(int slot, int k)   boost after a branch with internal daughters (tape slot)
-//   double& in(int node, int k)  msq, m, p of a branch (k = 0, 1, 2)
+//   double& T(int node, int k)   L1 of a branch: gamma, bg, ct, st, cp, sp (k = 0..5)
+//   double& in(int node, int k)  incoming msq and p_decay of a branch (k = 0, 1)
 //   double& st(int level, int k) stash level: 16 matrix adjoints + 3 scalar adjoints
+
+// P = product of the L1 of the ancestors of branch j (root first): the boost its own L1 is
+// composed onto.  The ancestors are found by walking the parent links (a few scalar loads).
+template <class Store>
+GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16]) {
+  int d = 0;
+  for (int a = prog[j].parent; a >= 0; a = prog[a].parent) d++;
+#pragma unroll
+  for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
+  for (int level = 0; level < d; level++) {
+    int a = prog[j].parent;
+    for (int u = 0; u < d - 1 - level; u++) a = prog[a].parent;
+    compose_l1(P, S.T(a, 0), S.T(a, 1), S.T(a, 2), S.T(a, 3), S.T(a, 4), S.T(a, 5));
+  }
+}
 
 // Forward pass of one event with tape recording, then the reverse sweep.
 //   xr: the event's n_x randoms (anything indexable); loss_mode 0: cotangents gp [n_out*4] (may be null) and gf
@@ -236,7 +254,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
   int depth = 0;
   for (int i = 0; i < N; i++) {
     const NodeCmd& nc = prog[i];
-    S.in(i, 0) = cur_msq; S.in(i, 1) = cur_m; S.in(i, 2) = cur_p;
+    S.in(i, 0) = cur_msq; S.in(i, 1) = cur_p;
     double msq[2], f[2], mk[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
@@ -254,9 +272,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     const CtEval ce = ct_eval(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1);
     F *= ce.f;
     compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
-    if (nc.tape_slot >= 0)
-#pragma unroll
-      for (int q = 0; q < 16; q++) S.M(nc.tape_slot, q) = M[q];
+    S.T(i, 0) = g; S.T(i, 1) = bg; S.T(i, 2) = ce.ct; S.T(i, 3) = ce.st; S.T(i, 4) = cp; S.T(i, 5) = sp;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       if (!nc.k[c].leaf) continue;
@@ -274,19 +290,18 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
         o[0] = q0; o[1] = q1; o[2] = q2; o[3] = q3;
       }
     }
-    if (nc.push >= 0) {   // the deferred daughter's boost and kinematics
-#pragma unroll
-      for (int q = 0; q < 16; q++) S.st(depth, q) = M[q];
-      S.st(depth, 16) = msq[nc.push]; S.st(depth, 17) = mk[nc.push]; S.st(depth, 18) = nc.push == 0 ? pk : -pk;
+    if (nc.push >= 0) {   // the deferred daughter's kinematics; its boost is re-formed at the pop
+      const int j = nc.k[nc.push].node;
+      S.in(j, 0) = msq[nc.push];
+      S.in(j, 1) = nc.push == 0 ? pk : -pk;
       depth++;
     }
     if (nc.desc >= 0) {
       cur_msq = msq[nc.desc]; cur_m = mk[nc.desc]; cur_p = nc.desc == 0 ? pk : -pk;
-    } else if (depth > 0) {
+    } else if (depth > 0 && i + 1 < N) {   // the next branch in visit order is the deferred one
       depth--;
-#pragma unroll
-      for (int q = 0; q < 16; q++) M[q] = S.st(depth, q);
-      cur_msq = S.st(depth, 16); cur_m = S.st(depth, 17); cur_p = S.st(depth, 18);
+      parent_boost(prog, i + 1, S, M);
+      cur_msq = S.in(i + 1, 0); cur_p = S.in(i + 1, 1); cur_m = math::sqrt_phys(cur_msq);
     }
   }
   *f_out = F;
@@ -317,17 +332,11 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       g_msq[nc.push] += S.st(depth, 16); g_m[nc.push] += S.st(depth, 17);
       g_pk += nc.push == 0 ? S.st(depth, 18) : -S.st(depth, 18);
     }
-    // recompute the branch's local quantities
-    const double a = S.in(i, 0), m = S.in(i, 1), p0 = S.in(i, 2);
+    // the branch's local quantities: boost parameters from the tape, maps recomputed
+    const double a = S.in(i, 0), p0 = S.in(i, 1);
+    const double m = i == 0 ? cd.sqrts : math::sqrt_phys(a);
     double P[16];
-    if (nc.parent < 0) {
-#pragma unroll
-      for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
-    } else {
-      const int ps = prog[nc.parent].tape_slot;
-#pragma unroll
-      for (int q = 0; q < 16; q++) P[q] = S.M(ps, q);
-    }
+    parent_boost(prog, i, S, P);
     MapEval r[2];
     double msq[2], f[2], mk[2], xk[2];
 #pragma unroll
@@ -338,10 +347,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     const double d = a - msq[0] - msq[1];
     const double lda = d * d - 4 * msq[0] * msq[1];
     const double sl = math::sqrt_phys(lda), pk = sl / (2 * m);
-    const double bg = m > 0 ? p0 / m : 0.0, g = math::sqrt_phys(1 + bg * bg);
-    const double xphi = xr[nc.x_phi], xct = xr[nc.x_phi + 1];
-    double sp, cp;
-    math::sincos_turns(xphi, sp, cp);
+    const double g = S.T(i, 0), bg = S.T(i, 1), cp = S.T(i, 4), sp = S.T(i, 5);
+    const double xct = xr[nc.x_phi + 1];
     const CtEval ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
     double Ma[16];
 #pragma unroll

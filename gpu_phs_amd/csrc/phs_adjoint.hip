@@ -1,5 +1,5 @@
 // Adjoint kernel: per event, the forward pass with a tape, then the reverse sweep
-// (phs_adjoint.h).  One workgroup = one wavefront = one tile of one channel; the tape, the
+// (phs_adjoint.h).  One workgroup = one wavefront = 64 events of one tile; the tape, the
 // stash and the lane's (M, Gamma) accumulators live in LDS, lane-strided so every access is
 // conflict-free.  The parameter adjoints of a tile are reduced across the wavefront and
 // written as one partial per tile; a second kernel sums the partials of each channel in a
@@ -16,8 +16,8 @@ namespace {
 struct LdsStore {
   double* base;
   int stride, off_in, off_st;
-  __device__ double& M(int slot, int k) { return base[(slot * 16 + k) * stride]; }
-  __device__ double& in(int node, int k) { return base[(off_in + node * 3 + k) * stride]; }
+  __device__ double& T(int node, int k) { return base[(node * 6 + k) * stride]; }
+  __device__ double& in(int node, int k) { return base[(off_in + node * 2 + k) * stride]; }
   __device__ double& st(int level, int k) { return base[(off_st + level * 19 + k) * stride]; }
 };
 
@@ -89,6 +89,7 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
 // units t, t + 256, ... and the 256 partial sums are combined by a fixed tree, so the result
 // does not depend on scheduling.
 constexpr int kReduceThreads = 256;
+constexpr int kMaxParams = 2 * (kMaxOut - 1);
 __global__ __launch_bounds__(kReduceThreads) void param_reduce_kernel(
     const double* __restrict__ gpart, int part_stride, int n_sub, const int* __restrict__ chan_tiles,
     const int* __restrict__ chan_tile_off, const ChannelDesc* __restrict__ chans, int n_chans,
@@ -100,20 +101,29 @@ __global__ __launch_bounds__(kReduceThreads) void param_reduce_kernel(
   const int t0 = chan_tile_off[c];
   const int n_units = (chan_tile_off[c + 1] - t0) * n_sub;
   const int tid = threadIdx.x;
-  for (int v = 0; v < np; v++) {
-    double sum = 0;
-    for (int k = tid; k < n_units; k += kReduceThreads) {
-      const int tile = chan_tiles[t0 + k / n_sub];
-      sum += gpart[((int64_t)tile * n_sub + k % n_sub) * part_stride + v];
-    }
-    red[tid] = sum;
-    __syncthreads();
-    for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
-      if (tid < w) red[tid] += red[tid + w];
+  // every unit's partials are np consecutive doubles: read them together
+  double sum[kMaxParams];
+#pragma unroll
+  for (int v = 0; v < kMaxParams; v++) sum[v] = 0;
+  for (int k = tid; k < n_units; k += kReduceThreads) {
+    const int tile = chan_tiles[t0 + k / n_sub];
+    const double* src = gpart + ((int64_t)tile * n_sub + k % n_sub) * part_stride;
+#pragma unroll
+    for (int v = 0; v < kMaxParams; v++)
+      if (v < np) sum[v] += src[v];
+  }
+#pragma unroll
+  for (int v = 0; v < kMaxParams; v++) {
+    if (v < np) {   // uniform
+      red[tid] = sum[v];
+      __syncthreads();
+      for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+      }
+      if (tid == 0) gparam[2 * chans[c].param0 + v] = red[0];
       __syncthreads();
     }
-    if (tid == 0) gparam[2 * chans[c].param0 + v] = red[0];
-    __syncthreads();
   }
 }
 

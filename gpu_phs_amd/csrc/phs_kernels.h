@@ -35,9 +35,9 @@ size_t forward_lds_limit();
 // Forward pass over tiles [t_begin, t_end).
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream);
 
-// Per-lane LDS layout of the adjoint (in doubles, lane-strided): tape boosts [0, off_in),
-// branch inputs [off_in, off_st), stash [off_st, off_acc), (M, Gamma) accumulators
-// [off_acc, words).
+// Per-lane LDS layout of the adjoint (in doubles, lane-strided): boost parameters of every
+// branch [0, off_in), branch inputs [off_in, off_st), stash [off_st, off_acc), (M, Gamma)
+// accumulators [off_acc, words).
 struct AdjLayout {
   int off_in, off_st, off_acc, words;
 };
