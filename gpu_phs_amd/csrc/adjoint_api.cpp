@@ -20,7 +20,10 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   lay.off_in = 6 * max_nodes;
   lay.off_st = lay.off_in + 2 * max_nodes;
   lay.off_acc = lay.off_st + 19 * max_banks;
-  lay.words = lay.off_acc + 2 * max_nodes;
+  // programs of up to 4 final-state particles keep their accumulators in registers
+  int max_nx = 0;
+  for (const auto& c : pl.chans) max_nx = std::max(max_nx, (int)c.n_x);
+  lay.words = lay.off_acc + (max_nx <= 8 ? 0 : 2 * max_nodes);
   if ((size_t)lay.words * 64 * sizeof(double) > 65536)
     return set_error(GPU_PHS_E_TREE, "adjoint: trees too large for the LDS tape");
   std::vector<SegDesc> segs = pl.segs;
@@ -54,7 +57,8 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   const size_t b_segs = al(n_seg * sizeof(SegDesc));
   const size_t b_part = al((size_t)std::max(n_t, 1) * pl.adj_sub * pl.adj_part_stride * sizeof(double));
   const size_t b_list = al(std::max(n_t, 1) * sizeof(int)), b_off = al((n_ch + 1) * sizeof(int));
-  const size_t total = b_segs + b_part + b_list + b_off;
+  const size_t b_ranges = al((size_t)std::max(n_ch, 1) * kAdjRanges * pl.adj_part_stride * sizeof(double));
+  const size_t total = b_segs + b_part + b_list + b_off + b_ranges;
   if (total > pl.adj_bytes) {
     if (pl.adj_dev) (void)hipFree(pl.adj_dev);
     pl.adj_dev = nullptr;
@@ -75,6 +79,7 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   pl.adj_part = (double*)(base + b_segs);
   pl.adj_chan_tiles = (int*)(base + b_segs + b_part);
   pl.adj_chan_off = (int*)(base + b_segs + b_part + b_list);
+  pl.adj_ranges = (double*)(base + b_segs + b_part + b_list + b_off);
   pl.adj_valid = true;
   return GPU_PHS_OK;
 }
@@ -126,8 +131,8 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
   int rc = adjoint_prepare(*pl, adj, write_forward);
   if (rc) return rc;
   hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, loss_mode, pl->adj_part, pl->adj_part_stride, pl->adj_sub,
-                                pl->adj_chan_tiles, pl->adj_chan_off, (int)pl->chans.size(), gparam, 0,
-                                (hipStream_t)stream);
+                                pl->adj_ranges, pl->adj_chan_tiles, pl->adj_chan_off, (int)pl->chans.size(), gparam,
+                                0, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "adjoint launch");
   return ok_status();
 }

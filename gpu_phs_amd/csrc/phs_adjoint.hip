@@ -33,7 +33,8 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-constexpr int kAdjXReg = 8;
+constexpr int kAdjXReg = 8;             // n_x <= 8 <=> n_out <= 4
+constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_out = 4 tree
 
 // XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
 template <int XN>
@@ -45,7 +46,10 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
   LdsStore S{lds + lane, 64, lay.off_in, lay.off_st};
-  LdsAcc acc{lds + lay.off_acc * 64 + lane, 64};
+  // the (M, Gamma) accumulators: in registers for the small programs (XN > 0: n_out <= 4, at
+  // most kSmallParams), else in LDS after the tape
+  double accv[XN > 0 ? kSmallParams : 1];
+  LdsAcc acc_lds{lds + lay.off_acc * 64 + lane, 64};
   // unit u = sub-tile (u % n_sub) of tile (u / n_sub): events [64 sub, 64 sub + 64) of the tile
   for (int u = (int)blockIdx.x; u < n_units; u += (int)gridDim.x) {
     const int t = u / n_sub, sub = u - t * n_sub;
@@ -54,7 +58,12 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
     const ChannelDesc cd = chans[sd.channel];
     const NodeCmd* prog = nodes + cd.node0;
     const int np = 2 * cd.n_nodes;
-    for (int v = 0; v < np; v++) acc[v] = 0;
+    if constexpr (XN > 0) {
+#pragma unroll
+      for (int v = 0; v < kSmallParams; v++) accv[v] = 0;
+    } else {
+      for (int v = 0; v < np; v++) acc_lds[v] = 0;
+    }
     const int end = min(td.count, 64 * (sub + 1));
     for (int local = 64 * sub + lane; local < end; local += 64) {
       const int64_t e = td.e0 + local;
@@ -68,44 +77,57 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
         double xv[XN];
 #pragma unroll
         for (int k = 0; k < XN; k++) xv[k] = xg[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
-        ok = event_adjoint(cd, prog, xv, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc, S);
+        ok = event_adjoint(cd, prog, xv, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
       } else {
-        ok = event_adjoint(cd, prog, xg, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc, S);
+        ok = event_adjoint(cd, prog, xg, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
       }
       if (sd.factor) sd.factor[e] = F;
       if (sd.volume) sd.volume[e] = cd.volume;
       if (sd.ok) sd.ok[e] = ok ? 1 : 0;
       if (sd.loss) sd.loss[e] = L;
     }
-    for (int v = 0; v < np; v++) {
-      const double tot = wave_sum(acc[v]);
-      if (lane == 0) gpart[(int64_t)u * part_stride + v] = tot;
+    if constexpr (XN > 0) {
+#pragma unroll
+      for (int v = 0; v < kSmallParams; v++) {
+        if (v < np) {   // uniform
+          const double tot = wave_sum(accv[v]);
+          if (lane == 0) gpart[(int64_t)u * part_stride + v] = tot;
+        }
+      }
+    } else {
+      for (int v = 0; v < np; v++) {
+        const double tot = wave_sum(acc_lds[v]);
+        if (lane == 0) gpart[(int64_t)u * part_stride + v] = tot;
+      }
     }
   }
 }
 
 // gparam[2 * slot + w] = sum over the sub-tiles of the tiles of the slot's channel (every slot
-// is written, channels without events get 0).  One workgroup per channel: thread t adds the
-// units t, t + 256, ... and the 256 partial sums are combined by a fixed tree, so the result
-// does not depend on scheduling.
+// is written, channels without events get 0).  Two fixed-order stages, so the result does not
+// depend on scheduling: kSplit workgroups per channel each sum a contiguous range of the
+// channel's units (thread t adds units t, t + 256, ..., then a fixed tree), and one workgroup
+// per channel adds the kSplit range sums in order.
 constexpr int kReduceThreads = 256;
+constexpr int kSplit = kAdjRanges;
 constexpr int kMaxParams = 2 * (kMaxOut - 1);
-__global__ __launch_bounds__(kReduceThreads) void param_reduce_kernel(
+__global__ __launch_bounds__(kReduceThreads) void param_reduce_ranges(
     const double* __restrict__ gpart, int part_stride, int n_sub, const int* __restrict__ chan_tiles,
     const int* __restrict__ chan_tile_off, const ChannelDesc* __restrict__ chans, int n_chans,
-    double* __restrict__ gparam) {
+    double* __restrict__ granges) {
   __shared__ double red[kReduceThreads];
-  const int c = blockIdx.x;
+  const int c = blockIdx.x / kSplit, r = blockIdx.x % kSplit;
   if (c >= n_chans) return;
   const int np = 2 * chans[c].n_nodes;
   const int t0 = chan_tile_off[c];
   const int n_units = (chan_tile_off[c + 1] - t0) * n_sub;
+  const int chunk = (n_units + kSplit - 1) / kSplit;
+  const int u0 = r * chunk, u1 = min(n_units, u0 + chunk);
   const int tid = threadIdx.x;
-  // every unit's partials are np consecutive doubles: read them together
   double sum[kMaxParams];
 #pragma unroll
   for (int v = 0; v < kMaxParams; v++) sum[v] = 0;
-  for (int k = tid; k < n_units; k += kReduceThreads) {
+  for (int k = u0 + tid; k < u1; k += kReduceThreads) {
     const int tile = chan_tiles[t0 + k / n_sub];
     const double* src = gpart + ((int64_t)tile * n_sub + k % n_sub) * part_stride;
 #pragma unroll
@@ -121,17 +143,28 @@ __global__ __launch_bounds__(kReduceThreads) void param_reduce_kernel(
         if (tid < w) red[tid] += red[tid + w];
         __syncthreads();
       }
-      if (tid == 0) gparam[2 * chans[c].param0 + v] = red[0];
+      if (tid == 0) granges[(int64_t)blockIdx.x * part_stride + v] = red[0];
       __syncthreads();
     }
   }
 }
 
+__global__ void param_reduce_final(const double* __restrict__ granges, int part_stride,
+                                   const ChannelDesc* __restrict__ chans, int n_chans, double* __restrict__ gparam) {
+  const int c = blockIdx.x;
+  if (c >= n_chans) return;
+  const int v = threadIdx.x;
+  if (v >= 2 * chans[c].n_nodes) return;
+  double sum = 0;
+  for (int r = 0; r < kSplit; r++) sum += granges[((int64_t)c * kSplit + r) * part_stride + v];
+  gparam[2 * chans[c].param0 + v] = sum;
+}
+
 }  // namespace
 
 hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
-                          int n_sub, const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam,
-                          int grid, hipStream_t stream) {
+                          int n_sub, double* granges, const int* chan_tiles, const int* chan_tile_off, int n_chans,
+                          double* gparam, int grid, hipStream_t stream) {
   const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
   if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
   const int64_t units = (int64_t)dp.n_tiles * n_sub;
@@ -144,8 +177,14 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_m
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(param_reduce_kernel, dim3(n_chans), dim3(kReduceThreads), 0, stream, gpart, part_stride, n_sub,
-                     chan_tiles, chan_tile_off, dp.chans, n_chans, gparam);
+  if (n_chans > 0) {
+    hipLaunchKernelGGL(param_reduce_ranges, dim3(n_chans * kSplit), dim3(kReduceThreads), 0, stream, gpart,
+                       part_stride, n_sub, chan_tiles, chan_tile_off, dp.chans, n_chans, granges);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(param_reduce_final, dim3(n_chans), dim3(64), 0, stream, granges, part_stride, dp.chans, n_chans,
+                       gparam);
+  }
   return hipGetLastError();
 }
 

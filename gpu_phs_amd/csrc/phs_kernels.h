@@ -43,10 +43,11 @@ struct AdjLayout {
 };
 
 // Adjoint over all tiles, one wavefront per 64-event sub-tile (n_sub per tile), then the
-// per-channel reduction of the parameter partials into
-// gparam[2 * (chans[c].param0 + node) + {0: M, 1: Gamma}].
+// per-channel reduction of the parameter partials (through granges, kAdjRanges * part_stride
+// doubles per channel) into gparam[2 * (chans[c].param0 + node) + {0: M, 1: Gamma}].
+constexpr int kAdjRanges = 16;
 hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
-                          int n_sub, const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam,
-                          int grid, hipStream_t stream);
+                          int n_sub, double* granges, const int* chan_tiles, const int* chan_tile_off, int n_chans,
+                          double* gparam, int grid, hipStream_t stream);
 
 }  // namespace gphs
