@@ -133,7 +133,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="",
-                    help="fixed launch config block,ept,grid,order,waves_per_eu,lds_bank instead of autotuning")
+                    help="fixed launch config block,ept,grid,order,waves_per_eu,lds instead of autotuning")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -162,7 +162,7 @@ def main():
     fixed = None
     if args.config:
         v = [int(t) for t in args.config.split(",")]
-        fixed = dict(zip(("block", "ept", "grid", "order", "waves_per_eu", "lds_bank"), v))
+        fixed = dict(zip(("block", "ept", "grid", "order", "waves_per_eu", "lds"), v))
     log(f"workload {args.workload}: setup")
     main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W, fixed=fixed)
     log(f"workload {args.workload}: {main_res['batched_ev_s']:.4g} ev/s batched, "

@@ -186,21 +186,21 @@ class Plan:
             self.set_config(**config)
 
     def set_config(self, **config):
-        """Launch configuration: block, ept, grid, order, waves_per_eu, lds_bank; keys left out
+        """Launch configuration: block, ept, grid, order, waves_per_eu, lds; keys left out
         keep their current values (the library's defaults for a new plan)."""
-        unknown = set(config) - {"block", "ept", "grid", "order", "waves_per_eu", "lds_bank"}
+        unknown = set(config) - {"block", "ept", "grid", "order", "waves_per_eu", "lds"}
         if unknown:
             raise TypeError(f"unknown launch parameters {sorted(unknown)}")
         c = self.config()
         c.update(config)
         _check(lib().gpu_phs_plan_set_config(self.handle, c["block"], c["ept"], c["grid"], c["order"],
-                                             c["waves_per_eu"], c["lds_bank"]), "plan_set_config")
+                                             c["waves_per_eu"], c["lds"]), "plan_set_config")
 
     def config(self):
         v = [ctypes.c_int() for _ in range(6)]
         _check(lib().gpu_phs_plan_get_config(self.handle, *[ctypes.byref(q) for q in v]), "plan_get_config")
         return dict(block=v[0].value, ept=v[1].value, grid=v[2].value, order=v[3].value,
-                    waves_per_eu=v[4].value, lds_bank=v[5].value)
+                    waves_per_eu=v[4].value, lds=v[5].value)
 
     def info(self):
         v = [ctypes.c_int() for _ in range(3)]

@@ -178,9 +178,12 @@ int plan_build(PlanObj& pl) {
   pl.dp.n_tiles = (int)pl.tiles.size();
   pl.dp.n_banks = pl.n_banks;
   pl.dp.nx_max = pl.dp.nout_max = 0;
+  pl.dp.root_banks_only = 1;
   for (const auto& c : pl.chans) {
     pl.dp.nx_max = std::max(pl.dp.nx_max, (int)c.n_x);
     pl.dp.nout_max = std::max(pl.dp.nout_max, (int)c.n_out);
+    for (int i = 1; i < c.n_nodes; i++)
+      if (pl.nodes[c.node0 + i].push >= 0) pl.dp.root_banks_only = 0;
   }
   return ok_status();
 }
@@ -296,29 +299,29 @@ int gpu_phs_plan_destroy(int plan) {
   return ok_status();
 }
 
-int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds_bank) {
+int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds) {
   std::lock_guard<std::mutex> lk(api_mutex());
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
   if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1 ||
-      !(waves_per_eu == 0 || waves_per_eu == 3 || waves_per_eu == 4) || lds_bank < 0 || lds_bank > 1)
+      !(waves_per_eu == 0 || waves_per_eu == 3) || lds < 0 || lds > 3)
     return set_error(GPU_PHS_E_ARG,
-                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4}, "
-                     "lds_bank in {0,1}");
+                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3}, "
+                     "lds in {0,1,2,3}");
   LaunchConfig want = pl->lc;
-  want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds_bank = lds_bank;
+  want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds = lds;
   if (forward_lds_bytes(pl->dp, want) > forward_lds_limit())
     return set_error(GPU_PHS_E_ARG, "the staging tables of this block size (and LDS save banks) exceed the LDS a "
                                     "workgroup may use on this device; choose a smaller block");
   bool rebuild = block != pl->lc.block || ept != pl->lc.ept || order != pl->order;
-  if (grid != pl->lc.grid || waves_per_eu != pl->lc.waves_per_eu || lds_bank != pl->lc.lds_bank) drop_graph(*pl);
+  if (grid != pl->lc.grid || waves_per_eu != pl->lc.waves_per_eu || lds != pl->lc.lds) drop_graph(*pl);
   pl->lc.block = block; pl->lc.ept = ept; pl->order = order;
-  pl->lc.grid = grid; pl->lc.waves_per_eu = waves_per_eu; pl->lc.lds_bank = lds_bank;
+  pl->lc.grid = grid; pl->lc.waves_per_eu = waves_per_eu; pl->lc.lds = lds;
   if (rebuild) return plan_build(*pl);
   return ok_status();
 }
 
-int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu, int* lds_bank) {
+int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu, int* lds) {
   std::lock_guard<std::mutex> lk(api_mutex());
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
@@ -327,7 +330,7 @@ int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* orde
   if (grid) *grid = pl->lc.grid;
   if (order) *order = pl->order;
   if (waves_per_eu) *waves_per_eu = pl->lc.waves_per_eu;
-  if (lds_bank) *lds_bank = pl->lc.lds_bank;
+  if (lds) *lds = pl->lc.lds;
   return ok_status();
 }
 
@@ -432,12 +435,13 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
   };
   int rc = GPU_PHS_OK;
   const bool banks = pl->n_banks > 0;
-  // stage 1: register budget (occupancy target) and where deferred boosts wait
-  for (int wpe : {0, 3, 4})
-    for (int lds : {0, 1})
-      if (!rc && (lds == 0 || banks)) {
+  // stage 1: register budget (occupancy target), where deferred boosts wait (lds bit 0) and
+  // where the randoms are kept (lds bit 1)
+  for (int wpe : {0, 3})
+    for (int lds : {0, 1, 2, 3})
+      if (!rc && (!(lds & 1) || banks)) {
         LaunchConfig lc = best.lc;
-        lc.waves_per_eu = wpe; lc.lds_bank = lds; lc.grid = 0;
+        lc.waves_per_eu = wpe; lc.lds = lds; lc.grid = 0;
         rc = consider(lc, 0);
       }
   // stage 2: workgroup size x events per thread
@@ -470,11 +474,11 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
     std::string s;
     char buf[96];
     snprintf(buf, sizeof buf, "best block=%d ept=%d grid=%d order=%d wpe=%d lds=%d %.4f ms;", best.lc.block,
-             best.lc.ept, best.lc.grid, best.order, best.lc.waves_per_eu, best.lc.lds_bank, best.ms);
+             best.lc.ept, best.lc.grid, best.order, best.lc.waves_per_eu, best.lc.lds, best.ms);
     s += buf;
     for (const auto& c : tried) {
       snprintf(buf, sizeof buf, " %d/%d/%d/%d/%d/%d:%.4f", c.lc.block, c.lc.ept, c.lc.grid, c.order,
-               c.lc.waves_per_eu, c.lc.lds_bank, c.ms);
+               c.lc.waves_per_eu, c.lc.lds, c.ms);
       s += buf;
     }
     snprintf(report, report_len, "%s", s.c_str());

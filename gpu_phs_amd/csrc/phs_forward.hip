@@ -30,46 +30,60 @@ namespace gphs {
 
 namespace {
 
-template <int NBANK>
-struct Banks {
-  double L[NBANK][16];
-  double msq[NBANK], m[NBANK], p[NBANK];
+// Save banks: where the walk keeps what it needs to resume a deferred subtree -- the boost M
+// after the branch that deferred it and the subtree's incoming msq, m and p_decay.
+//   BK_NONE  no branch defers a subtree (every branch has at most one internal daughter);
+//   BK_ROOT  only the root defers: its boost is L1 itself, kept as its six parameters and
+//            re-formed at the pop (9 doubles of registers instead of 19);
+//   BK_VGPR  one level anywhere in the tree, in registers;
+//   BK_LDS   any number of levels in LDS.
+enum BankKind { BK_NONE = 0, BK_ROOT = 1, BK_VGPR = 2, BK_LDS = 3 };
 
-  __device__ __forceinline__ void push(int d, const double (&M)[16], double m2, double mm, double pp) {
-#pragma unroll
-    for (int b = 0; b < NBANK; b++)
-      if (b == d) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) L[b][i] = M[i];
-        msq[b] = m2; m[b] = mm; p[b] = pp;
-      }
+struct L1Par { double g, bg, ct, st, cp, sp; };
+
+template <int KIND>
+struct Bank {   // BK_NONE
+  __device__ __forceinline__ void push(int, const double (&)[16], const L1Par&, double, double, double) {}
+  __device__ __forceinline__ void pop(int, double (&)[16], double&, double&, double&) const {}
+};
+
+template <>
+struct Bank<BK_ROOT> {
+  L1Par l;
+  double msq, m, p;
+  __device__ __forceinline__ void push(int, const double (&)[16], const L1Par& par, double m2, double mm, double pp) {
+    l = par; msq = m2; m = mm; p = pp;
   }
-  __device__ __forceinline__ void pop(int d, double (&M)[16], double& m2, double& mm, double& pp) const {
-#pragma unroll
-    for (int b = 0; b < NBANK; b++)
-      if (b == d) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) M[i] = L[b][i];
-        m2 = msq[b]; mm = m[b]; pp = p[b];
-      }
+  __device__ __forceinline__ void pop(int, double (&M)[16], double& m2, double& mm, double& pp) const {
+    set_l1(M, l.g, l.bg, l.ct, l.st, l.cp, l.sp);
+    m2 = msq; mm = m; pp = p;
   }
 };
 
 template <>
-struct Banks<0> {
-  __device__ __forceinline__ void push(int, const double (&)[16], double, double, double) {}
-  __device__ __forceinline__ void pop(int, double (&)[16], double&, double&, double&) const {}
+struct Bank<BK_VGPR> {
+  double L[16];
+  double msq, m, p;
+  __device__ __forceinline__ void push(int, const double (&M)[16], const L1Par&, double m2, double mm, double pp) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) L[i] = M[i];
+    msq = m2; m = mm; p = pp;
+  }
+  __device__ __forceinline__ void pop(int, double (&M)[16], double& m2, double& mm, double& pp) const {
+#pragma unroll
+    for (int i = 0; i < 16; i++) M[i] = L[i];
+    m2 = msq; mm = m; pp = p;
+  }
 };
 
-// The same save bank in LDS: level d of lane t holds kBankWords doubles at
-// base[(kBankWords d + i) * block + t] (consecutive lanes hit consecutive 8-byte words, so
-// every access is conflict-free).  Frees 38 VGPRs per level for occupancy at the price of 19
-// LDS stores and loads per deferral.
+// In LDS: level d of lane t holds kBankWords doubles at base[(kBankWords d + i) * block + t]
+// (consecutive lanes hit consecutive 8-byte words, so every access is conflict-free).
 constexpr int kBankWords = 19;
-struct LdsBanks {
+template <>
+struct Bank<BK_LDS> {
   double* base;
   int stride;
-  __device__ __forceinline__ void push(int d, const double (&M)[16], double m2, double mm, double pp) {
+  __device__ __forceinline__ void push(int d, const double (&M)[16], const L1Par&, double m2, double mm, double pp) {
     double* b = base + kBankWords * d * stride;
 #pragma unroll
     for (int i = 0; i < 16; i++) b[i * stride] = M[i];
@@ -87,45 +101,44 @@ struct LdsBanks {
   }
 };
 
+template <int KIND>
+__device__ __forceinline__ Bank<KIND> make_bank(double* bank_base) {
+  if constexpr (KIND == BK_LDS) return Bank<BK_LDS>{bank_base + threadIdx.x, (int)blockDim.x};
+  else return Bank<KIND>{};
+}
+
 extern __shared__ double lds[];
 
 // 16-byte words of global memory (two doubles)
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) dvec2 gdouble2;
 
-template <int NBANK, bool LDS>
-struct BankSel { using type = Banks<NBANK>; };
-template <int NBANK>
-struct BankSel<NBANK, true> { using type = LdsBanks; };
-
-template <int NBANK, bool LDS>
-__device__ __forceinline__ typename BankSel<NBANK, LDS>::type make_banks(double* bank_base) {
-  if constexpr (LDS) return LdsBanks{bank_base + threadIdx.x, (int)blockDim.x};
-  else return Banks<NBANK>{};
-}
-
-// Randoms of one event: with XREG the row is held in registers (n_x <= kXReg) and indexed by
-// the program's wave-uniform indices, else it is read from memory at each use.
-constexpr int kXReg = 16;
+// Randoms of one event: with XN > 0 the row is held in XN registers (n_x <= XN) and indexed
+// by the program's wave-uniform indices; with XN = kXLds it is copied into the lane's column
+// of an LDS table x[k][event] when the event starts.
+constexpr int kXLds = -1;
 
 // One event of channel `cd` (program `prog`).  xg: the event's randoms; pl: this lane's column
 // of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
-template <int NBANK, bool LDS, bool XREG>
+template <int BANK, int XN>
 __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
-                                              const double* __restrict__ xg, double* __restrict__ pl, int S,
-                                              double* bank_base, double& factor_out, bool& ok_out) {
+                                              const double* __restrict__ xg, double* __restrict__ pl,
+                                              double* __restrict__ xl, int S, double* bank_base,
+                                              double& factor_out, bool& ok_out) {
   const double s = cd.s;
   const gdouble* __restrict__ xrow = as_global(xg);
-  double xv[XREG ? kXReg : 1];
-  if constexpr (XREG) {
+  double xv[XN > 0 ? XN : 1];
+  if constexpr (XN > 0) {
 #pragma unroll
-    for (int k = 0; k < kXReg; k++) xv[k] = xrow[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
+    for (int k = 0; k < XN; k++) xv[k] = xrow[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
+  } else {
+    for (int k = 0; k < cd.n_x; k++) xl[k * S] = xrow[k];
   }
-#define GPHS_X(k) (XREG ? xv[XREG ? (k) : 0] : xrow[k])
+#define GPHS_X(k) (XN > 0 ? xv[XN > 0 ? (k) : 0] : xl[(k) * S])
 
   // boost of the parent branch; the root's parent is the identity (phs_cpu.cpp:568-573)
   double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  auto bank = make_banks<NBANK, LDS>(bank_base);
+  auto bank = make_bank<BANK>(bank_base);
   double cur_msq = s;    // root: msq = s (phs_cpu.cpp:195)
   double cur_p = 0.0;    // root: p_decay = 0 (never set by the reference)
   double factor = 1.0;
@@ -190,7 +203,7 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     const double msq_push = nc.push == 0 ? msq0 : msq1;
     const double m_push = nc.push == 0 ? m1 : m2;
     const double p_push = nc.push == 0 ? pk : -pk;
-    if (nc.push >= 0) { bank.push(depth, M, msq_push, m_push, p_push); depth++; }
+    if (nc.push >= 0) { bank.push(depth, M, L1Par{g, bg, ct, st, cp, sp}, msq_push, m_push, p_push); depth++; }
     if (nc.desc >= 0) {
       cur_msq = nc.desc == 0 ? msq0 : msq1;
       cur_m = nc.desc == 0 ? m1 : m2;
@@ -234,17 +247,19 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
 // WPE > 0 asks the register allocator for at least WPE waves per SIMD (fewer VGPRs, spills
 // allowed); the autotuner measures which occupancy target wins.
 //
-// LDS per workgroup (S = blockDim.x + 1 doubles per row): momenta 4 nout_max rows, then the
-// save banks when LDS is set.  A tile of `count` events is done in sub-tiles of blockDim.x
-// events.
-template <int NBANK, bool LDS, int WPE, bool XREG>
+// LDS per workgroup (S = blockDim.x + 1 doubles per row): momenta 4 nout_max rows, the
+// randoms nx_max rows (XN = kXLds), then the save banks (BANK = BK_LDS).  A tile of `count`
+// events is done in sub-tiles of blockDim.x events.
+template <int BANK, int WPE, int XN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
-                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nout_max) {
+                    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
+                    int nout_max) {
   const int B = (int)blockDim.x;
   const int S = B + 1;
   double* ps = lds;
-  double* bank_base = ps + 4 * nout_max * S;
+  double* xs = ps + 4 * nout_max * S;
+  double* bank_base = xs + (XN == kXLds ? nx_max * S : 0);
   const int lane = (int)threadIdx.x;
   for (int t = t_begin + (int)blockIdx.x; t < t_end; t += (int)gridDim.x) {
     const TileDesc td = tiles[t];
@@ -259,8 +274,8 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
       bool ok;
       __syncthreads();   // the previous sub-tile's momentum table has been read
       // lanes past the end repeat the last event and store nothing
-      forward_event<NBANK, LDS, XREG>(cd, prog, sd.x + (e0 + (valid ? lane : cnt - 1)) * cd.n_x, ps + lane, S,
-                                      bank_base, factor, ok);
+      forward_event<BANK, XN>(cd, prog, sd.x + (e0 + (valid ? lane : cnt - 1)) * cd.n_x, ps + lane, xs + lane, S,
+                              bank_base, factor, ok);
       if (valid) {
         as_global(sd.factor)[e0 + lane] = factor;
         as_global(sd.volume)[e0 + lane] = cd.volume;
@@ -275,39 +290,48 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
 }  // namespace
 
 namespace {
-using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int);
+using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*, int, int);
 
-template <bool LDS, int WPE, bool XREG>
-FwdKernel pick_banks(int n_banks) {
-  switch (n_banks) {
-    case 0: return forward_kernel<0, false, WPE, XREG>;
-    case 1: return forward_kernel<1, LDS, WPE, XREG>;
-    case 2: return forward_kernel<2, LDS, WPE, XREG>;
-    default: return forward_kernel<3, LDS, WPE, XREG>;
+// Where a plan's randoms live under `lc`: kXLds, or the register row size.
+int x_mode(const DevicePlan& dp, const LaunchConfig& lc) {
+  if ((lc.lds & 2) || dp.nx_max > 16) return kXLds;
+  return dp.nx_max <= 8 ? 8 : 16;
+}
+
+template <int BANK, int WPE>
+FwdKernel pick_xn(int xm) {
+  if (xm == 8) return forward_kernel<BANK, WPE, 8>;
+  if (xm == 16) return forward_kernel<BANK, WPE, 16>;
+  return forward_kernel<BANK, WPE, kXLds>;
+}
+
+template <int BANK>
+FwdKernel pick_wpe(const DevicePlan& dp, const LaunchConfig& lc) {
+  const int xm = x_mode(dp, lc);
+  return lc.waves_per_eu == 3 ? pick_xn<BANK, 3>(xm) : pick_xn<BANK, 0>(xm);
+}
+
+// The bank kind a plan runs with under `lc`.
+int bank_kind(const DevicePlan& dp, const LaunchConfig& lc) {
+  if (dp.n_banks == 0) return BK_NONE;
+  if ((lc.lds & 1) || dp.n_banks > 1) return BK_LDS;
+  return dp.root_banks_only ? BK_ROOT : BK_VGPR;
+}
+
+FwdKernel pick(const DevicePlan& dp, const LaunchConfig& lc) {
+  switch (bank_kind(dp, lc)) {
+    case BK_NONE: return pick_wpe<BK_NONE>(dp, lc);
+    case BK_ROOT: return pick_wpe<BK_ROOT>(dp, lc);
+    case BK_VGPR: return pick_wpe<BK_VGPR>(dp, lc);
+    default: return pick_wpe<BK_LDS>(dp, lc);
   }
 }
-
-template <bool LDS, bool XREG>
-FwdKernel pick_wpe(int n_banks, int wpe) {
-  switch (wpe) {
-    case 3: return pick_banks<LDS, 3, XREG>(n_banks);
-    case 4: return pick_banks<LDS, 4, XREG>(n_banks);
-    default: return pick_banks<LDS, 0, XREG>(n_banks);
-  }
-}
-
-template <bool XREG>
-FwdKernel pick(const DevicePlan& dp, const LaunchConfig& lc, bool lds_bank) {
-  return lds_bank ? pick_wpe<true, XREG>(dp.n_banks, lc.waves_per_eu) : pick_wpe<false, XREG>(dp.n_banks, lc.waves_per_eu);
-}
-
-bool use_lds_bank(const DevicePlan& dp, const LaunchConfig& lc) { return lc.lds_bank && dp.n_banks > 0; }
 }  // namespace
 
 size_t forward_lds_bytes(const DevicePlan& dp, const LaunchConfig& lc) {
-  const size_t rows = 4 * (size_t)dp.nout_max;
+  const size_t rows = 4 * (size_t)dp.nout_max + (x_mode(dp, lc) == kXLds ? (size_t)dp.nx_max : 0);
   size_t bytes = rows * (size_t)(lc.block + 1) * sizeof(double);
-  if (use_lds_bank(dp, lc)) bytes += (size_t)kBankWords * sizeof(double) * lc.block * dp.n_banks;
+  if (bank_kind(dp, lc) == BK_LDS) bytes += (size_t)kBankWords * sizeof(double) * lc.block * dp.n_banks;
   return bytes;
 }
 
@@ -330,10 +354,9 @@ hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const La
   if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
   const int n = t_end - t_begin;
   const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
-  const bool lds_bank = use_lds_bank(dp, lc);
-  FwdKernel k = dp.nx_max <= kXReg ? pick<true>(dp, lc, lds_bank) : pick<false>(dp, lc, lds_bank);
+  FwdKernel k = pick(dp, lc);
   hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,
-                     dp.nodes, dp.nout_max);
+                     dp.nodes, dp.nx_max, dp.nout_max);
   return hipGetLastError();
 }
 

@@ -16,14 +16,16 @@ struct DevicePlan {
   int n_banks = 0;       // save-bank depth the deepest program needs (0..3)
   int nx_max = 0;        // largest n_x of the plan's channels
   int nout_max = 0;      // largest n_out of the plan's channels
+  int root_banks_only = 0;  // 1: no branch but a root defers a subtree
 };
 
 struct LaunchConfig {
   int block = 256;       // threads per workgroup (multiple of 64, <= 256)
   int ept = 1;           // events per thread per tile
   int grid = 0;          // workgroups; 0 = one per tile, else a persistent grid that strides
-  int waves_per_eu = 0;  // register-allocation occupancy target: 0 (compiler's choice), 3, 4
-  int lds_bank = 0;      // 1: keep the save banks of deferred subtrees in LDS instead of VGPRs
+  int waves_per_eu = 0;  // register-allocation occupancy target: 0 (compiler's choice) or 3
+  int lds = 0;           // bit 0: save banks of deferred subtrees in LDS (always for 2+ levels),
+                         // bit 1: each event's randoms in LDS instead of registers
 };
 
 // LDS bytes per workgroup of the forward kernel under `lc`: the staged randoms and momenta of

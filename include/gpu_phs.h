@@ -134,13 +134,15 @@ int gpu_phs_plan_destroy(int plan);
 /* Launch shape: workgroup size (64..256, multiple of 64), events per thread per tile
  * (1..16), persistent grid (0 = one workgroup per tile), tile order (0 = segment order,
  * 1 = tiles of the deepest trees first), kernel variant: waves_per_eu = register budget for
- * at least that many waves per SIMD (0 = compiler's choice, 3, 4), lds_bank = 1 keeps
- * the boosts of deferred subtrees in LDS instead of VGPRs.  Changing block, ept or order
- * rebuilds the tile table.  A block whose LDS tables (4 n_out rows of block + 1 doubles, plus
- * the LDS save banks) exceed the device's per-workgroup LDS is rejected. */
-int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds_bank);
+ * at least that many waves per SIMD (0 = compiler's choice, 3), lds = bit flags: 1 keeps
+ * the boosts of deferred subtrees in LDS instead of VGPRs (trees that defer two levels always
+ * use LDS), 2 keeps each event's randoms in LDS instead of VGPRs (always for n_x > 16).  Changing block, ept or order
+ * rebuilds the tile table.  A block whose LDS tables (4 n_out rows of block + 1 doubles, n_x
+ * more with randoms in LDS, plus the LDS save banks) exceed the device's per-workgroup LDS is
+ * rejected. */
+int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds);
 int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu,
-                            int* lds_bank);
+                            int* lds);
 /* Forward pass of every segment in one kernel launch on `stream` (hipStream_t, 0 = null). */
 int gpu_phs_plan_run(int plan, void* stream);
 /* Forward pass with one launch per segment (the plain per-channel form). */
