@@ -45,7 +45,9 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   pl.adj_part_stride = 2 * max_nodes;
   pl.adj_segs = std::move(segs);
   // one wavefront per 64 events of a tile
-  pl.adj_sub = std::max(1, (pl.lc.block * pl.lc.ept + 63) / 64);
+  int tile_max = 1;
+  for (int c = 0; c < kTileClasses; c++) tile_max = std::max(tile_max, pl.lcs[c].block * pl.lcs[c].ept);
+  pl.adj_sub = (tile_max + 63) / 64;
   // tiles of every channel, in tile order
   std::vector<int> off(n_ch + 1, 0), list(n_t);
   for (const auto& t : pl.tiles) off[pl.segs[t.seg].channel + 1]++;

@@ -25,6 +25,8 @@ struct ProcessObj {
   CompiledProcess cp;
 };
 
+constexpr int kTileClasses = 2;
+
 struct PlanObj {
   std::vector<gpu_phs_segment> user;       // segments as given
   std::vector<ChannelDesc> chans;
@@ -33,7 +35,15 @@ struct PlanObj {
   std::vector<TileDesc> tiles;
   std::vector<int> seg_tile_begin, seg_tile_end;  // tile ranges per segment (segment order)
   int n_banks = 0;
-  LaunchConfig lc;
+  // Tile classes: channels of up to 8 randoms whose only deferring branch is the root (class 0)
+  // and the rest (class 1) run as separate launches, each with its own kernel variant and
+  // launch shape.  Class c's tiles are [class_begin[c], class_end[c]) of the tile table.
+  // defaults from measurement: the small programs with their randoms in LDS, 256 threads; the
+  // others with 64 threads and 4 events per thread
+  LaunchConfig lcs[kTileClasses] = {LaunchConfig{256, 1, 0, 0, 2}, LaunchConfig{64, 4, 0, 0, 0}};
+  DevicePlan dps[kTileClasses];
+  int class_begin[kTileClasses] = {0, 0}, class_end[kTileClasses] = {0, 0};
+  std::vector<int> chan_class;   // class of each plan channel
   int order = 0;
   void* dev = nullptr;                      // one allocation: tiles | segs | chans | nodes
   size_t dev_bytes = 0;
@@ -41,6 +51,9 @@ struct PlanObj {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipStream_t capture_stream = nullptr;
+  // the second tile class runs beside the first: a side stream joined by events
+  hipStream_t side_stream = nullptr;
+  hipEvent_t fork_event = nullptr, join_event = nullptr;
   // adjoint: a copy of the tables with the gradient buffers, per-tile partials, tiles by channel
   void* adj_dev = nullptr;
   size_t adj_bytes = 0;

@@ -273,7 +273,7 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
   }
   PlanObj pl;
   pl.user = segs;
-  pl.lc.block = w.block;
+  for (int c = 0; c < kTileClasses; c++) pl.lcs[c].block = w.block;
   if (plan_build(pl)) return;
   if (plan_run(pl, nullptr)) return;
   std::vector<double> f(n);
