@@ -124,6 +124,22 @@ def cpu_baseline(wl_name, W):
                        f"single thread, {total_t:.1f} s; reference phs_cpu.cpp built -O3")
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the forward kernel from the committed counter profile
+    (profiles/forward_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by
+    what the counter reads for a plain device copy) and the launch configuration it was taken
+    with, if it was taken on this workload; else (None, None)."""
+    path = os.path.join(ROOT, "profiles", "forward_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if t.get("workload") != workload:
+        return None, None
+    return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024, t.get("config")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +150,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="",
                     help="fixed launch config block,ept,grid,order,waves_per_eu,lds instead of autotuning")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the forward + adjoint workload (c4) and print its figures")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -158,6 +176,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n * 8, "fetch_expected_kib": n * 8 / 1024,
                           "write_expected_kib": n * 8 / 1024}))
+        return
+    if args.adjoint_only:
+        print(json.dumps(run_adjoint(max(5, args.steps), args.warmup, torch, G, W)))
         return
     fixed = None
     if args.config:
@@ -195,6 +216,7 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(args.workload, W)
     steps = args.steps
+    traffic, traffic_config = measured_traffic(args.workload)
     out = {
         "metric": "events/s @1 GPU (FP64, <=1e-12 rel vs phs_cpu), batched vs plain; fwd+adjoint ev/s",
         "value": main_res["batched_ev_s"],
@@ -214,7 +236,8 @@ def main():
         "plain_ev_s": main_res["plain_ev_s"],
         "batched_vs_plain": main_res["batched_vs_plain"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "traffic_config": traffic_config,
                      "kernel": "forward_kernel", "kernel_ms": main_res["batched_kernel_ms"],
                      "bytes_per_launch": main_res["algorithmic_bytes"]},
         "cpu_baseline": cpu,
