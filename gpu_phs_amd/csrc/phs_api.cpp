@@ -129,8 +129,10 @@ int plan_build(PlanObj& pl) {
   for (size_t c = 0; c < pl.chans.size(); c++) {
     const ChannelDesc& cd = pl.chans[c];
     bool small = cd.n_x <= 8;
-    for (int i = 1; i < cd.n_nodes; i++)
-      if (pl.nodes[cd.node0 + i].push >= 0) small = false;
+    for (int i = 0; i < cd.n_nodes; i++) {
+      if (i > 0 && pl.nodes[cd.node0 + i].push >= 0) small = false;
+      if (pl.nodes[cd.node0 + i].friend_sign != 0) small = false;   // friend trees run in class 1
+    }
     pl.chan_class[c] = small ? 0 : 1;
   }
   // tiles of each segment (segment order, for the plain launches), then the table in class
@@ -215,15 +217,19 @@ int plan_build(PlanObj& pl) {
   pl.dp.n_tiles = (int)pl.tiles.size();
   // maxima over all channels (adjoint) and over each class (forward launches)
   auto maxima = [&](DevicePlan& d, int cl) {
-    d.n_banks = 0; d.nx_max = d.nout_max = 0; d.root_banks_only = 1;
+    d.n_banks = 0; d.nx_max = d.nout_max = 0; d.root_banks_only = 1; d.has_friends = 0;
     for (size_t c = 0; c < pl.chans.size(); c++) {
       if (cl >= 0 && pl.chan_class[c] != cl) continue;
       const ChannelDesc& cd = pl.chans[c];
       d.n_banks = std::max(d.n_banks, (int)cd.n_banks);
       d.nx_max = std::max(d.nx_max, (int)cd.n_x);
       d.nout_max = std::max(d.nout_max, (int)cd.n_out);
-      for (int i = 1; i < cd.n_nodes; i++)
-        if (pl.nodes[cd.node0 + i].push >= 0) d.root_banks_only = 0;
+      for (int i = 0; i < cd.n_nodes; i++) {
+        const NodeCmd& nd = pl.nodes[cd.node0 + i];
+        if (i > 0 && nd.push >= 0) d.root_banks_only = 0;
+        // a save bank of the root's six boost parameters re-forms B_z R_z R_y, not a friend's matrix
+        if (nd.friend_sign != 0) { d.has_friends = 1; if (i == 0) d.root_banks_only = 0; }
+      }
     }
   };
   maxima(pl.dp, -1);

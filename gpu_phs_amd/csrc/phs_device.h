@@ -119,6 +119,49 @@ __device__ __forceinline__ void compose_cols03(double (&M)[16], double g, double
   }
 }
 
+// M <- M * L1 for a branch whose polar axis is an incoming beam, its "friend"
+// (phs.cu:853-910).  The beam k = (1, 0, 0, f) (f = +1 along +z, -1 along -z; the reference
+// scales k by the beam energy, which the normalisation below removes) is seen from the
+// branch's rest frame as n = spatial part of B_z(bg)^-1 M^-1 k, with M^-1 = eta M^T eta.
+// R is the rotation about e_z x n that takes e_z to n / |n| (Rodrigues' formula; for n along
+// -e_z, the half turn about e_x), and L1 = B_z(bg) (1 (+) R R_z(phi) R_y(theta)), so the
+// first daughter's polar angle theta is measured from the beam.  The reference's matrix for
+// R (L_rot_to_2nd, phs.cu:753-781) is not orthogonal and the bg term of its n_z has the
+// opposite sign; this is the rotation those lines describe.  Every product and sum is in
+// the order of the CPU restatement, which forms the same matrices.
+__device__ __forceinline__ void compose_friend(double (&M)[16], double g, double bg, double ct, double st, double cp,
+                                               double sp, double f) {
+  const double u0 = M[0] - f * M[12];
+  const double n0 = -M[1] + f * M[13];
+  const double n1 = -M[2] + f * M[14];
+  const double u3 = -M[3] + f * M[15];
+  const double n2 = -bg * u0 + g * u3;
+  const double nn = math::sqrt_phys(n0 * n0 + n1 * n1 + n2 * n2);
+  const double a0 = n0 / nn, a1 = n1 / nn, c = n2 / nn;
+  const double sa = math::sqrt_phys(a0 * a0 + a1 * a1);
+  const double kx = sa > 0 ? -a1 / sa : 1.0;
+  const double ky = sa > 0 ? a0 / sa : 0.0;
+  const double h = 1 - c;
+  const double r00 = c + h * kx * kx, r01 = h * kx * ky, r02 = a0;
+  const double r10 = r01, r11 = c + h * ky * ky, r12 = a1;
+  const double r20 = -a0, r21 = -a1, r22 = c;
+  // Q = R R_z(phi) R_y(theta)
+  const double rc0 = cp * ct, rc1 = sp * ct, rs0 = cp * st, rs1 = sp * st;
+  const double q00 = r00 * rc0 + r01 * rc1 + r02 * -st, q01 = r00 * -sp + r01 * cp, q02 = r00 * rs0 + r01 * rs1 + r02 * ct;
+  const double q10 = r10 * rc0 + r11 * rc1 + r12 * -st, q11 = r10 * -sp + r11 * cp, q12 = r10 * rs0 + r11 * rs1 + r12 * ct;
+  const double q20 = r20 * rc0 + r21 * rc1 + r22 * -st, q21 = r20 * -sp + r21 * cp, q22 = r20 * rs0 + r21 * rs1 + r22 * ct;
+  const double l01 = bg * q20, l02 = bg * q21, l03 = bg * q22;
+  const double l31 = g * q20, l32 = g * q21, l33 = g * q22;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double x0 = M[4 * i], x1 = M[4 * i + 1], x2 = M[4 * i + 2], x3 = M[4 * i + 3];
+    M[4 * i + 0] = x0 * g + x3 * bg;
+    M[4 * i + 1] = x0 * l01 + x1 * q00 + x2 * q10 + x3 * l31;
+    M[4 * i + 2] = x0 * l02 + x1 * q01 + x2 * q11 + x3 * l32;
+    M[4 * i + 3] = x0 * l03 + x1 * q02 + x2 * q12 + x3 * l33;
+  }
+}
+
 // L1 itself (the root's parent boost is the identity).
 __device__ __forceinline__ void set_l1(double (&M)[16], double g, double bg, double ct, double st, double cp, double sp) {
   M[0] = g;   M[1] = -bg * st; M[2] = 0;   M[3] = bg * ct;

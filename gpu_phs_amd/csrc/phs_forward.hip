@@ -120,7 +120,7 @@ constexpr int kXLds = -1;
 
 // One event of channel `cd` (program `prog`).  xg: the event's randoms; pl: this lane's column
 // of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
-template <int BANK, int XN>
+template <int BANK, int XN, bool FR>
 __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
                                               const double* __restrict__ xg, double* __restrict__ pl,
                                               double* __restrict__ xl, int S, double* bank_base,
@@ -173,8 +173,10 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     factor *= fct;
     // the root's parent boost is the identity (phs_cpu.cpp:568-573): M = L1; after a branch
     // with two final-state daughters the walk pops a saved matrix, so only the columns the
-    // daughters' momenta use are formed
-    if (i == 0) set_l1(M, g, bg, ct, st, cp, sp);
+    // daughters' momenta use are formed.  A branch with a friend takes its polar axis from
+    // an incoming beam (phs.cu:853-910).
+    if (FR && nc.friend_sign != 0) compose_friend(M, g, bg, ct, st, cp, sp, (double)nc.friend_sign);
+    else if (i == 0) set_l1(M, g, bg, ct, st, cp, sp);
     else if (nc.k[0].leaf && nc.k[1].leaf) compose_cols03(M, g, bg, ct, st, cp, sp);
     else compose(M, g, bg, ct, st, cp, sp);
 
@@ -250,7 +252,7 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
 // LDS per workgroup (S = blockDim.x + 1 doubles per row): momenta 4 nout_max rows, the
 // randoms nx_max rows (XN = kXLds), then the save banks (BANK = BK_LDS).  A tile of `count`
 // events is done in sub-tiles of blockDim.x events.
-template <int BANK, int WPE, int XN>
+template <int BANK, int WPE, int XN, bool FR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
                     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
@@ -274,7 +276,7 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
       bool ok;
       __syncthreads();   // the previous sub-tile's momentum table has been read
       // lanes past the end repeat the last event and store nothing
-      forward_event<BANK, XN>(cd, prog, sd.x + (e0 + (valid ? lane : cnt - 1)) * cd.n_x, ps + lane, xs + lane, S,
+      forward_event<BANK, XN, FR>(cd, prog, sd.x + (e0 + (valid ? lane : cnt - 1)) * cd.n_x, ps + lane, xs + lane, S,
                               bank_base, factor, ok);
       if (valid) {
         as_global(sd.factor)[e0 + lane] = factor;
@@ -298,17 +300,20 @@ int x_mode(const DevicePlan& dp, const LaunchConfig& lc) {
   return dp.nx_max <= 8 ? 8 : 16;
 }
 
-template <int BANK, int WPE>
+template <int BANK, int WPE, bool FR>
 FwdKernel pick_xn(int xm) {
-  if (xm == 8) return forward_kernel<BANK, WPE, 8>;
-  if (xm == 16) return forward_kernel<BANK, WPE, 16>;
-  return forward_kernel<BANK, WPE, kXLds>;
+  if (xm == 8) return forward_kernel<BANK, WPE, 8, FR>;
+  if (xm == 16) return forward_kernel<BANK, WPE, 16, FR>;
+  return forward_kernel<BANK, WPE, kXLds, FR>;
 }
 
+// Plans with friend branches run the variant that has the friend-axis composition (at the
+// compiler-chosen occupancy); the others never carry its code.
 template <int BANK>
 FwdKernel pick_wpe(const DevicePlan& dp, const LaunchConfig& lc) {
   const int xm = x_mode(dp, lc);
-  return lc.waves_per_eu == 3 ? pick_xn<BANK, 3>(xm) : pick_xn<BANK, 0>(xm);
+  if (dp.has_friends) return pick_xn<BANK, 0, true>(xm);
+  return lc.waves_per_eu == 3 ? pick_xn<BANK, 3, false>(xm) : pick_xn<BANK, 0, false>(xm);
 }
 
 // The bank kind a plan runs with under `lc`.

@@ -16,7 +16,8 @@ struct DevicePlan {
   int n_banks = 0;       // save-bank depth the deepest program needs (0..3)
   int nx_max = 0;        // largest n_x of the plan's channels
   int nout_max = 0;      // largest n_out of the plan's channels
-  int root_banks_only = 0;  // 1: no branch but a root defers a subtree
+  int root_banks_only = 0;  // 1: no branch but a root defers a subtree, and the root has no friend
+  int has_friends = 0;   // 1: some branch takes its polar axis from an incoming beam
 };
 
 struct LaunchConfig {

@@ -55,8 +55,10 @@ struct NodeCmd {
   int32_t parent;    // visit index of the parent branch, -1 for the root
   int32_t slot;      // which daughter (0/1) of the parent this branch is
   int32_t tape_slot; // adjoint tape slot of the boost after this branch, -1: no internal daughter
+  int32_t friend_sign; // polar axis: 0 the branch's z axis, +1 / -1 the incoming beam moving along +z / -z
+  int32_t pad0;
   double m_max;      // kinematic bound of this branch's mass (root: sqrt(s))
-  double b0, b1;     // cos-theta constants b[0], b[1], phs_cpu.cpp:514-527
+  double b0, b1;     // cos-theta constants b[0], b[1], phs_cpu.cpp:514-527 (friend: at s = 4 E1 E2)
   double m_min;      // sum of leaf masses below the branch
   double m;          // map mass of this branch (cos-theta constants per event)
   double db_dm[2];   // d b[k] / d M (adjoint)

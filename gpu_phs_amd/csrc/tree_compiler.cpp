@@ -64,11 +64,14 @@ inline double val(const DualMW& a) { return a.v; }
 
 // phs_cpu.cpp:452-531 (init_mapping_constants_cpu) for one branch, same expressions and
 // evaluation order; the values are those of the reference, bit for bit, and dm / dw carry
-// their derivatives with respect to the map mass and width.
-void map_constants_d(int id, double m_, double w_, double m_min, double m_tot, double sqrts, DualMW a[3], DualMW b[3]) {
+// their derivatives with respect to the map mass and width.  s_ct, when nonzero, is the s the
+// cos-theta constants are taken at (a branch with a friend: 4 E1 E2, phs.cu:879).
+void map_constants_d(int id, double m_, double w_, double m_min, double m_tot, double sqrts, DualMW a[3], DualMW b[3],
+                     double s_ct = 0) {
   const DualMW m(m_, 1, 0), w(w_, 0, 1);
   double m_max = sqrts - m_tot + m_min;
   double msq_min = m_min * m_min, msq_max = m_max * m_max, s = sqrts * sqrts;
+  const double sb = s_ct != 0 ? s_ct : s;
   DualMW msq0;
   a[0] = a[1] = a[2] = DualMW(0);
   switch (id) {
@@ -101,7 +104,7 @@ void map_constants_d(int id, double m_, double w_, double m_min, double m_tot, d
   }
   switch (id) {
     case MAP_TCHANNEL: case MAP_UCHANNEL: case MAP_COLLINEAR:
-      b[0] = m * m / s; b[1] = log((b[0] + 1) / b[0]); b[2] = 0; break;
+      b[0] = m * m / sb; b[1] = log((b[0] + 1) / b[0]); b[2] = 0; break;
     default:
       b[0] = b[1] = b[2] = DualMW(0); break;
   }
@@ -273,9 +276,14 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
         snprintf(msg, sizeof msg, "channel %d: branch %d has map id %d, which has no cos-theta mapping", c + 1, b + 1, mid(b));
         return msg;
       }
+      // friend sign (phs.cu:276-284): the incoming particle with bincode 2^n_out gives -1,
+      // the one with 2^(n_out + 1) gives +1, any other entry no friend
+      const int fr = sp.friends.empty() ? 0 : sp.friends[c * n_prt + b];
+      nd.friend_sign = fr == (1 << sp.n_out) ? -1 : fr == (1 << (sp.n_out + 1)) ? 1 : 0;
+      if (nd.friend_sign) out.has_friends = true;
       DualMW a[3], bb[3];
       double m = sp.map_masses[c * n_prt_out + b], w = sp.map_widths[c * n_prt_out + b];
-      map_constants_d(mid(b), m, w, ms[b], m_tot, sqrts, a, bb);
+      map_constants_d(mid(b), m, w, ms[b], m_tot, sqrts, a, bb, nd.friend_sign ? 4 * sp.E1 * sp.E2 : 0);
       nd.x_phi = xphi[b];
       nd.map_id = mid(b);
       nd.parent = parent;

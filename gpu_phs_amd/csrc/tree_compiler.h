@@ -34,6 +34,7 @@ struct CompiledProcess {
   std::vector<int> node_branch;        // branch index (bincode - 1) of every NodeCmd
   std::vector<double> mass_sum;        // [n_channels][n_prt_out]
   int max_banks = 0;
+  bool has_friends = false;            // some branch takes its polar axis from an incoming beam
 };
 
 // Validate and compile.  Returns an empty string on success, else the reason.

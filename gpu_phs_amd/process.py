@@ -133,15 +133,33 @@ def tree_channel(n_in: int, n_out: int, tree, maps=None):
     return d1, d2, hc, mid, mm, mw
 
 
-def make_process(n_in, n_out, trees, maps_list, flv_masses, sqrts, flv_widths=None, name=""):
+def tree_friends(n_in: int, n_out: int, tree, friends=None) -> np.ndarray:
+    """The friends row of one channel: ``friends`` maps a node (tuple, or "root") to the
+    incoming particle (1 or 2) whose beam gives that branch's polar axis; the row holds the
+    incoming particle's bincode (``2**n_out`` or ``2**(n_out + 1)``) at the branch's index,
+    0 elsewhere (whizard_gpu_phs.cpp:79-87, read by phs.cu:276-284)."""
+    row = np.zeros(2 ** (n_in + n_out) - 1, np.int32)
+    for node, which in (friends or {}).items():
+        if which not in (1, 2):
+            raise ValueError("a friend is incoming particle 1 or 2")
+        b = _bincode(tree if node == "root" else node)
+        row[b - 1] = 1 << (n_out + which - 1)
+    return row
+
+
+def make_process(n_in, n_out, trees, maps_list, flv_masses, sqrts, flv_widths=None, name="",
+                 friends_list=None):
     rows = [tree_channel(n_in, n_out, t, m) for t, m in zip(trees, maps_list)]
     cols = list(zip(*rows))
     flv_masses = np.asarray(flv_masses, np.float64)
     if flv_widths is None:
         flv_widths = np.zeros_like(flv_masses)
+    friends = None
+    if friends_list is not None:
+        friends = np.stack([tree_friends(n_in, n_out, t, f) for t, f in zip(trees, friends_list)])
     return Process(n_in, n_out, np.stack(cols[0]), np.stack(cols[1]), np.stack(cols[2]),
                    np.stack(cols[3]), np.stack(cols[4]), np.stack(cols[5]), flv_masses,
-                   np.asarray(flv_widths, np.float64), float(sqrts), None, name,
+                   np.asarray(flv_widths, np.float64), float(sqrts), friends, name,
                    [repr(t) for t in trees])
 
 

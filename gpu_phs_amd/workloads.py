@@ -96,6 +96,39 @@ def process_2to6(n_channels: int = 16) -> Process:
                         name="2->6: six 173 GeV finals")
 
 
+def process_friends(n_out: int = 4) -> Process:
+    """Trees whose branches take their polar axis from an incoming beam (friends), at
+    sqrt(s) = 1 TeV: friend signs -1 and +1, friends on the root and on inner branches, flat
+    and peaked cos-theta maps on friend branches."""
+    bw = (MAP_SCHANNEL, 80.4, 2.1)
+    t_ch = (MAP_TCHANNEL, 10.0, 0.0)
+    if n_out == 4:
+        trees = [((0, 1), (2, 3)), (0, (1, (2, 3))), ((0, 3), (1, 2)), ((0, 1), (2, 3))]
+        maps = [{(0, 1): bw, (2, 3): bw},
+                {"root": t_ch, (1, (2, 3)): t_ch, (2, 3): bw},
+                {(0, 3): bw, (1, 2): bw},
+                {"root": t_ch, (0, 1): t_ch, (2, 3): (MAP_COLLINEAR, 10.0, 0.0)}]
+        friends = [{"root": 1, (0, 1): 2},
+                   {(1, (2, 3)): 1, (2, 3): 2},
+                   {"root": 2},
+                   {(0, 1): 1, (2, 3): 1}]
+        return make_process(2, 4, trees, maps, [M_E, M_E, 0.0, 0.0, 5.0, 5.0], 1000.0,
+                            name="2->4 with friends", friends_list=friends)
+    if n_out == 6:
+        chain = (0, (1, (2, (3, (4, 5)))))
+        pairs = (((0, 1), (2, 3)), (4, 5))
+        trees = [chain, pairs]
+        maps = [{"root": t_ch, (1, (2, (3, (4, 5)))): t_ch, (2, (3, (4, 5))): (MAP_NO, 0.0, 0.0),
+                 (3, (4, 5)): t_ch, (4, 5): (MAP_SCHANNEL, 420.0, 12.0)},
+                {((0, 1), (2, 3)): (MAP_NO, 0.0, 0.0), (0, 1): (MAP_SCHANNEL, 420.0, 12.0),
+                 (2, 3): t_ch, (4, 5): (MAP_SCHANNEL, 420.0, 12.0)}]
+        friends = [{(1, (2, (3, (4, 5)))): 1, (2, (3, (4, 5))): 2, (3, (4, 5)): 1, (4, 5): 2},
+                   {"root": 1, (0, 1): 2, (2, 3): 1}]
+        return make_process(2, 6, trees, maps, [M_E, M_E] + [173.0] * 6, 3000.0,
+                            name="2->6 with friends", friends_list=friends)
+    raise ValueError("n_out must be 4 or 6")
+
+
 @dataclass
 class Segment:
     process: int      # index into Workload.processes

@@ -60,7 +60,11 @@ void c_whizard_fill_mapping(int* channel, int* map_ids, double* masses, double* 
 void c_whizard_init_tree_structures(void);
 /* whizard_gpu_phs.cpp:79-87: one channel's tree (1-based channel), each n_prt =
  * 2^(n_in+n_out) - 1 long: daughters as bincodes (0 = none), has_children flags, friends
- * (bincode of the incoming particle whose axis a branch's polar angle refers to, 0 = none). */
+ * (bincode of the incoming particle whose beam a branch's polar angle is measured from,
+ * read as phs.cu:276-284 reads it: 2^n_out -> the beam along -z, 2^(n_out+1) -> the beam
+ * along +z, any other value -> the branch's own z axis).  The rotation onto the beam is a
+ * proper rotation (Rodrigues), where the reference's L_rot_to_2nd (phs.cu:753-781) is not
+ * orthogonal; trees with friends have no adjoint. */
 void c_whizard_fill_tree_structure(int* channel, int* daughters1, int* daughters2, int* has_children,
                                    int* friends);
 /* whizard_gpu_phs.cpp:89-98: channel of every event of the next batch from cumulative
@@ -100,7 +104,8 @@ void c_whizard_show_module(void);
 /* A process: n_channels trees in the reference format (see c_whizard_fill_tree_structure
  * and c_whizard_fill_mapping; arrays are [n_channels][n_prt] and [n_channels][n_prt_out]),
  * flavour masses [n_in + n_out], beams E1, E2 (sqrt(s) = 2 sqrt(E1 E2)).  friends may be
- * NULL.  Returns a handle >= 0. */
+ * NULL (no branch has a friend; semantics as in c_whizard_fill_tree_structure).  Returns a
+ * handle >= 0. */
 int gpu_phs_process_create(int n_in, int n_out, int n_channels, const int* d1, const int* d2,
                            const int* has_children, const int* friends, const int* map_ids,
                            const double* map_masses, const double* map_widths, const double* flv_masses,

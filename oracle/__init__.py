@@ -43,6 +43,7 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIB)
         _lib.oracle_forward.restype = _I
+        _lib.oracle_forward_friends.restype = _I
         _lib.oracle_forward_dual.restype = _I
         _lib.oracle_constants.restype = _I
         _lib.oracle_mass_sums.restype = _I
@@ -78,7 +79,9 @@ def _tree_args(proc):
 def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
     """Oracle forward pass: returns dict(p, factor, volume, ok, scale, cond[, msq]); ``scale`` is
     the componentwise forward-error scale |L1|...|L1||v| of every momentum component, ``cond``
-    the event's condition number (lambda near a threshold, sin theta of peaked maps)."""
+    the event's condition number (lambda near a threshold, sin theta of peaked maps, the
+    friend axis).  Branches with a friend (``proc.friends``) take their polar axis from the
+    incoming beam."""
     x = np.ascontiguousarray(x, np.float64)
     ch = np.ascontiguousarray(channels, np.int32)
     n = ch.shape[0]
@@ -93,10 +96,14 @@ def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
     t = _tree_args(proc)
     e1 = None if E1 is None else np.ascontiguousarray(E1, np.float64)
     e2 = None if E2 is None else np.ascontiguousarray(E2, np.float64)
-    rc = lib().oracle_forward(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
-                              *[_ptr(a) for a in t], _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n),
-                              _ptr(ch), _I(proc.n_x), _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok),
-                              _ptr(msq), _ptr(scale), _ptr(cond))
+    fr = None
+    if proc.friends is not None and np.any(proc.friends):
+        fr = np.ascontiguousarray(proc.friends, np.int32)
+    rc = lib().oracle_forward_friends(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                                      *[_ptr(a) for a in t[:3]], _ptr(fr), *[_ptr(a) for a in t[3:]],
+                                      _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n), _ptr(ch),
+                                      _I(proc.n_x), _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok),
+                                      _ptr(msq), _ptr(scale), _ptr(cond))
     if rc:
         raise ValueError(f"oracle_forward failed ({rc})")
     out = dict(p=p, factor=f, volume=v, ok=ok.astype(bool), scale=scale, cond=cond)
@@ -106,7 +113,10 @@ def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
 
 
 def forward_dual(proc, x, channels, x_tan=None, m_tan=None, w_tan=None, K=1):
-    """Dual-number forward: tangents of p and factor along K seeded directions."""
+    """Dual-number forward: tangents of p and factor along K seeded directions (trees
+    without friends)."""
+    if proc.friends is not None and np.any(proc.friends):
+        raise ValueError("forward_dual: the friend axis has no dual-number path")
     x = np.ascontiguousarray(x, np.float64)
     ch = np.ascontiguousarray(channels, np.int32)
     n = ch.shape[0]

@@ -22,7 +22,11 @@
 //   Parts with no reference counterpart and therefore pinned by invariants only:
 //   the per-event sqrt(s) (reduces to the fixed-sqrt(s) path when E1 = E2), the
 //   lab-frame boost (phs.cu:1286-1301; identity for E1 = E2, Lorentz invariants and
-//   total momentum otherwise).
+//   total momentum otherwise), and the friend axis (phs.cu:853-910; GPU-only in the
+//   reference, whose rotation matrix there is not orthogonal): parity unpinned, checked
+//   by momentum conservation, on-shell masses, the polar angle of the first daughter
+//   measured from the beam in the branch's rest frame, and the f = +1 root reducing to
+//   the plain composition.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -93,6 +97,7 @@ struct Proc {
   std::vector<int> map_id;          // [c * n_prt_out + i]
   std::vector<double> mass_sum;     // [c * n_prt_out + i]
   std::vector<double> flv;          // [n_in + n_out]
+  std::vector<int> fsign;           // [c * n_prt + i] friend sign -1 / 0 / +1, empty: no friends
 };
 
 template <class T> struct MapPar {
@@ -272,6 +277,7 @@ template <class T> struct Ev {
   std::vector<T> mom;       // 4 per branch index (prt[].p)
   std::vector<double> scale;  // 4 per branch index: forward-error scale of mom
   double cond = 1;            // condition number of the event (see set_msq, set_angles)
+  double s_friend = 0;        // s of the cos-theta maps of branches with a friend: 4 E1 E2
 };
 
 // phs_cpu.cpp:174-234 (set_msq_cpu): post-order recursion, msq randoms consumed in
@@ -331,6 +337,53 @@ void set_msq(const Proc& P, const MapPar<T>& M, int c, int b, double m_tot, cons
   }
 }
 
+// The boost of a branch whose polar axis is an incoming beam, its "friend"
+// (phs.cu:853-910, GPU-only in the reference).  The beam k = (1, 0, 0, f) (the reference
+// scales it by the beam energy, phs.cu:885-886, which the normalisation removes) seen from
+// the branch's rest frame is n = spatial part of B_z(bg)^-1 L0^-1 k, L0^-1 = eta L0^T eta
+// (phs.cu:888-890, whose n_z has the opposite sign on the bg L0[3][0] term).  R is the
+// rotation about e_z x n that takes e_z to n / |n| (Rodrigues' formula; for n along -e_z the
+// half turn about e_x), where the reference's L_rot_to_2nd (phs.cu:753-781) builds a matrix
+// that is not orthogonal.  L1 = B_z(bg) (1 (+) R R_z(phi) R_y(theta)), phs.cu:896-906.
+// cond grows by the conditioning of the axis: the terms n is summed from over |n|, times the
+// sensitivity (1 - n_z) / |n_xy| of the rotation axis near n = -e_z.
+template <class T>
+void friend_l1(int fs, const T L0[4][4], const T& bg, const T& gamma, const T& ct, const T& st, const T& cp,
+               const T& sp, T L1[4][4], double* cond) {
+  const double f = fs;
+  T u0 = L0[0][0] - f * L0[3][0];
+  T n0 = -L0[0][1] + f * L0[3][1];
+  T n1 = -L0[0][2] + f * L0[3][2];
+  T u3 = -L0[0][3] + f * L0[3][3];
+  T n2 = -bg * u0 + gamma * u3;
+  T nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  T a0 = n0 / nn, a1 = n1 / nn, cc = n2 / nn;
+  T sa = sqrt(a0 * a0 + a1 * a1);
+  T kx = val(sa) > 0 ? T(-a1 / sa) : T(1.0);
+  T ky = val(sa) > 0 ? T(a0 / sa) : T(0.0);
+  T h = 1 - cc;
+  T R[3][3] = {{cc + h * kx * kx, h * kx * ky, a0}, {h * kx * ky, cc + h * ky * ky, a1}, {-a0, -a1, cc}};
+  T Rot[3][3] = {{cp * ct, -sp, cp * st}, {sp * ct, cp, sp * st}, {-st, T(0.0), ct}};
+  T Q[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      // the kernel skips the structural zero Rot[2][1]; adding 0 changes no value
+      Q[i][j] = R[i][0] * Rot[0][j] + R[i][1] * Rot[1][j];
+      if (j != 1) Q[i][j] = Q[i][j] + R[i][2] * Rot[2][j];
+    }
+  L1[0][0] = gamma; L1[0][1] = bg * Q[2][0];    L1[0][2] = bg * Q[2][1];    L1[0][3] = bg * Q[2][2];
+  L1[1][0] = 0;     L1[1][1] = Q[0][0];         L1[1][2] = Q[0][1];         L1[1][3] = Q[0][2];
+  L1[2][0] = 0;     L1[2][1] = Q[1][0];         L1[2][2] = Q[1][1];         L1[2][3] = Q[1][2];
+  L1[3][0] = bg;    L1[3][1] = gamma * Q[2][0]; L1[3][2] = gamma * Q[2][1]; L1[3][3] = gamma * Q[2][2];
+  const double ab = std::fabs(val(bg)), g = std::fabs(val(gamma));
+  double t = 0;
+  for (int j = 1; j < 3; j++) t += std::fabs(val(L0[0][j])) + std::fabs(val(L0[3][j]));
+  t += ab * (std::fabs(val(L0[0][0])) + std::fabs(val(L0[3][0]))) + g * (std::fabs(val(L0[0][3])) + std::fabs(val(L0[3][3])));
+  double k = t / std::max(val(nn), 1e-300);
+  if (val(sa) > 0) k *= std::max(1.0, val(h) / val(sa));
+  *cond = std::max(*cond, k);
+}
+
 // phs_cpu.cpp:283-345 (set_angles_cpu): pre-order recursion; phi then cos-theta randoms
 // per internal branch; L_new = L0 * B_z(bg) R_z(phi) R_y(theta).
 //
@@ -358,7 +411,18 @@ void set_angles(const Proc& P, const MapPar<T>& M, int c, int b, const T* x, int
     (*idx_x)++;
     int ib = c * P.n_prt_out + b;
     T ct, st, f;
-    map_ct(P.map_id[ib], xx, &M.b[ib * 3], &ct, &st, &f);
+    const int fs = P.fsign.empty() ? 0 : P.fsign[c * P.n_prt + b];
+    if (fs != 0) {
+      // a branch with a friend evaluates its cos-theta map at s = 4 E1 E2 (phs.cu:879;
+      // b[] as mappings_gpu.hpp:186-188)
+      T bf[3];
+      bf[0] = M.m[ib] * M.m[ib] / E.s_friend;
+      bf[1] = log((bf[0] + 1) / bf[0]);
+      bf[2] = 0;
+      map_ct(P.map_id[ib], xx, bf, &ct, &st, &f);
+    } else {
+      map_ct(P.map_id[ib], xx, &M.b[ib * 3], &ct, &st, &f);
+    }
     *factor *= f;
     // sin theta = sqrt(1 - ct^2) of the peaked map loses accuracy as 1 / sin theta, and
     // ct = tmp - b0 - 1 cancels near ct = 0
@@ -368,10 +432,14 @@ void set_angles(const Proc& P, const MapPar<T>& M, int c, int b, const T* x, int
       if (c > 0) E.cond = std::max(E.cond, (1 + std::fabs(val(M.b[ib * 3])) + 1) / c);
     }
     T L1[4][4];
-    L1[0][0] = gamma; L1[0][1] = -bg * st;    L1[0][2] = 0;  L1[0][3] = bg * ct;
-    L1[1][0] = 0;     L1[1][1] = ct * cp;     L1[1][2] = -sp; L1[1][3] = st * cp;
-    L1[2][0] = 0;     L1[2][1] = ct * sp;     L1[2][2] = cp; L1[2][3] = st * sp;
-    L1[3][0] = bg;    L1[3][1] = -gamma * st; L1[3][2] = 0;  L1[3][3] = gamma * ct;
+    if (fs == 0) {
+      L1[0][0] = gamma; L1[0][1] = -bg * st;    L1[0][2] = 0;  L1[0][3] = bg * ct;
+      L1[1][0] = 0;     L1[1][1] = ct * cp;     L1[1][2] = -sp; L1[1][3] = st * cp;
+      L1[2][0] = 0;     L1[2][1] = ct * sp;     L1[2][2] = cp; L1[2][3] = st * sp;
+      L1[3][0] = bg;    L1[3][1] = -gamma * st; L1[3][2] = 0;  L1[3][3] = gamma * ct;
+    } else {
+      friend_l1(fs, L0, bg, gamma, ct, st, cp, sp, L1, &E.cond);
+    }
     T Ln[4][4];
     double An[4][4];
     for (int i = 0; i < 4; i++)
@@ -410,14 +478,26 @@ bool build_proc(Proc& P, int n_in, int n_out, int n_channels, const int* d1, con
   return true;
 }
 
+// Friend signs from the caller's friends array [n_channels][n_prt] of incoming bincodes
+// (whizard_gpu_phs.cpp:79-87), as extract_boost_origins reads it (phs.cu:276-284):
+// 2^n_out gives -1, 2^(n_out + 1) gives +1, anything else no friend.
+void set_friends(Proc& P, const int* friends) {
+  P.fsign.assign((size_t)P.n_channels * P.n_prt, 0);
+  if (!friends) return;
+  const int in1 = 1 << P.n_out;
+  for (size_t i = 0; i < P.fsign.size(); i++)
+    P.fsign[i] = friends[i] == in1 ? -1 : friends[i] == 2 * in1 ? 1 : 0;
+}
+
 // One event, phs_cpu.cpp:582-600 (reset prt, init_msq_cpu, set_msq_cpu, set_angles_cpu),
 // then the lab-frame boost of the final leaves, phs.cu:1286-1301, when E1 != E2.
 template <class T>
-bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg_lab, const T* x,
+bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg_lab, double s_friend, const T* x,
                Ev<T>& E, T* factor, T* volume, T* p_out, double* msq_out, double* scale_out, double* cond_out) {
   int n = P.n_prt;
   E.p2.assign(n, T(0)); E.pd.assign(n, T(0)); E.mom.assign(4 * n, T(0)); E.scale.assign(4 * n, 0.0);
   E.cond = 1;
+  E.s_friend = s_friend;
   // phs_cpu.cpp:541-547 (init_msq_cpu): leaf msq = flavour mass squared.
   for (int i = 0, p = 1; i < P.n_out; i++, p *= 2) E.p2[p - 1] = P.flv[P.n_in + i] * P.flv[P.n_in + i];
   bool ok = true;
@@ -508,13 +588,32 @@ void oracle_map_ct(int type, double x, const double* b, double* ct, double* st, 
 //   scale of every momentum component (may be NULL; see set_angles), cond [n_events] the
 //   event's condition number (may be NULL; see set_msq).  Non-ok events keep p = 0 and whatever factor
 //   and volume the early return left, as the reference does.  Returns 0 on success.
+int oracle_forward_friends(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                           const int* friends, const int* map_ids, const double* map_m, const double* map_w,
+                           const double* flv, double sqrts, const double* E1, const double* E2, long n_events,
+                           const int* channels, int n_x, const double* x, double* p, double* factors,
+                           double* volumes, unsigned char* oks, double* msq, double* scale, double* cond);
+
 int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
                    const int* map_ids, const double* map_m, const double* map_w, const double* flv,
                    double sqrts, const double* E1, const double* E2, long n_events, const int* channels,
                    int n_x, const double* x, double* p, double* factors, double* volumes,
                    unsigned char* oks, double* msq, double* scale, double* cond) {
+  return oracle_forward_friends(n_in, n_out, n_channels, d1, d2, hc, nullptr, map_ids, map_m, map_w, flv, sqrts,
+                                E1, E2, n_events, channels, n_x, x, p, factors, volumes, oks, msq, scale, cond);
+}
+
+// oracle_forward with the friends array [n_channels][n_prt] (incoming bincodes, 0 = none,
+// whizard_gpu_phs.cpp:79-87); NULL = no friends.  A branch with a friend evaluates its
+// cos-theta map at s = 4 E1 E2 (sqrts^2 when E1 / E2 are NULL).
+int oracle_forward_friends(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                           const int* friends, const int* map_ids, const double* map_m, const double* map_w,
+                           const double* flv, double sqrts, const double* E1, const double* E2, long n_events,
+                           const int* channels, int n_x, const double* x, double* p, double* factors,
+                           double* volumes, unsigned char* oks, double* msq, double* scale, double* cond) {
   Proc P;
   if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  if (friends) set_friends(P, friends);
   if (n_x < 3 * n_out - 4) return 2;
   MapPar<double> M;
   size_t nb = (size_t)n_channels * P.n_prt_out;
@@ -526,8 +625,9 @@ int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int
   for (long i = 0; i < n_events; i++) {
     int c = channels[i];
     if (c < 0 || c >= n_channels) return 3;
-    double s_ev = sqrts, bg_lab = 0;
+    double s_ev = sqrts, bg_lab = 0, s_friend = sqrts * sqrts;
     if (E1 && E2) {
+      s_friend = 4 * E1[i] * E2[i];
       // phs.cu:1313-1317 (per-event sqrt(s)) and 1289 (lab boost parameter).
       s_ev = 2 * std::sqrt(E1[i] * E2[i]);
       bg_lab = (E1[i] - E2[i]) / (2 * std::sqrt(E1[i] * E2[i]));
@@ -539,7 +639,7 @@ int oracle_forward(int n_in, int n_out, int n_channels, const int* d1, const int
     double f = 0, v = 0;
     std::memset(&p[4 * n_out * i], 0, 4 * n_out * sizeof(double));
     if (scale) std::memset(&scale[4 * n_out * i], 0, 4 * n_out * sizeof(double));
-    bool ok = one_event<double>(P, M, c, s_ev, bg_lab, &x[(size_t)n_x * i], E, &f, &v, &p[4 * n_out * i],
+    bool ok = one_event<double>(P, M, c, s_ev, bg_lab, s_friend, &x[(size_t)n_x * i], E, &f, &v, &p[4 * n_out * i],
                                 msq ? &msq[(size_t)P.n_prt_out * i] : nullptr,
                                 scale ? &scale[4 * n_out * i] : nullptr, cond ? &cond[i] : nullptr);
     factors[i] = f; volumes[i] = v; oks[i] = ok ? 1 : 0;
@@ -582,7 +682,7 @@ int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, cons
     }
     for (auto& q : pd) q = Dual(0.0);
     Dual f, v;
-    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr, nullptr);
+    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, sqrts * sqrts, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr, nullptr);
     oks[i] = ok ? 1 : 0;
     factors[i] = ok ? f.v : 0;
     for (int j = 0; j < 4 * n_out; j++) {

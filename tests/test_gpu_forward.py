@@ -29,11 +29,11 @@ def load_golden(path):
     return proc, z["x"], z["channels"], ref
 
 
-def run_gpu(proc, x, channels, **cfg):
+def run_gpu(proc, x, channels, E1=None, E2=None, **cfg):
     """Events grouped by channel into segments (one plan), results in the caller's order."""
     torch = _torch()
     import gpu_phs_amd as G
-    ph = G.ProcessHandle(proc)
+    ph = G.ProcessHandle(proc, E1=E1, E2=E2)
     order = np.argsort(channels, kind="stable")
     xs = torch.from_numpy(np.ascontiguousarray(x[order])).cuda()
     chs = channels[order]

@@ -49,9 +49,10 @@ def test_whizard_symbols_match_reference_names():
     assert sorted(expected) == sorted(n for n in declared_functions() if n.startswith("c_whizard_"))
 
 
-@pytest.mark.parametrize("name", ["c0", "c1", "c2", "2to6"])
+@pytest.mark.parametrize("name", ["c0", "c1", "c2", "2to6", "friends4", "friends6"])
 def test_process_create(name):
-    proc = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6}[name]()
+    proc = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6,
+            "friends4": lambda: W.process_friends(4), "friends6": lambda: W.process_friends(6)}[name]()
     ph = G.ProcessHandle(proc)
     n_out, n_x, n_ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     assert G.lib().gpu_phs_process_info(ph.handle, ctypes.byref(n_out), ctypes.byref(n_x), ctypes.byref(n_ch)) == 0
