@@ -112,6 +112,29 @@ def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
     return out
 
 
+def inverse(proc, p, E1=None, E2=None):
+    """Inverse mapping of events with momenta p [n][n_out][4] into every channel: returns
+    (factors [n][n_channels], x_out [n_channels][n][n_x])."""
+    p = np.ascontiguousarray(p, np.float64)
+    n = p.shape[0]
+    assert p.shape == (n, proc.n_out, 4)
+    f = np.zeros((n, proc.n_channels))
+    xo = np.zeros((proc.n_channels, n, proc.n_x))
+    t = _tree_args(proc)
+    fr = None
+    if proc.friends is not None and np.any(proc.friends):
+        fr = np.ascontiguousarray(proc.friends, np.int32)
+    e1 = None if E1 is None else np.ascontiguousarray(E1, np.float64)
+    e2 = None if E2 is None else np.ascontiguousarray(E2, np.float64)
+    lib().oracle_inverse.restype = _I
+    rc = lib().oracle_inverse(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels), *[_ptr(a) for a in t[:3]],
+                              _ptr(fr), *[_ptr(a) for a in t[3:]], _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n),
+                              _I(proc.n_x), _ptr(p), _ptr(f), _ptr(xo))
+    if rc:
+        raise ValueError(f"oracle_inverse failed ({rc})")
+    return f, xo
+
+
 def forward_dual(proc, x, channels, x_tan=None, m_tan=None, w_tan=None, K=1):
     """Dual-number forward: tangents of p and factor along K seeded directions (trees
     without friends)."""

@@ -535,6 +535,214 @@ bool one_event(const Proc& P, const MapPar<T>& M, int c, double sqrts, double bg
   return ok;
 }
 
+// ---------------------------------------------------------------------------
+// Inverse mapping: the randoms and the phase-space factor under which channel c would have
+// produced a given event (phs.cu:626-675, 924-1284 on the GPU, driven per channel by
+// phs.cu:1499-1554; CPU counterpart phs_cpu.cpp:63-110, 148-172, 236-281, 347-449).  Follows
+// the GPU reference's steps with the defects of both versions corrected, so that the inverse
+// of the channel that generated an event returns its randoms and factor:
+//   * STEP_H: the root of the quadratic with a1 / (a0 a2) (phs_cpu.cpp:97-99), where the GPU
+//     puts z1 (mappings_gpu.hpp:167-169);
+//   * T/U channel, upper half: x = 1 - log(z / a0) / a1 (mappings_gpu.hpp:119-121), where the
+//     CPU drops the "1 -" (phs_cpu.cpp:93);
+//   * phi = atan2(n_y, n_x) in [0, 2 pi) (phs.cu:912-915), where the CPU takes atan(n_y / n_x);
+//   * the angles of a branch are those of its first daughter (phs.cu:938-962), where the CPU
+//     uses the branch's own momentum;
+//   * lambda <= 0 gives factor 0 and p_decay 0 (phs_cpu.cpp:270-279) instead of NaN;
+//   * cos theta = n_z / |n| and sin theta = |n_xy| / |n| (phs.cu:917-920 takes
+//     sqrt(1 - cos^2)), and cos phi, sin phi = n_x, n_y over |n_xy| instead of cos(atan2)
+//     (phs.cu:1084-1085);
+//   * a branch with a friend measures its angles from the friend beam as the forward pass
+//     does (friend_l1), not with the reference's non-orthogonal rotation (phs.cu:1128-1240).
+// Inputs are the event's momenta in the centre-of-mass frame.
+
+// x from msq, mappings_gpu.hpp:15-172 (step_h: phs_cpu.cpp:95-101).
+void map_x_from_msq(int type, double msq, double s, double m, double w, double msq_min, double msq_max,
+                    const double* a, double* x, double* factor) {
+  double z, tmp;
+  switch (type) {
+    case MAP_NO:
+      *x = (msq - msq_min) / (msq_max - msq_min);
+      *factor = (msq_max - msq_min) / s;
+      break;
+    case MAP_SCHANNEL:
+      z = (msq - m * m) / (m * w);
+      *x = (std::atan(z) - a[0]) / (a[1] - a[0]);
+      *factor = a[2] * (1 + z * z);
+      break;
+    case MAP_COLLINEAR: case MAP_INFRARED: case MAP_RADIATION:
+      z = msq - msq_min + a[0];
+      *x = std::log(z / a[0]) / a[1];
+      *factor = a[2] * z;
+      break;
+    case MAP_TCHANNEL: case MAP_UCHANNEL:
+      if (msq < (msq_max + msq_min) / 2) {
+        z = msq - msq_min + a[0];
+        *x = std::log(z / a[0]) / a[1];
+      } else {
+        z = msq_max - msq + a[0];
+        *x = 1 - std::log(z / a[0]) / a[1];
+      }
+      *factor = a[2] * z;
+      break;
+    case MAP_STEP_E:
+      z = (msq - msq_min) / (msq_max - msq_min);
+      tmp = 1 + a[1] * std::exp(z / a[2]);
+      *x = (z - a[2] * std::log(tmp / (1 + a[1]))) / a[0];
+      *factor = a[0] * tmp * (msq_max - msq_min) / s;
+      break;
+    case MAP_STEP_H:
+      z = (msq - msq_min) / (msq_max - msq_min);
+      tmp = a[1] / (a[0] * a[2]);
+      *x = ((a[0] + z / a[2] + tmp) -
+            std::sqrt((a[0] - z / a[2]) * (a[0] - z / a[2]) + 2 * tmp * (a[0] + z / a[2]) + tmp * tmp)) / 2;
+      *factor = (a[1] / (a[0] - *x) / (a[0] - *x) + a[2]) * (msq_max - msq_min) / s;
+      break;
+    default:
+      *x = 0; *factor = 0;
+      break;
+  }
+}
+
+// x from cos theta, mappings_gpu.hpp:181-219 (phs_cpu.cpp:148-172).
+void map_x_from_ct(int type, double ct, const double* b, double* x, double* factor) {
+  switch (type) {
+    case MAP_COLLINEAR: case MAP_TCHANNEL: case MAP_UCHANNEL: {
+      double t;
+      if (ct < 0) {
+        t = ct + b[0] + 1;
+        *x = std::log(t / b[0]) / (2 * b[1]);
+      } else {
+        t = -ct + b[0] + 1;
+        *x = 1 - std::log(t / b[0]) / (2 * b[1]);
+      }
+      *factor = t * b[1];
+      break;
+    }
+    default:
+      *x = (ct + 1) / 2;
+      *factor = 1;
+      break;
+  }
+}
+
+struct InvState {
+  std::vector<double> P, msq, pd;   // per branch index: four-momentum, msq, p_decay
+};
+
+// phs.cu:1262-1284 (_combine_particles): branch momenta as sums of their daughters, msq of
+// every branch from its four-momentum (leaves included).
+void combine(const Proc& P, int c, int b, InvState& S) {
+  if (!P.hc[c * P.n_prt + b]) return;
+  int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
+  combine(P, c, k1, S);
+  combine(P, c, k2, S);
+  for (int i = 0; i < 4; i++) S.P[4 * b + i] = S.P[4 * k1 + i] + S.P[4 * k2 + i];
+}
+
+// phs.cu:626-675 (_apply_msq_inv), post-order as the forward pass draws the msq randoms.
+void inv_msq(const Proc& P, const MapPar<double>& M, int c, int b, double sqrts, double m_tot, InvState& S,
+             double* x, int* id_x, double* factor) {
+  int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
+  double f1 = 1, f2 = 1;
+  if (P.hc[c * P.n_prt + k1]) inv_msq(P, M, c, k1, sqrts, m_tot, S, x, id_x, &f1);
+  if (P.hc[c * P.n_prt + k2]) inv_msq(P, M, c, k2, sqrts, m_tot, S, x, id_x, &f2);
+  int ib = c * P.n_prt_out + b;
+  if (b == P.root) {
+    S.msq[b] = sqrts * sqrts;
+    *factor = f1 * f2;
+  } else {
+    double m_min = P.mass_sum[ib], m_max = sqrts - m_tot + m_min;
+    double f;
+    map_x_from_msq(P.map_id[ib], S.msq[b], sqrts * sqrts, M.m[ib], M.w[ib], m_min * m_min, m_max * m_max,
+                   &M.a[3 * ib], &x[*id_x], &f);
+    (*id_x)++;
+    *factor = f * f1 * f2;
+  }
+  double msq0 = S.msq[b], msq1 = S.msq[k1], msq2 = S.msq[k2];
+  double lda = (msq0 - msq1 - msq2) * (msq0 - msq1 - msq2) - 4 * msq1 * msq2;
+  if (lda > 0) {
+    double m0 = std::sqrt(msq0);
+    S.pd[k1] = std::sqrt(lda) / (2 * m0);
+    S.pd[k2] = -S.pd[k1];
+    *factor *= std::sqrt(lda) / msq0;
+  } else {
+    S.pd[k1] = S.pd[k2] = 0;
+    *factor = 0;
+  }
+}
+
+// phs.cu:924-1241 (_create_boosts_inv_*), pre-order.  G maps centre-of-mass vectors into the
+// frame of the parent's boost matrix (G = L_parent^-1); the branch's randoms are the angles
+// of its first daughter in B_z(-bg) G.
+void inv_angles(const Proc& P, const MapPar<double>& M, int c, int b, double s_friend, InvState& S,
+                const double G[4][4], double* x, int* id_x, double* factor) {
+  if (!P.hc[c * P.n_prt + b]) return;
+  int k1 = P.d1[c * P.n_prt + b], k2 = P.d2[c * P.n_prt + b];
+  int ib = c * P.n_prt_out + b;
+  double m = std::sqrt(S.msq[b]);
+  double bg = m > 0 ? S.pd[b] / m : 0;
+  double g = std::sqrt(1 + bg * bg);
+  double W[4][4];
+  for (int j = 0; j < 4; j++) {
+    W[0][j] = g * G[0][j] - bg * G[3][j];
+    W[1][j] = G[1][j];
+    W[2][j] = G[2][j];
+    W[3][j] = -bg * G[0][j] + g * G[3][j];
+  }
+  const int fs = P.fsign.empty() ? 0 : P.fsign[c * P.n_prt + b];
+  double v[3];
+  for (int i = 0; i < 3; i++)
+    v[i] = W[i + 1][0] * S.P[4 * k1] + W[i + 1][1] * S.P[4 * k1 + 1] + W[i + 1][2] * S.P[4 * k1 + 2] +
+           W[i + 1][3] * S.P[4 * k1 + 3];
+  double Rt[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};   // R^T of the friend rotation
+  if (fs != 0) {
+    // the beam k = (1, 0, 0, f) in the same frame; R takes e_z to it (friend_l1)
+    double n0 = W[1][0] + fs * W[1][3], n1 = W[2][0] + fs * W[2][3], n2 = W[3][0] + fs * W[3][3];
+    double nn = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    double a0 = n0 / nn, a1 = n1 / nn, cc = n2 / nn;
+    double sa = std::sqrt(a0 * a0 + a1 * a1);
+    double kx = sa > 0 ? -a1 / sa : 1.0, ky = sa > 0 ? a0 / sa : 0.0, h = 1 - cc;
+    double R[3][3] = {{cc + h * kx * kx, h * kx * ky, a0}, {h * kx * ky, cc + h * ky * ky, a1}, {-a0, -a1, cc}};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) Rt[i][j] = R[j][i];
+    double u[3] = {v[0], v[1], v[2]};
+    for (int i = 0; i < 3; i++) v[i] = Rt[i][0] * u[0] + Rt[i][1] * u[1] + Rt[i][2] * u[2];
+  }
+  double rho = std::sqrt(v[0] * v[0] + v[1] * v[1]);
+  double vv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  double phi = std::atan2(v[1], v[0]);
+  if (phi < 0) phi += TWOPI_;
+  double cp = rho > 0 ? v[0] / rho : 1.0, sp = rho > 0 ? v[1] / rho : 0.0;
+  double ct = v[2] / vv, st = rho / vv;
+  x[*id_x] = phi / TWOPI_;
+  (*id_x)++;
+  double bb[3] = {M.b[3 * ib], M.b[3 * ib + 1], M.b[3 * ib + 2]};
+  if (fs != 0) {
+    bb[0] = M.m[ib] * M.m[ib] / s_friend;
+    bb[1] = std::log((bb[0] + 1) / bb[0]);
+    bb[2] = 0;
+  }
+  double f;
+  map_x_from_ct(P.map_id[ib], ct, bb, &x[*id_x], &f);
+  (*id_x)++;
+  *factor *= f;
+  // G_b = R_y(-theta) R_z(-phi) R^T W  (= L_b^-1)
+  double Gn[4][4];
+  for (int j = 0; j < 4; j++) {
+    double w1 = Rt[0][0] * W[1][j] + Rt[0][1] * W[2][j] + Rt[0][2] * W[3][j];
+    double w2 = Rt[1][0] * W[1][j] + Rt[1][1] * W[2][j] + Rt[1][2] * W[3][j];
+    double w3 = Rt[2][0] * W[1][j] + Rt[2][1] * W[2][j] + Rt[2][2] * W[3][j];
+    double t1 = cp * w1 + sp * w2;
+    Gn[0][j] = W[0][j];
+    Gn[1][j] = ct * t1 - st * w3;
+    Gn[2][j] = -sp * w1 + cp * w2;
+    Gn[3][j] = st * t1 + ct * w3;
+  }
+  inv_angles(P, M, c, k1, s_friend, S, Gn, x, id_x, factor);
+  inv_angles(P, M, c, k2, s_friend, S, Gn, x, id_x, factor);
+}
+
 }  // namespace
 
 extern "C" {
@@ -690,6 +898,71 @@ int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, cons
       for (int t = 0; t < K; t++) p_tan[((size_t)4 * n_out * i + j) * K + t] = ok ? pd[j].d[t] : 0;
     }
     for (int t = 0; t < K; t++) factor_tan[(size_t)K * i + t] = ok ? f.d[t] : 0;
+  }
+  return 0;
+}
+
+// Inverse mapping of n_events events into every channel (phs.cu:1499-1554): the randoms
+// x_out [n_channels][n_events][n_x] (channel-major, as the Whizard interface returns them,
+// phs.cu:222, 1585) and the factors [n_events][n_channels] each channel would give the event.
+// p [n_events][n_out][4] are the momenta as the forward pass returns them (lab frame when
+// E1 != E2; they are boosted back first).  Every channel is inverted, the generating one too.
+// friends may be NULL.  Returns 0 on success.
+int oracle_inverse(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
+                   const int* friends, const int* map_ids, const double* map_m, const double* map_w,
+                   const double* flv, double sqrts, const double* E1, const double* E2, long n_events,
+                   int n_x, const double* p, double* factors, double* x_out) {
+  Proc P;
+  if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  if (n_x < 3 * n_out - 4) return 2;
+  if (friends) set_friends(P, friends);
+  MapPar<double> M;
+  size_t nb = (size_t)n_channels * P.n_prt_out;
+  M.m.assign(map_m, map_m + nb); M.w.assign(map_w, map_w + nb);
+  M.a.assign(3 * nb, 0.0); M.b.assign(3 * nb, 0.0);
+  double cur_sqrts = sqrts;
+  for (int c = 0; c < n_channels; c++) init_constants(P, M, c, cur_sqrts);
+  InvState S;
+  S.P.assign(4 * (size_t)P.n_prt, 0.0); S.msq.assign(P.n_prt, 0.0); S.pd.assign(P.n_prt, 0.0);
+  std::vector<double> xb(n_x);
+  for (long i = 0; i < n_events; i++) {
+    double s_ev = sqrts, bg_lab = 0, s_friend = sqrts * sqrts;
+    if (E1 && E2) {
+      s_ev = 2 * std::sqrt(E1[i] * E2[i]);
+      bg_lab = (E1[i] - E2[i]) / (2 * std::sqrt(E1[i] * E2[i]));
+      s_friend = 4 * E1[i] * E2[i];
+      if (s_ev != cur_sqrts) {
+        cur_sqrts = s_ev;
+        for (int cc = 0; cc < n_channels; cc++) init_constants(P, M, cc, cur_sqrts);
+      }
+    }
+    const double g_lab = std::sqrt(1 + bg_lab * bg_lab);
+    for (int c = 0; c < n_channels; c++) {
+      // leaves: the event's momenta, back in the centre-of-mass frame (inverse of phs.cu:1286-1301)
+      for (int j = 0, b = 1; j < n_out; j++, b *= 2) {
+        const double* q = &p[(4 * (size_t)n_out) * i + 4 * j];
+        double* Pl = &S.P[4 * (b - 1)];
+        Pl[0] = q[0]; Pl[1] = q[1]; Pl[2] = q[2]; Pl[3] = q[3];
+        if (bg_lab != 0) {
+          Pl[0] = g_lab * q[0] - bg_lab * q[3];
+          Pl[3] = -bg_lab * q[0] + g_lab * q[3];
+        }
+      }
+      combine(P, c, P.root, S);
+      for (int b = 0; b < P.n_prt_out; b++) {
+        const double* q = &S.P[4 * b];
+        S.msq[b] = q[0] * q[0] - q[1] * q[1] - q[2] * q[2] - q[3] * q[3];
+      }
+      S.pd[P.root] = 0;
+      double f = 1;
+      int id_x = 0;
+      double m_tot = P.mass_sum[c * P.n_prt_out + P.root];
+      inv_msq(P, M, c, P.root, s_ev, m_tot, S, xb.data(), &id_x, &f);
+      double G[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+      inv_angles(P, M, c, P.root, s_friend, S, G, xb.data(), &id_x, &f);
+      factors[(size_t)n_channels * i + c] = f;
+      for (int k = 0; k < n_x; k++) x_out[((size_t)c * n_events + i) * n_x + k] = k < id_x ? xb[k] : 0.0;
+    }
   }
   return 0;
 }

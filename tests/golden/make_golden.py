@@ -8,6 +8,7 @@ tree is mounted; the fixtures are committed, the reference never travels).
    (momenta of the final particles, factor, volume, ok, msq of every branch).
 2. ``ttH_head.ref`` / ``bwbw_head.ref``: the reference's sample files samples/ttH.ref and
    samples/bwbw.ref cut to their header, trees and first events (Whizard's own momenta).
+3. ``bwbw_friends.ref``: samples/bwbw.ref with only the events of its trees that have friends.
 
     python tests/golden/make_golden.py
 """
@@ -76,6 +77,32 @@ def head_ref(src, dst, n_events):
     print(f"{dst}: {len(keep)} lines")
 
 
+def friend_ref(src, dst):
+    """The header, trees and flavours of ``src`` and those of its events whose channel has
+    ``hasfriends: 1`` (Whizard's momenta and all-channel factors for trees with friends)."""
+    with open(src) as f:
+        lines = f.readlines()
+    hdr = {}
+    for ln in lines[:6]:
+        k, v = ln.split()
+        hdr[k] = int(v)
+    n_ch, n_out = hdr["n_channels:"], hdr["NOUT:"]
+    n_tree_lines = 7 * n_ch + 2
+    friend = [int(lines[6 + 7 * c + 3].split()[1]) for c in range(n_ch)]
+    per_event = 5 + 2 + n_out
+    keep = lines[:6 + n_tree_lines]
+    pos, n = 6 + n_tree_lines, 0
+    while pos + per_event <= len(lines):
+        blk = lines[pos:pos + per_event]
+        pos += per_event
+        if friend[int(blk[0].split()[1]) - 1]:
+            keep += blk
+            n += 1
+    with open(dst, "w") as f:
+        f.writelines(keep)
+    print(f"{dst}: {n} events")
+
+
 def main():
     if not oracle.ref_available():
         oracle.build()
@@ -87,6 +114,7 @@ def main():
     if os.path.isdir(REF_SAMPLES):
         head_ref(os.path.join(REF_SAMPLES, "ttH.ref"), os.path.join(HERE, "ttH_head.ref"), 300)
         head_ref(os.path.join(REF_SAMPLES, "bwbw.ref"), os.path.join(HERE, "bwbw_head.ref"), 150)
+        friend_ref(os.path.join(REF_SAMPLES, "bwbw.ref"), os.path.join(HERE, "bwbw_friends.ref"))
 
 
 if __name__ == "__main__":
