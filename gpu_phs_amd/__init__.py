@@ -64,6 +64,8 @@ _SIGS = {
                                             ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_phs_plan_n_params": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_plan_param_map": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int]),
+    "gpu_phs_plan_inverse": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
 }
 
 
@@ -215,6 +217,29 @@ class Plan:
 
     def run_graph(self, stream=None):
         _check(lib().gpu_phs_plan_run_graph(self.handle, _stream_ptr(stream)), "plan_run_graph")
+
+    def inverse(self, seg_rows=None, n_rows=None, factors=None, x_out=None, block=0, stream=None):
+        """All-channel inverse mapping of the plan's events (after a forward run).  Rows are
+        the segments' events in segment order unless ``seg_rows`` gives each segment's first
+        row.  Returns (factors [n_rows][n_channels], x_out [n_channels][n_rows][n_x])."""
+        import torch
+        ns = [buf.x.shape[0] for _, buf in self._keep]
+        if seg_rows is None:
+            seg_rows = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+        seg_rows = np.ascontiguousarray(seg_rows, np.int64)
+        if n_rows is None:
+            n_rows = int(max(r + n for r, n in zip(seg_rows, ns)))
+        ph = self._keep[0][0]
+        n_ch = ph.proc.n_channels
+        dev = self._keep[0][1].x.device
+        if factors is None:
+            factors = torch.empty((n_rows, n_ch), dtype=torch.float64, device=dev)
+        if x_out is None:
+            x_out = torch.empty((n_ch, n_rows, ph.n_x), dtype=torch.float64, device=dev)
+        _check(lib().gpu_phs_plan_inverse(self.handle, _ptr(seg_rows), n_rows, ctypes.c_void_p(factors.data_ptr()),
+                                          ctypes.c_void_p(x_out.data_ptr()), block, _stream_ptr(stream)),
+               "plan_inverse")
+        return factors, x_out
 
     def n_params(self):
         return _check(lib().gpu_phs_plan_n_params(self.handle), "plan_n_params")

@@ -63,6 +63,7 @@ int drop_plan(int h) { return g_plans.erase(h) ? GPU_PHS_OK : GPU_PHS_E_HANDLE; 
 
 PlanObj::~PlanObj() {
   if (adj_dev) (void)hipFree(adj_dev);
+  if (inv_dev) (void)hipFree(inv_dev);
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (capture_stream) (void)hipStreamDestroy(capture_stream);

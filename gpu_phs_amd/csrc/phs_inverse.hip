@@ -1,0 +1,297 @@
+// Inverse mapping kernel: for every event and every channel of its process, the randoms and
+// the phase-space factor under which that channel would have produced the event's momenta
+// (what Whizard's multi-channel weight needs; reference phs.cu:626-675, 924-1284, driven per
+// channel by phs.cu:1499-1554 with 2 + 2(n-1) dependent launches per channel).
+//
+// One lane = one event, one launch for all channels: a workgroup takes a tile of events of
+// one generating channel, stages their momenta (boosted back to the centre-of-mass frame) in
+// an LDS column per lane, and walks every channel's program in turn (wave-uniform: all lanes
+// invert into the same channel at the same time, reading its NodeCmd records with scalar
+// loads).  The generating channel's slot gets the forward pass's own factor and randoms, as
+// the reference's _move_factors / _move_x do (phs.cu:1495-1497).
+//
+// The steps per branch follow the GPU reference with its defects corrected (the oracle's
+// inverse lists them): the msq of both daughters from sums of final-state momenta, x and the
+// map factor of each internal daughter from its msq, lambda and the decay momentum, then the
+// angles of the first daughter in B_z(-bg) G, where G = L_parent^-1 is carried down the tree
+// (a branch with a friend measures them from the friend beam), x_phi = atan2 / 2 pi, x of
+// cos theta from its map, and G <- R_y(-theta) R_z(-phi) B_z(-bg) G for the daughters.
+// Deferred subtrees wait in an LDS save bank (G and the subtree's msq, m, p_decay).
+#include <hip/hip_runtime.h>
+
+#include "phs_device.h"
+#include "phs_kernels.h"
+
+namespace gphs {
+
+namespace {
+
+constexpr double kTwoPi = 6.28318530717958647693;   // mappings.h:5
+constexpr int kInvBankWords = 19;
+
+extern __shared__ double inv_lds[];
+
+// x and the map factor of an internal daughter from its msq (mappings_gpu.hpp:15-172; STEP_H
+// as phs_cpu.cpp:95-101, whose root of the quadratic is the right one).
+__device__ __forceinline__ void x_from_msq(const ChildCmd& c, double msq, double s, double& x, double& f) {
+  switch (c.kind) {
+    case MK_LINEAR:
+      x = (msq - c.msq_min) / (c.msq_max - c.msq_min);
+      f = c.f_lin;
+      break;
+    case MK_BREIT_WIGNER: {
+      const double z = (msq - c.m * c.m) / (c.m * c.w);
+      x = (atan(z) - c.a0) / (c.a1 - c.a0);
+      f = c.a2 * (1 + z * z);
+      break;
+    }
+    case MK_EXP: {
+      const double z = msq - c.msq_min + c.a0;
+      x = math::log_bounded(z / c.a0) / c.a1;
+      f = c.a2 * z;
+      break;
+    }
+    case MK_TU: {
+      const bool lo = msq < (c.msq_max + c.msq_min) / 2;
+      const double z = lo ? msq - c.msq_min + c.a0 : c.msq_max - msq + c.a0;
+      const double l = math::log_bounded(z / c.a0) / c.a1;
+      x = lo ? l : 1 - l;
+      f = c.a2 * z;
+      break;
+    }
+    case MK_STEP_H: {
+      const double z = (msq - c.msq_min) / (c.msq_max - c.msq_min);
+      const double t = c.a1 / (c.a0 * c.a2);
+      const double u = c.a0 + z / c.a2, v = c.a0 - z / c.a2;
+      x = ((u + t) - math::sqrt_phys(v * v + 2 * t * u + t * t)) / 2;
+      f = (c.a1 / (c.a0 - x) / (c.a0 - x) + c.a2) * (c.msq_max - c.msq_min) / s;
+      break;
+    }
+    default: {  // MK_STEP_E
+      const double z = (msq - c.msq_min) / (c.msq_max - c.msq_min);
+      const double t = 1 + c.a1 * math::exp_bounded(z / c.a2);
+      x = (z - c.a2 * math::log_bounded(t / (1 + c.a1))) / c.a0;
+      f = c.a0 * t * (c.msq_max - c.msq_min) / s;
+      break;
+    }
+  }
+}
+
+// x and factor of cos theta (mappings_gpu.hpp:181-219).
+__device__ __forceinline__ void x_from_ct(int kind, double ct, double b0, double b1, double& x, double& f) {
+  if (kind == CK_FLAT) {
+    x = (ct + 1) / 2;
+    f = 1;
+  } else {
+    const bool neg = ct < 0;
+    const double t = neg ? ct + b0 + 1 : -ct + b0 + 1;
+    const double l = math::log_bounded(t / b0) / (2 * b1);
+    x = neg ? l : 1 - l;
+    f = t * b1;
+  }
+}
+
+// Sum of the staged final-state momenta in `leaves` (lane column pl, stride S).
+__device__ __forceinline__ void leaf_sum(const double* __restrict__ pl, int S, int n_out, uint32_t leaves,
+                                         double (&q)[4]) {
+  q[0] = q[1] = q[2] = q[3] = 0;
+  for (int j = 0; j < n_out; j++) {
+    if ((leaves >> j) & 1u) {
+      const double* r = pl + 4 * j * S;
+      q[0] = q[0] + r[0]; q[1] = q[1] + r[S]; q[2] = q[2] + r[2 * S]; q[3] = q[3] + r[3 * S];
+    }
+  }
+}
+
+// Inverse of one event into channel `cd` (program `prog`): the randoms go to xo[0..n_x),
+// returns the factor.
+template <bool FR>
+__device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
+                                                const double* __restrict__ pl, int S, double* bank, int B,
+                                                gdouble* __restrict__ xo) {
+  const double s = cd.s;
+  double G[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  double cur_msq = s, cur_m = cd.sqrts, cur_p = 0, factor = 1;
+  int depth = 0;
+  for (int i = 0; i < cd.n_nodes; ++i) {
+    const NodeCmd& nc = prog[i];
+    double q0[4], q1[4];
+    leaf_sum(pl, S, cd.n_out, nc.k[0].leaves, q0);
+    leaf_sum(pl, S, cd.n_out, nc.k[1].leaves, q1);
+    const double msq0 = q0[0] * q0[0] - q0[1] * q0[1] - q0[2] * q0[2] - q0[3] * q0[3];
+    const double msq1 = q1[0] * q1[0] - q1[1] * q1[1] - q1[2] * q1[2] - q1[3] * q1[3];
+    // x and factor of the internal daughters' msq maps
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      if (!nc.k[d].leaf) {
+        double x, f;
+        x_from_msq(nc.k[d], d == 0 ? msq0 : msq1, s, x, f);
+        xo[nc.k[d].xi] = x;
+        factor *= f;
+      }
+    }
+    // this branch's decay (phs.cu:658-664), lambda <= 0 gives factor 0 (phs_cpu.cpp:275-279)
+    const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
+    const bool pos = lda > 0;
+    const double sl = pos ? math::sqrt_phys(lda) : 0.0;
+    const double pk = sl / (2 * cur_m);
+    factor *= pos ? sl / cur_msq : 0.0;
+
+    // angles of the first daughter in W = B_z(-bg) G
+    const double bg = cur_m > 0 ? cur_p / cur_m : 0.0;
+    const double g = math::sqrt_phys(1 + bg * bg);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double r0 = G[j], r3 = G[12 + j];
+      G[j] = g * r0 - bg * r3;
+      G[12 + j] = -bg * r0 + g * r3;
+    }
+    double v0 = G[4] * q0[0] + G[5] * q0[1] + G[6] * q0[2] + G[7] * q0[3];
+    double v1 = G[8] * q0[0] + G[9] * q0[1] + G[10] * q0[2] + G[11] * q0[3];
+    double v2 = G[12] * q0[0] + G[13] * q0[1] + G[14] * q0[2] + G[15] * q0[3];
+    if (FR && nc.friend_sign != 0) {
+      // the beam k = (1, 0, 0, f) in the same frame fixes R (R e_z = n / |n|); v <- R^T v and
+      // the rows of W likewise, so that G below is R_y(-theta) R_z(-phi) R^T W
+      const double fs = (double)nc.friend_sign;
+      const double n0 = G[4] + fs * G[7], n1 = G[8] + fs * G[11], n2 = G[12] + fs * G[15];
+      const double nn = math::sqrt_phys(n0 * n0 + n1 * n1 + n2 * n2);
+      const double a0 = n0 / nn, a1 = n1 / nn, c = n2 / nn;
+      const double sa = math::sqrt_phys(a0 * a0 + a1 * a1);
+      const double kx = sa > 0 ? -a1 / sa : 1.0, ky = sa > 0 ? a0 / sa : 0.0, h = 1 - c;
+      const double r00 = c + h * kx * kx, r01 = h * kx * ky, r11 = c + h * ky * ky;
+      // R^T rows: (r00, r01, -a0), (r01, r11, -a1), (a0, a1, c)
+      const double u0 = r00 * v0 + r01 * v1 + -a0 * v2;
+      const double u1 = r01 * v0 + r11 * v1 + -a1 * v2;
+      const double u2 = a0 * v0 + a1 * v1 + c * v2;
+      v0 = u0; v1 = u1; v2 = u2;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const double w1 = G[4 + j], w2 = G[8 + j], w3 = G[12 + j];
+        G[4 + j] = r00 * w1 + r01 * w2 + -a0 * w3;
+        G[8 + j] = r01 * w1 + r11 * w2 + -a1 * w3;
+        G[12 + j] = a0 * w1 + a1 * w2 + c * w3;
+      }
+    }
+    const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
+    const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
+    double phi = atan2(v1, v0);
+    phi = phi < 0 ? phi + kTwoPi : phi;
+    const double cp = rho > 0 ? v0 / rho : 1.0, sp = rho > 0 ? v1 / rho : 0.0;
+    const double ct = v2 / vv, st = rho / vv;
+    xo[nc.x_phi] = phi / kTwoPi;
+    double xc, fct;
+    x_from_ct(nc.ct_kind, ct, nc.b0, nc.b1, xc, fct);
+    xo[nc.x_phi + 1] = xc;
+    factor *= fct;
+    // G <- R_y(-theta) R_z(-phi) G (rows 1..3)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double w1 = G[4 + j], w2 = G[8 + j], w3 = G[12 + j];
+      const double t1 = cp * w1 + sp * w2;
+      G[4 + j] = ct * t1 - st * w3;
+      G[8 + j] = -sp * w1 + cp * w2;
+      G[12 + j] = st * t1 + ct * w3;
+    }
+
+    // next branch: a daughter (G stays), or the deferred one from the save bank
+    if (nc.push >= 0) {
+      double* b = bank + kInvBankWords * depth * B;
+#pragma unroll
+      for (int j = 0; j < 16; j++) b[j * B] = G[j];
+      const double mp = nc.push == 0 ? msq0 : msq1;
+      b[16 * B] = mp;
+      b[17 * B] = math::sqrt_phys(mp);
+      b[18 * B] = nc.push == 0 ? pk : -pk;
+      depth++;
+    }
+    if (nc.desc >= 0) {
+      cur_msq = nc.desc == 0 ? msq0 : msq1;
+      cur_m = math::sqrt_phys(cur_msq);
+      cur_p = nc.desc == 0 ? pk : -pk;
+    } else if (depth > 0) {
+      depth--;
+      const double* b = bank + kInvBankWords * depth * B;
+#pragma unroll
+      for (int j = 0; j < 16; j++) G[j] = b[j * B];
+      cur_msq = b[16 * B];
+      cur_m = b[17 * B];
+      cur_p = b[18 * B];
+    }
+  }
+  return factor;
+}
+
+// LDS per workgroup: the momenta column (4 n_out rows of blockDim.x + 1 doubles), then the
+// save banks (19 n_banks rows of blockDim.x doubles).
+template <bool FR>
+__global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict__ tiles, int n_tiles,
+                                                      const SegDesc* __restrict__ segs,
+                                                      const InvSeg* __restrict__ isegs,
+                                                      const ChannelDesc* __restrict__ chans,
+                                                      const NodeCmd* __restrict__ nodes, int n_ch, int n_out,
+                                                      int n_x, int64_t n_rows, double* __restrict__ factors,
+                                                      double* __restrict__ x_out) {
+  const int B = (int)blockDim.x;
+  const int S = B + 1;
+  const int lane = (int)threadIdx.x;
+  double* ps = inv_lds;
+  double* bank = ps + 4 * n_out * S + lane;
+  double* pl = ps + lane;
+  const double bg_lab = chans[0].bg_lab, g_lab = chans[0].g_lab;
+  for (int t = (int)blockIdx.x; t < n_tiles; t += (int)gridDim.x) {
+    const TileDesc td = tiles[t];
+    const SegDesc sd = segs[td.seg];
+    const InvSeg is = isegs[td.seg];
+    for (int base = 0; base < td.count; base += B) {
+      const int cnt = min(B, td.count - base);
+      const bool valid = lane < cnt;   // lanes past the end stage the last event and store nothing
+      const int64_t e = td.e0 + base + (valid ? lane : cnt - 1);
+      const int64_t row = is.row0 + e;
+      // momenta back in the centre-of-mass frame (inverse of phs.cu:1286-1301)
+      const gdouble* pe = as_global((const double*)sd.p) + e * 4 * n_out;
+      for (int j = 0; j < n_out; j++) {
+        double q0 = pe[4 * j], q1 = pe[4 * j + 1], q2 = pe[4 * j + 2], q3 = pe[4 * j + 3];
+        if (bg_lab != 0) {
+          const double a = g_lab * q0 - bg_lab * q3;
+          const double b = -bg_lab * q0 + g_lab * q3;
+          q0 = a; q3 = b;
+        }
+        pl[(4 * j) * S] = q0; pl[(4 * j + 1) * S] = q1; pl[(4 * j + 2) * S] = q2; pl[(4 * j + 3) * S] = q3;
+      }
+      for (int c = 0; c < n_ch; c++) {
+        gdouble* xo = as_global(x_out) + ((int64_t)c * n_rows + row) * n_x;
+        if (c == is.own) {
+          if (valid) {
+            const gdouble* xi = as_global(sd.x) + e * n_x;
+            for (int k = 0; k < n_x; k++) xo[k] = xi[k];
+            as_global(factors)[row * n_ch + c] = as_global((const double*)sd.factor)[e];
+          }
+          continue;
+        }
+        const ChannelDesc cd = chans[c];
+        if (valid) as_global(factors)[row * n_ch + c] = inverse_event<FR>(cd, nodes + cd.node0, pl, S, bank, B, xo);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+size_t inverse_lds_bytes(int block, int n_out, int n_banks) {
+  return ((size_t)4 * n_out * (block + 1) + (size_t)kInvBankWords * n_banks * block) * sizeof(double);
+}
+
+hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
+                          int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream) {
+  if (n_tiles <= 0) return hipSuccess;
+  if (block < 64 || block > 256 || block % 64) return hipErrorInvalidValue;
+  const size_t shmem = inverse_lds_bytes(block, it.n_out, it.n_banks);
+  if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
+  const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
+  auto k = it.has_friends ? inverse_kernel<true> : inverse_kernel<false>;
+  hipLaunchKernelGGL(k, dim3(g), dim3(block), shmem, stream, tiles, n_tiles, segs, it.isegs, it.chans, it.nodes,
+                     it.n_ch, it.n_out, it.n_x, n_rows, factors, x_out);
+  return hipGetLastError();
+}
+
+}  // namespace gphs

@@ -53,4 +53,23 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_m
                           int n_sub, double* granges, const int* chan_tiles, const int* chan_tile_off, int n_chans,
                           double* gparam, int grid, hipStream_t stream);
 
+// Inverse mapping.  Per segment of the plan: the channel that generated its events (its
+// factor and randoms are copied) and the output row of its first event.
+struct InvSeg {
+  int32_t own;
+  int32_t pad;
+  int64_t row0;
+};
+// Device copies of one process's complete channel table (every channel is a target).
+struct InverseTables {
+  const InvSeg* isegs = nullptr;
+  const ChannelDesc* chans = nullptr;
+  const NodeCmd* nodes = nullptr;
+  int n_ch = 0, n_out = 0, n_x = 0, n_banks = 0, has_friends = 0;
+};
+size_t inverse_lds_bytes(int block, int n_out, int n_banks);
+// factors [n_rows][n_ch] and x_out [n_ch][n_rows][n_x] for the events of tiles [0, n_tiles).
+hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
+                          int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream);
+
 }  // namespace gphs

@@ -68,6 +68,12 @@ struct PlanObj {
   int adj_sub = 1;            // 64-event sub-tiles per tile (one wavefront each)
   bool adj_valid = false;     // the device copies above match adj_segs
   std::vector<int> chan_process, chan_channel;   // plan channel -> (process handle, channel)
+  // inverse mapping: the process's whole channel table and the per-segment rows, on the device
+  void* inv_dev = nullptr;
+  size_t inv_bytes = 0;
+  std::vector<InvSeg> inv_segs;
+  int inv_process = -1;
+  InverseTables inv;
   ~PlanObj();
 };
 
@@ -83,6 +89,9 @@ int drop_plan(int h);
 int plan_build(PlanObj& pl);
 int plan_run(PlanObj& pl, hipStream_t stream);
 int plan_run_plain(PlanObj& pl, hipStream_t stream);
+// Inverse mapping of the plan's events into every channel of their process (block 0 = 128).
+int plan_inverse(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows, double* factors, double* x_out, int block,
+                 hipStream_t stream);
 
 // Device check used by every entry point that touches the GPU.
 int require_device();

@@ -43,7 +43,7 @@ struct ChildCmd {
   double da_dm[3];   // d a[k] / d M of this daughter's map (adjoint)
   double da_dw[3];   // d a[k] / d Gamma
   int32_t node;      // visit index of this daughter's NodeCmd (internal), -1 for a leaf
-  int32_t pad0;
+  uint32_t leaves;   // bit j set: final-state particle j lies below this daughter (inverse mapping)
 };
 
 struct NodeCmd {

@@ -223,6 +223,7 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
     auto mid = [&](int b) { return sp.map_ids[c * n_prt_out + b]; };
     auto fill_child = [&](int k, ChildCmd& cc) -> std::string {
       cc = ChildCmd();
+      cc.leaves = (uint32_t)(k + 1);   // a bincode is the set of final-state particles below it
       if (!HC[k]) {
         int j = ilog2(k + 1);
         double mj = sp.flv_masses[sp.n_in + j];

@@ -3,6 +3,7 @@
 // of the process/plan machinery of phs_api.cpp.  The reference keeps its state in process
 // globals (global_phs.cpp:6-27, phs.cu:27-41); this file keeps the equivalent in one static
 // module object, so the calls keep their order-dependent semantics.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,6 +27,9 @@ struct WhizardModule {
   uint8_t* ok_d = nullptr;
   size_t cap = 0;                   // events the buffers hold
   int cap_nx = 0, cap_nout = 0;
+  // inverse mapping outputs: factors [n][n_channels], randoms [n_channels][n][n_x]
+  double* fa_d = nullptr; double* xo_d = nullptr;
+  size_t inv_cap = 0;               // doubles of xo_d
   double* last_p = nullptr;         // device momenta of the last generate call
 };
 
@@ -36,9 +40,12 @@ WhizardModule& W() {
 
 void free_buffers(WhizardModule& w) {
   (void)hipFree(w.x_d); (void)hipFree(w.p_d); (void)hipFree(w.f_d); (void)hipFree(w.v_d); (void)hipFree(w.ok_d);
+  (void)hipFree(w.fa_d); (void)hipFree(w.xo_d);
   w.x_d = w.p_d = w.f_d = w.v_d = nullptr;
+  w.fa_d = w.xo_d = nullptr;
   w.ok_d = nullptr;
   w.cap = 0;
+  w.inv_cap = 0;
   w.last_p = nullptr;
 }
 
@@ -55,6 +62,22 @@ int ensure_buffers(WhizardModule& w, size_t n) {
     return hip_error(e, "hipMalloc(event buffers)");
   }
   w.cap = n; w.cap_nx = w.n_x; w.cap_nout = w.n_out;
+  return GPU_PHS_OK;
+}
+
+int ensure_inverse_buffers(WhizardModule& w, size_t n) {
+  const size_t need = n * w.n_channels * w.n_x;
+  if (need <= w.inv_cap && w.fa_d) return GPU_PHS_OK;
+  (void)hipFree(w.fa_d); (void)hipFree(w.xo_d);
+  w.fa_d = w.xo_d = nullptr;
+  w.inv_cap = 0;
+  hipError_t e;
+  if ((e = hipMalloc(&w.fa_d, n * w.n_channels * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.xo_d, std::max<size_t>(need, 1) * sizeof(double))) != hipSuccess) {
+    (void)hipFree(w.fa_d); w.fa_d = nullptr;
+    return hip_error(e, "hipMalloc(inverse buffers)");
+  }
+  w.inv_cap = need;
   return GPU_PHS_OK;
 }
 
@@ -276,23 +299,22 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
   for (int c = 0; c < kTileClasses; c++) pl.lcs[c].block = w.block;
   if (plan_build(pl)) return;
   if (plan_run(pl, nullptr)) return;
-  std::vector<double> f(n);
+  // the inverse mapping of every event into every channel (the reference always asks for it,
+  // whizard_gpu_phs.cpp:118): factors [n][n_channels] at [n_channels i + c] (phs.cu:207) and
+  // x_out channel-major [n_x n c + n_x i + k] (phs.cu:222), own channel included
+  if (ensure_inverse_buffers(w, (size_t)n)) return;
+  std::vector<int64_t> rows(segs.size());
+  for (size_t k = 0, r = 0; k < segs.size(); r += segs[k].n_events, k++) rows[k] = (int64_t)r;
+  if (plan_inverse(pl, rows.data(), n, w.fa_d, w.xo_d, 0, nullptr)) return;
+  const int nc = w.n_channels;
   if ((e = hipMemcpy(p, w.p_d, (size_t)n * w.n_out * 4 * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-      (e = hipMemcpy(f.data(), w.f_d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(factors, w.fa_d, (size_t)n * nc * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
       (e = hipMemcpy(volumes, w.v_d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-      (e = hipMemcpy(oks, w.ok_d, (size_t)n, hipMemcpyDeviceToHost)) != hipSuccess) {
+      (e = hipMemcpy(oks, w.ok_d, (size_t)n, hipMemcpyDeviceToHost)) != hipSuccess ||
+      (x_out && (e = hipMemcpy(x_out, w.xo_d, (size_t)n * nc * w.n_x * sizeof(double), hipMemcpyDeviceToHost)) !=
+                    hipSuccess)) {
     hip_error(e, "hipMemcpy(outputs)");
     return;
-  }
-  // own-channel factor at [n_channels * i + c] (phs.cu:207); x_out channel-major (phs.cu:222)
-  const int nc = w.n_channels;
-  std::memset(factors, 0, (size_t)n * nc * sizeof(double));
-  for (int64_t i = 0; i < n; i++) factors[nc * i + w.channel_ids[i]] = f[i];
-  if (x_out) {
-    std::memset(x_out, 0, (size_t)n * nc * w.n_x * sizeof(double));
-    for (int64_t i = 0; i < n; i++)
-      std::memcpy(&x_out[(size_t)w.n_x * n * w.channel_ids[i] + (size_t)w.n_x * i], &x[(size_t)w.n_x * i],
-                  w.n_x * sizeof(double));
   }
   w.last_p = w.p_d;
   ok_status();

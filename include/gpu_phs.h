@@ -182,6 +182,20 @@ int gpu_phs_plan_n_params(int plan);
  * width it refers to.  Arrays hold at least n_params / 2 entries. */
 int gpu_phs_plan_param_map(int plan, int* process, int* channel, int* bincode, int n);
 
+/* Inverse mapping (replaces the do_inverse_mapping branch of gen_phs_from_x_gpu,
+ * phs.cu:1491-1554, and its kernels phs.cu:626-675, 924-1284): for every event of the
+ * plan's segments and every channel c of their process, the randoms x_out and the factor
+ * under which channel c would have produced the event's momenta.  Run after the forward
+ * pass (it reads the segments' p, x and factor).  All segments must come from one process.
+ * seg_row0[s] is the output row of segment s's first event; outputs are device arrays
+ * factors [n_rows][n_channels] and x_out [n_channels][n_rows][n_x] (channel-major, as
+ * Whizard reads them, phs.cu:222, 1585).  The generating channel's slot gets the forward
+ * factor and randoms.  block: threads per workgroup (64..256, 0 = 128).  Where the reference
+ * has defects (the STEP_H inverse, the T/U inverse of the CPU version, the friend rotation)
+ * the inverse of the generating channel returns its randoms instead. */
+int gpu_phs_plan_inverse(int plan, const int64_t* seg_row0, int64_t n_rows, double* factors, double* x_out,
+                         int block, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

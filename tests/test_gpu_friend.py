@@ -60,3 +60,19 @@ def test_adjoint_rejects_friends():
     g_x = [torch.zeros_like(b.x) for b in bufs]
     with pytest.raises(G.GpuPhsError, match="friend"):
         plan.adjoint(g_x)
+
+
+def test_friend_trees_reproduce_whizard_momenta():
+    """The 149 events of samples/bwbw.ref whose trees have friends, with the friend assignment
+    that Whizard's own factors single out (test_whizard_friends.py), through the kernel."""
+    import os
+    from refio import read_ref
+    from test_whizard_friends import FRIENDS
+    proc, ev = read_ref(os.path.join(os.path.dirname(__file__), "golden", "bwbw_friends.ref"))
+    proc.friends = np.zeros_like(proc.d1)
+    for c, (b, which) in FRIENDS.items():
+        proc.friends[c, b - 1] = 1 << (proc.n_out + which - 1)
+    out, _, _ = run_gpu(proc, ev["x"], ev["channel"])
+    assert out["ok"].all()
+    dev = np.abs(out["p"] - ev["p"][:, proc.n_in:, :]).max() / proc.sqrts
+    assert dev < 1e-6, dev

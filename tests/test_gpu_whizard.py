@@ -11,6 +11,7 @@ import oracle
 from gpu_phs_amd import workloads as W
 from parity import assert_parity, compare
 from refio import read_ref
+from test_gpu_inverse import assert_inverse_parity, inverse_stats, sensitivity
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -81,12 +82,14 @@ def test_whizard_abi_matches_oracle():
     r = compare(dict(p=p, factor=own, volume=volumes, ok=oks), ref)
     print(r)
     assert_parity(r)
-    others = factors.copy()
-    others[np.arange(n), ch] = 0
-    assert (others == 0).all()
+    # every other channel's factor and randoms: the inverse mapping of the event's momenta
     for c in range(3):
         sel = ch == c
-        assert np.array_equal(x_out[c][sel], x[sel]) and (x_out[c][~sel] == 0).all()
+        assert np.array_equal(x_out[c][sel], x[sel])
+    p_ok = np.where(oks[:, None, None], p, 0.0)
+    f_ref, x_ref = oracle.inverse(proc, p_ok)
+    df, dx = inverse_stats(proc, factors, x_out, f_ref, x_ref, ch, oks)
+    assert_inverse_parity(df, dx, *sensitivity(proc, p_ok, f_ref, x_ref, ch, oks))
     # the device pointer of the last call's momenta
     addr = np.zeros(1, np.float64)
     L.c_whizard_get_momentum_device_pointer(_p(addr))
