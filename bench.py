@@ -98,6 +98,28 @@ def run_adjoint(steps, warmup, torch, G, W):
                 loss=float(sum(l.sum().item() for l in loss)))
 
 
+def run_inverse(name, steps, warmup, torch, G, W, scale=1.0, block=0):
+    """All-channel inverse mapping of a workload's events (after one forward pass): every
+    event's randoms and factor in every channel of its process, one launch."""
+    wl = W.workload(name, scale=scale)
+    handles, segs = make_inputs(wl, torch, G)
+    plan = G.Plan(segs)
+    plan.run()
+    proc = wl.processes[0]
+    n = wl.n_events
+    f = torch.empty((n, proc.n_channels), dtype=torch.float64, device="cuda")
+    xo = torch.empty((proc.n_channels, n, proc.n_x), dtype=torch.float64, device="cuda")
+    plan.inverse(factors=f, x_out=xo, block=block)
+    wall, ev_s = time_steps(lambda: plan.inverse(factors=f, x_out=xo, block=block), steps, warmup, torch)
+    nc = proc.n_channels
+    # reads: momenta, the own randoms and factor; writes: every channel's factor and randoms
+    bytes_ = n * (32 * proc.n_out + 8 * proc.n_x + 8) + n * nc * (8 + 8 * proc.n_x)
+    ms = ev_s / steps * 1e3
+    return dict(n_events=n, channels=nc, kernel_ms=ms, events_s=n * steps / wall,
+                pairs_s=n * nc * steps / wall, bytes_per_launch=bytes_,
+                hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, block=block or 128)
+
+
 def cpu_baseline(wl_name, W):
     """The reference's own phs_cpu.cpp (oracle/_ref build) on one host core, over a bounded
     sample of the workload's events (repeated until ~10 s of CPU work)."""
@@ -152,6 +174,8 @@ def main():
                     help="fixed launch config block,ept,grid,order,waves_per_eu,lds instead of autotuning")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the forward + adjoint workload (c4) and print its figures")
+    ap.add_argument("--inverse-only", action="store_true",
+                    help="only run the all-channel inverse mapping of c1 and c2 and print its figures")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -179,6 +203,11 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(run_adjoint(max(5, args.steps), args.warmup, torch, G, W)))
+        return
+    if args.inverse_only:
+        r = {"c1": run_inverse("c1", max(3, args.steps // 2), 1, torch, G, W),
+             "c2": run_inverse("c2", max(3, args.steps // 2), 1, torch, G, W, scale=0.1)}
+        print(json.dumps(r))
         return
     fixed = None
     if args.config:
@@ -211,6 +240,12 @@ def main():
         log(f"workload c4: {extra['c4']['fwd_adjoint_ev_s']:.4g} ev/s fwd+adjoint, "
             f"{extra['c4']['fwd_adjoint_over_forward']:.2f}x forward")
         torch.cuda.empty_cache()
+        for name, scale in (("c1", 1.0), ("c2", 0.1)):
+            log(f"inverse mapping {name}: setup")
+            extra[f"inverse_{name}"] = run_inverse(name, max(3, args.steps // 4), 1, torch, G, W, scale=scale)
+            r = extra[f"inverse_{name}"]
+            log(f"inverse mapping {name}: {r['pairs_s']:.4g} (event, channel) pairs/s, {r['hbm_frac']:.3f} of HBM")
+            torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
