@@ -230,6 +230,7 @@ def main():
             r, p2, s2, _ = run_workload(name, max(3, args.steps // 2), 1, torch, G, W)
             extra[name] = {k: r[k] for k in ("n_events", "batched_ev_s", "plain_ev_s", "batched_vs_plain",
                                               "batched_kernel_ms", "config")}
+            extra[name]["tune_summary"] = r["tune"].split(" c0:")[0].split(" c1:")[0]
             extra[name]["hbm_frac"] = r["algorithmic_bytes"] / (r["batched_kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
             log(f"workload {name}: {r['batched_ev_s']:.4g} ev/s batched, {r['plain_ev_s']:.4g} plain")
             del p2, s2
@@ -267,7 +268,8 @@ def main():
         "data": "synthetic: seeded U(0,1) FP64 randoms generated on the device",
         "config": {"workload": f"{args.workload}: {W.DESCRIPTIONS[args.workload]}",
                    "n_events": main_res["n_events"], "channels": main_res["n_segments"],
-                   "launch": "one batched launch replayed from a hipGraph", "tuned": main_res["config"]},
+                   "launch": "one batched launch replayed from a hipGraph", "tuned": main_res["config"],
+                   "tune_summary": main_res["tune"].split(" c0:")[0].split(" c1:")[0]},
         "plain_ev_s": main_res["plain_ev_s"],
         "batched_vs_plain": main_res["batched_vs_plain"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -249,7 +249,7 @@ hipError_t launch_class(PlanObj& pl, int c, hipStream_t stream) {
 // the side stream between a fork and a join event, so the two kernels can share the GPU.
 hipError_t launch_classes(PlanObj& pl, hipStream_t stream) {
   const bool two = pl.class_end[0] > pl.class_begin[0] && pl.class_end[1] > pl.class_begin[1];
-  if (!two) {
+  if (!two || pl.serial_classes) {
     for (int c = 0; c < kTileClasses; c++) {
       hipError_t e = launch_class(pl, c, stream);
       if (e != hipSuccess) return e;
@@ -565,22 +565,27 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
       rc = plan_build(*pl);
     }
   }
-  // tile order, for the whole plan
+  // tile order, and whether two classes share the GPU or run one after the other, for the
+  // whole plan
   float best_order_ms = 1e30f;
-  int best_order = pl->order;
-  for (int order : {0, 1}) {
-    if (rc) break;
-    pl->order = order;
-    drop_graph(*pl);
-    rc = plan_build(*pl);
-    float ms = 0;
-    if (!rc) rc = time_now(-1, &ms);
-    if (!rc && ms < 0.98f * best_order_ms) { best_order_ms = ms; best_order = order; }
-  }
+  int best_order = pl->order, best_serial = pl->serial_classes;
+  const bool two = pl->class_end[0] > pl->class_begin[0] && pl->class_end[1] > pl->class_begin[1];
+  for (int order : {0, 1})
+    for (int serial : {0, 1}) {
+      if (rc || (serial && !two)) continue;
+      pl->order = order;
+      pl->serial_classes = serial;
+      drop_graph(*pl);
+      rc = plan_build(*pl);
+      float ms = 0;
+      if (!rc) rc = time_now(-1, &ms);
+      if (!rc && ms < 0.98f * best_order_ms) { best_order_ms = ms; best_order = order; best_serial = serial; }
+    }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;
   pl->order = best_order;
+  pl->serial_classes = best_serial;
   drop_graph(*pl);
   rc = plan_build(*pl);
   if (rc) return rc;
@@ -594,7 +599,7 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
                lc.waves_per_eu, lc.lds);
       s += buf;
     }
-    snprintf(buf, sizeof buf, "order=%d %.4f ms;", best_order, best_order_ms);
+    snprintf(buf, sizeof buf, "order=%d serial=%d %.4f ms;", best_order, best_serial, best_order_ms);
     s += buf;
     for (const auto& c : tried) {
       snprintf(buf, sizeof buf, " c%d:%d/%d/%d/%d/%d:%.4f", c.cls, c.lc.block, c.lc.ept, c.lc.grid, c.lc.waves_per_eu,

@@ -45,6 +45,7 @@ struct PlanObj {
   int class_begin[kTileClasses] = {0, 0}, class_end[kTileClasses] = {0, 0};
   std::vector<int> chan_class;   // class of each plan channel
   int order = 0;
+  int serial_classes = 0;                   // 1: the two classes run one after the other on one stream
   void* dev = nullptr;                      // one allocation: tiles | segs | chans | nodes
   size_t dev_bytes = 0;
   DevicePlan dp;
