@@ -19,14 +19,17 @@
 //     f_ct = (d ct/dx)/2 by finite differences, momentum conservation, on-shell
 //     masses;
 //   * the dual-number path against central finite differences of the double path.
-//   Parts with no reference counterpart and therefore pinned by invariants only:
-//   the per-event sqrt(s) (reduces to the fixed-sqrt(s) path when E1 = E2), the
-//   lab-frame boost (phs.cu:1286-1301; identity for E1 = E2, Lorentz invariants and
-//   total momentum otherwise), and the friend axis (phs.cu:853-910; GPU-only in the
-//   reference, whose rotation matrix there is not orthogonal): parity unpinned, checked
-//   by momentum conservation, on-shell masses, the polar angle of the first daughter
-//   measured from the beam in the branch's rest frame, and the f = +1 root reducing to
-//   the plain composition.
+//   Parts with no counterpart in phs_cpu.cpp:
+//   * the per-event sqrt(s) (reduces to the fixed-sqrt(s) path when E1 = E2) and the
+//     lab-frame boost (phs.cu:1286-1301; identity for E1 = E2, Lorentz invariants and
+//     total momentum otherwise), pinned by those invariants;
+//   * the friend axis (phs.cu:853-910; GPU-only in the reference, whose rotation matrix
+//     there is not orthogonal): Whizard's momenta and all-channel factors of the friend
+//     trees of samples/bwbw.ref, the polar angle of the first daughter measured from the
+//     beam in the branch's rest frame, conservation, on-shell masses, and the f = +1 root
+//     reducing to the plain composition;
+//   * the inverse mapping (phs.cu:626-675, 924-1284): the round trip into the generating
+//     channel and Whizard's all-channel factors of samples/ttH.ref and samples/bwbw.ref.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
