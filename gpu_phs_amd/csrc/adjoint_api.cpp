@@ -19,9 +19,14 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
     max_banks = std::max(max_banks, (int)c.n_banks);
   }
   AdjLayout lay;
+  // only the root defers (and the randoms fit the register row): the compact stash; it keeps
+  // eight one-wavefront workgroups of the 2->4 trees within a CU's LDS
+  int max_nx_all = 0;
+  for (const auto& c : pl.chans) max_nx_all = std::max(max_nx_all, (int)c.n_x);
+  lay.compact = pl.dp.root_banks_only && max_nx_all <= 8 ? 1 : 0;
   lay.off_in = 6 * max_nodes;
   lay.off_st = lay.off_in + 2 * max_nodes;
-  lay.off_acc = lay.off_st + 19 * max_banks;
+  lay.off_acc = lay.off_st + (lay.compact ? kAdjCompactStashWords : kAdjStashWords) * max_banks;
   // programs of up to 4 final-state particles keep their accumulators in registers
   int max_nx = 0;
   for (const auto& c : pl.chans) max_nx = std::max(max_nx, (int)c.n_x);
@@ -46,9 +51,10 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   pl.adj_lay = lay;
   pl.adj_part_stride = 2 * max_nodes;
   pl.adj_segs = std::move(segs);
-  // one wavefront per 64 events of a tile
+  // one wavefront per 64 events of a tile (tile sizes of the classes that have tiles)
   int tile_max = 1;
-  for (int c = 0; c < kTileClasses; c++) tile_max = std::max(tile_max, pl.lcs[c].block * pl.lcs[c].ept);
+  for (int c = 0; c < kTileClasses; c++)
+    if (pl.class_end[c] > pl.class_begin[c]) tile_max = std::max(tile_max, pl.lcs[c].block * pl.lcs[c].ept);
   pl.adj_sub = (tile_max + 63) / 64;
   // tiles of every channel, in tile order
   std::vector<int> off(n_ch + 1, 0), list(n_t);
@@ -134,6 +140,8 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
       return set_error(GPU_PHS_E_ARG, "adjoint: the built-in loss assumes symmetric beams");
   int rc = adjoint_prepare(*pl, adj, write_forward);
   if (rc) return rc;
+  // one workgroup per 64-event unit: a persistent grid measured slower (its workgroups do not
+  // all fit the LDS at once), and units without events would cost a workgroup each
   hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, loss_mode, pl->adj_part, pl->adj_part_stride, pl->adj_sub,
                                 pl->adj_ranges, pl->adj_chan_tiles, pl->adj_chan_off, (int)pl->chans.size(), gparam,
                                 0, (hipStream_t)stream);

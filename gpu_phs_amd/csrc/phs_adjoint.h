@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "phs_kernels.h"
 #include "phs_math.h"
 #include "phs_types.h"
 
@@ -184,14 +185,14 @@ GPHS_HD void ct_eval_adj(int kind, double x, double b0, double b1, const CtEval&
   if (kind == CK_FLAT) {
     // ct = 2x - 1, st = sqrt(tmp (2 - tmp)), tmp = 2 (1 - x)
     gx = 2 * gct;
-    if (r.st > 0) gx += gst * (-2 * (1 - r.tmp) / r.st);
+    if (r.st > 0) gx += gst * math::quot(-2 * (1 - r.tmp), r.st);
     return;
   }
   if (!r.in) return;
   const double sg = r.lo ? 1.0 : -1.0;     // d ct / d tmp
   const double u = r.lo ? x : 1 - x;
   double gc = gct;
-  if (r.st > 0) gc += gst * (-r.ct / r.st);
+  if (r.st > 0) gc += gst * math::quot(-r.ct, r.st);
   const double gtmp = gc * sg + gf * b1;
   gb0 = gc * (-sg) + gtmp * r.e;            // direct -b0 (lo) / +b0 (hi) in ct, tmp = b0 e
   gb1 = gf * r.tmp + gtmp * r.tmp * 2 * u;
@@ -217,7 +218,18 @@ GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double 
 // Where the per-event adjoint keeps its tape and stash.  `Store` provides
 //   double& T(int node, int k)   L1 of a branch: gamma, bg, ct, st, cp, sp (k = 0..5)
 //   double& in(int node, int k)  incoming msq and p_decay of a branch (k = 0, 1)
-//   double& st(int level, int k) stash level: 16 matrix adjoints + 3 scalar adjoints
+//   st_put(level, k, v), st_get(level, k): stash level, 16 matrix adjoints (k = 0..15) and 3
+//     scalar adjoints (k = 16..18).  A store may drop the matrix entries 2, 4, 8 and 14 (st_get
+//     then returns 0) when only the root defers: the stash then only ever meets the root,
+//     whose parent boost is the identity, and the adjoint of its L1 reads only the entries at
+//     the structural non-zeros of L1.
+
+// Stash slot of entry k in the compact layout (-1: not kept).
+GPHS_HD constexpr int compact_stash_slot(int k) {
+  return (k == 2 || k == 4 || k == 8 || k == 14) ? -1
+         : k < 2 ? k : k < 4 ? k - 1 : k < 8 ? k - 2 : k < 14 ? k - 3 : k - 4;
+}
+
 
 // P = product of the L1 of the ancestors of branch j (root first): the boost its own L1 is
 // composed onto.  The ancestors are found by walking the parent links (a few scalar loads).
@@ -225,9 +237,22 @@ template <class Store>
 GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16]) {
   int d = 0;
   for (int a = prog[j].parent; a >= 0; a = prog[a].parent) d++;
+  if (d == 0) {
 #pragma unroll
-  for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
-  for (int level = 0; level < d; level++) {
+    for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    return;
+  }
+  // the root's L1 itself (the identity times L1: the same values), then the others
+  int r = prog[j].parent;
+  for (int u = 0; u < d - 1; u++) r = prog[r].parent;
+  {
+    const double g = S.T(r, 0), bg = S.T(r, 1), ct = S.T(r, 2), st = S.T(r, 3), cp = S.T(r, 4), sp = S.T(r, 5);
+    P[0] = g;   P[1] = -bg * st; P[2] = 0;   P[3] = bg * ct;
+    P[4] = 0;   P[5] = ct * cp;  P[6] = -sp; P[7] = st * cp;
+    P[8] = 0;   P[9] = ct * sp;  P[10] = cp; P[11] = st * sp;
+    P[12] = bg; P[13] = -g * st; P[14] = 0;  P[15] = g * ct;
+  }
+  for (int level = 1; level < d; level++) {
     int a = prog[j].parent;
     for (int u = 0; u < d - 1 - level; u++) a = prog[a].parent;
     compose_l1(P, S.T(a, 0), S.T(a, 1), S.T(a, 2), S.T(a, 3), S.T(a, 4), S.T(a, 5));
@@ -264,9 +289,11 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     const double m = cur_m;
     const double lda = (cur_msq - msq[0] - msq[1]) * (cur_msq - msq[0] - msq[1]) - 4 * msq[0] * msq[1];
     ok = ok && (lda > 0 && m > mk[0] + mk[1] && m <= nc.m_max);
-    const double sl = math::sqrt_phys(lda), pk = sl / (2 * m);
-    F *= f[0] * f[1] * (sl / cur_msq);
-    const double bg = m > 0 ? cur_p / m : 0.0, g = math::sqrt_phys(1 + bg * bg);
+    // quotients to a few ulps (math::quot): the outputs agree with the forward kernel's to
+    // ~1e-15 relative, far inside the parity bound
+    const double sl = math::sqrt_phys(lda), pk = math::quot(sl, 2 * m);
+    F *= f[0] * f[1] * math::quot(sl, cur_msq);
+    const double bg = m > 0 ? math::quot(cur_p, m) : 0.0, g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
     math::sincos_turns(xr[nc.x_phi], sp, cp);
     const CtEval ce = ct_eval(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1);
@@ -328,9 +355,9 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     if (nc.push >= 0) {
       depth--;
 #pragma unroll
-      for (int q = 0; q < 16; q++) G[q] += S.st(depth, q);
-      g_msq[nc.push] += S.st(depth, 16); g_m[nc.push] += S.st(depth, 17);
-      g_pk += nc.push == 0 ? S.st(depth, 18) : -S.st(depth, 18);
+      for (int q = 0; q < 16; q++) G[q] += S.st_get(depth, q);
+      g_msq[nc.push] += S.st_get(depth, 16); g_m[nc.push] += S.st_get(depth, 17);
+      g_pk += nc.push == 0 ? S.st_get(depth, 18) : -S.st_get(depth, 18);
     }
     // the branch's local quantities: boost parameters from the tape, maps recomputed
     const double a = S.in(i, 0), p0 = S.in(i, 1);
@@ -346,14 +373,32 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     }
     const double d = a - msq[0] - msq[1];
     const double lda = d * d - 4 * msq[0] * msq[1];
-    const double sl = math::sqrt_phys(lda), pk = sl / (2 * m);
+    const double sl = math::sqrt_phys(lda), pk = math::quot(sl, 2 * m);
     const double g = S.T(i, 0), bg = S.T(i, 1), cp = S.T(i, 4), sp = S.T(i, 5);
     const double xct = xr[nc.x_phi + 1];
-    const CtEval ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
+    CtEval ce;
+    if (nc.ct_kind == CK_FLAT) {   // from the tape: sin theta as the forward sweep formed it
+      ce.tmp = 2 * (1 - xct);
+      ce.ct = 1 - ce.tmp;
+      ce.st = S.T(i, 3);
+      ce.f = 1;
+      ce.e = 0;
+      ce.in = true;
+      ce.lo = false;
+    } else {
+      ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
+    }
     double Ma[16];
+    if (i == 0) {   // the root's parent boost is the identity: Ma = L1
+      Ma[0] = g;   Ma[1] = -bg * ce.st; Ma[2] = 0;   Ma[3] = bg * ce.ct;
+      Ma[4] = 0;   Ma[5] = ce.ct * cp;  Ma[6] = -sp; Ma[7] = ce.st * cp;
+      Ma[8] = 0;   Ma[9] = ce.ct * sp;  Ma[10] = cp; Ma[11] = ce.st * sp;
+      Ma[12] = bg; Ma[13] = -g * ce.st; Ma[14] = 0;  Ma[15] = g * ce.ct;
+    } else {
 #pragma unroll
-    for (int q = 0; q < 16; q++) Ma[q] = P[q];
-    compose_l1(Ma, g, bg, ce.ct, ce.st, cp, sp);
+      for (int q = 0; q < 16; q++) Ma[q] = P[q];
+      compose_l1(Ma, g, bg, ce.ct, ce.st, cp, sp);
+    }
 
     // final-state daughters: q = Ma[:,0] E + Ma[:,3] pc
 #pragma unroll
@@ -385,22 +430,28 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
         gE += gq[mu] * Ma[4 * mu + 0];
         gpc += gq[mu] * Ma[4 * mu + 3];
       }
-      if (E > 0) gpc += gE * pc / E;
+      if (E > 0) gpc += math::quot(gE * pc, E);
       g_pk += c == 0 ? gpc : -gpc;
     }
     // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros of L1 only)
     const double l01 = -bg * ce.st, l03 = bg * ce.ct, l11 = ce.ct * cp, l12 = -sp, l13 = ce.st * cp;
     const double l21 = ce.ct * sp, l22 = cp, l23 = ce.st * sp, l31 = -g * ce.st, l33 = g * ce.ct;
     double gP[16];
+    if (i > 0) {   // the root has no parent to hand gP to
 #pragma unroll
-    for (int ii = 0; ii < 4; ii++) {
-      const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
-      gP[4 * ii + 0] = G0 * g + G1 * l01 + G3 * l03;
-      gP[4 * ii + 1] = G1 * l11 + G2 * l12 + G3 * l13;
-      gP[4 * ii + 2] = G1 * l21 + G2 * l22 + G3 * l23;
-      gP[4 * ii + 3] = G0 * bg + G1 * l31 + G3 * l33;
+      for (int ii = 0; ii < 4; ii++) {
+        const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
+        gP[4 * ii + 0] = G0 * g + G1 * l01 + G3 * l03;
+        gP[4 * ii + 1] = G1 * l11 + G2 * l12 + G3 * l13;
+        gP[4 * ii + 2] = G1 * l21 + G2 * l22 + G3 * l23;
+        gP[4 * ii + 3] = G0 * bg + G1 * l31 + G3 * l33;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; q++) gP[q] = 0;
     }
     auto gl = [&](int k, int j) {
+      if (i == 0) return G[4 * k + j];   // P = identity
       double acc = 0;
 #pragma unroll
       for (int ii = 0; ii < 4; ii++) acc += P[4 * ii + k] * G[4 * ii + j];
@@ -417,29 +468,30 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     // phi = 2 pi x
     gx[nc.x_phi] = (gsp * cp - gcp * sp) * kTwoPi;
     // cos theta map of this branch (its own M through b0, b1)
-    const double gfct = ce.f != 0 ? W / ce.f : 0.0;
+    const double gfct = ce.f != 0 ? math::quot(W, ce.f) : 0.0;
     double gxc, gb0, gb1;
     ct_eval_adj(nc.ct_kind, xct, nc.b0, nc.b1, ce, gct, gst, gfct, gxc, gb0, gb1);
     gx[nc.x_phi + 1] = gxc;
     gpar[2 * i] += gb0 * nc.db_dm[0] + gb1 * nc.db_dm[1];
     // gamma = sqrt(1 + bg^2), bg = p / m
-    gbg += gg * bg / g;
+    gbg += math::quot(gg * bg, g);
     double g_a = 0, g_mnode = 0, g_p0 = 0;
-    if (m > 0) { g_p0 += gbg / m; g_mnode -= gbg * p0 / (m * m); }
+    const double inv_m = m > 0 ? math::quot(1.0, m) : 0.0;
+    if (m > 0) { g_p0 += gbg * inv_m; g_mnode -= gbg * p0 * inv_m * inv_m; }
     // pk = sl / (2 m)
-    double gsl = g_pk / (2 * m);
-    g_mnode -= g_pk * sl / (2 * m * m);
+    double gsl = g_pk * inv_m * 0.5;
+    g_mnode -= g_pk * sl * inv_m * inv_m * 0.5;
     // factor term f0 f1 sl / a
     double gfk[2] = {0, 0};
     if (W != 0) {
 #pragma unroll
       for (int c = 0; c < 2; c++)
-        if (f[c] != 0) gfk[c] = W / f[c];
-      if (sl != 0) gsl += W / sl;
-      g_a -= W / a;
+        if (f[c] != 0) gfk[c] = math::quot(W, f[c]);
+      if (sl != 0) gsl += math::quot(W, sl);
+      g_a -= math::quot(W, a);
     }
     // lambda = d^2 - 4 msq0 msq1, d = a - msq0 - msq1
-    const double glda = sl > 0 ? gsl / (2 * sl) : 0.0;
+    const double glda = sl > 0 ? math::quot(gsl, 2 * sl) : 0.0;
     const double gd = 2 * d * glda;
     g_msq[0] += -4 * msq[1] * glda - gd;
     g_msq[1] += -4 * msq[0] * glda - gd;
@@ -448,7 +500,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       if (nc.k[c].leaf) continue;
-      if (mk[c] > 0) g_msq[c] += g_m[c] / (2 * mk[c]);
+      if (mk[c] > 0) g_msq[c] += math::quot(g_m[c], 2 * mk[c]);
       double gxk = 0, gM = 0, gW = 0;
       msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
       gx[nc.k[c].xi] = gxk;
@@ -461,8 +513,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     gc_msq = g_a; gc_m = g_mnode; gc_p = g_p0;
     if (i >= 1 && prog[i - 1].desc < 0) {   // reached by a pop in the forward pass
 #pragma unroll
-      for (int q = 0; q < 16; q++) S.st(depth, q) = Gc[q];
-      S.st(depth, 16) = gc_msq; S.st(depth, 17) = gc_m; S.st(depth, 18) = gc_p;
+      for (int q = 0; q < 16; q++) S.st_put(depth, q, Gc[q]);
+      S.st_put(depth, 16, gc_msq); S.st_put(depth, 17, gc_m); S.st_put(depth, 18, gc_p);
       depth++;
     }
   }

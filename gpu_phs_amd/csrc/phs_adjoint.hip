@@ -13,12 +13,29 @@ namespace gphs {
 
 namespace {
 
+// COMPACT: the stash keeps 15 of its 19 words (only the root defers; phs_adjoint.h).
+template <bool COMPACT>
 struct LdsStore {
   double* base;
   int stride, off_in, off_st;
   __device__ double& T(int node, int k) { return base[(node * 6 + k) * stride]; }
   __device__ double& in(int node, int k) { return base[(off_in + node * 2 + k) * stride]; }
-  __device__ double& st(int level, int k) { return base[(off_st + level * 19 + k) * stride]; }
+  __device__ __forceinline__ void st_put(int level, int k, double v) {
+    if constexpr (COMPACT) {
+      const int c = compact_stash_slot(k);
+      if (c >= 0) base[(off_st + level * kAdjCompactStashWords + c) * stride] = v;
+    } else {
+      base[(off_st + level * kAdjStashWords + k) * stride] = v;
+    }
+  }
+  __device__ __forceinline__ double st_get(int level, int k) const {
+    if constexpr (COMPACT) {
+      const int c = compact_stash_slot(k);
+      return c >= 0 ? base[(off_st + level * kAdjCompactStashWords + c) * stride] : 0.0;
+    } else {
+      return base[(off_st + level * kAdjStashWords + k) * stride];
+    }
+  }
 };
 
 struct LdsAcc {
@@ -37,7 +54,7 @@ constexpr int kAdjXReg = 8;             // n_x <= 8 <=> n_out <= 4
 constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_out = 4 tree
 
 // XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
-template <int XN>
+template <int XN, bool COMPACT>
 __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict__ tiles, int n_units, int n_sub,
                                                      const SegDesc* __restrict__ segs,
                                                      const ChannelDesc* __restrict__ chans,
@@ -45,7 +62,7 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
                                                      double* __restrict__ gpart, int part_stride, AdjLayout lay) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
-  LdsStore S{lds + lane, 64, lay.off_in, lay.off_st};
+  LdsStore<COMPACT> S{lds + lane, 64, lay.off_in, lay.off_st};
   // the (M, Gamma) accumulators: in registers for the small programs (XN > 0: n_out <= 4, at
   // most kSmallParams), else in LDS after the tape
   double accv[XN > 0 ? kSmallParams : 1];
@@ -171,7 +188,8 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_m
   if (units > 0x7fffffff) return hipErrorInvalidValue;
   if (units > 0) {
     const int g = grid > 0 && grid < units ? grid : (int)units;
-    auto k = dp.nx_max <= kAdjXReg ? adjoint_kernel<kAdjXReg> : adjoint_kernel<0>;
+    auto k = dp.nx_max > kAdjXReg ? adjoint_kernel<0, false>
+             : lay.compact ? adjoint_kernel<kAdjXReg, true> : adjoint_kernel<kAdjXReg, false>;
     hipLaunchKernelGGL(k, dim3(g), dim3(64), shmem, stream, dp.tiles, (int)units, n_sub, dp.segs, dp.chans, dp.nodes,
                        loss_mode, gpart, part_stride, lay);
     hipError_t e = hipGetLastError();

@@ -43,7 +43,9 @@ hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const La
 // accumulators [off_acc, words).
 struct AdjLayout {
   int off_in, off_st, off_acc, words;
+  int compact;   // 1: kAdjCompactStashWords per stash level (only the root defers), else kAdjStashWords
 };
+constexpr int kAdjStashWords = 19, kAdjCompactStashWords = 15;
 
 // Adjoint over all tiles, one wavefront per 64-event sub-tile (n_sub per tile), then the
 // per-channel reduction of the parameter partials (through granges, kAdjRanges * part_stride
