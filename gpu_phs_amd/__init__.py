@@ -66,6 +66,7 @@ _SIGS = {
     "gpu_phs_plan_param_map": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int]),
     "gpu_phs_plan_inverse": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "gpu_phs_plan_set_event_beams": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
@@ -208,6 +209,23 @@ class Plan:
         v = [ctypes.c_int() for _ in range(3)]
         _check(lib().gpu_phs_plan_info(self.handle, *[ctypes.byref(q) for q in v]), "plan_info")
         return dict(n_tiles=v[0].value, n_segments=v[1].value, n_banks=v[2].value)
+
+    def set_event_beams(self, segment, E1=None, E2=None):
+        """Per-event beam energies (cuda float64 tensors [n_events] of the segment), or None for
+        the process's fixed beams.  The tensors must stay alive while the plan is used."""
+        if (E1 is None) != (E2 is None):
+            raise ValueError("give both E1 and E2, or neither")
+        if E1 is not None:
+            n = self._keep[segment][1].x.shape[0]
+            for t in (E1, E2):
+                if t.dtype != self._keep[segment][1].x.dtype or not t.is_contiguous() or t.shape != (n,):
+                    raise ValueError("beam energies must be contiguous float64 tensors of the segment's length")
+            self._beams = getattr(self, "_beams", {})
+            self._beams[segment] = (E1, E2)
+        _check(lib().gpu_phs_plan_set_event_beams(self.handle, segment,
+                                                  None if E1 is None else ctypes.c_void_p(E1.data_ptr()),
+                                                  None if E2 is None else ctypes.c_void_p(E2.data_ptr())),
+               "plan_set_event_beams")
 
     def run(self, stream=None):
         _check(lib().gpu_phs_plan_run(self.handle, _stream_ptr(stream)), "plan_run")

@@ -13,6 +13,8 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   const int n_seg = (int)pl.segs.size(), n_ch = (int)pl.chans.size(), n_t = pl.dp.n_tiles;
   if (pl.dp.has_friends)
     return set_error(GPU_PHS_E_TREE, "adjoint: branches with a friend (beam polar axis) have no adjoint");
+  if (pl.dp.per_event_beams)
+    return set_error(GPU_PHS_E_ARG, "adjoint: segments with per-event beam energies have no adjoint");
   int max_nodes = 0, max_banks = 0;
   for (const auto& c : pl.chans) {
     max_nodes = std::max(max_nodes, (int)c.n_nodes);

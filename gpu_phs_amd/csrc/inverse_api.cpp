@@ -27,7 +27,8 @@ int inverse_prepare(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows) {
     is[s].pad = 0;
     is[s].row0 = r0;
   }
-  if (pl.inv_dev && pl.inv_process == proc && is.size() == pl.inv_segs.size() &&
+  if (pl.inv_dev && pl.inv_process == proc && pl.inv.per_event_beams == pl.dp.per_event_beams &&
+      is.size() == pl.inv_segs.size() &&
       std::memcmp(is.data(), pl.inv_segs.data(), is.size() * sizeof(InvSeg)) == 0)
     return ok_status();
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -57,6 +58,7 @@ int inverse_prepare(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows) {
   pl.inv.n_x = cp.spec.n_x();
   pl.inv.n_banks = cp.max_banks;
   pl.inv.has_friends = cp.has_friends ? 1 : 0;
+  pl.inv.per_event_beams = pl.dp.per_event_beams;
   pl.inv_segs = std::move(is);
   pl.inv_process = proc;
   return ok_status();

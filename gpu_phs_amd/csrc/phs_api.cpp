@@ -122,6 +122,9 @@ int plan_build(PlanObj& pl) {
     sd.x = us.x; sd.p = us.p; sd.factor = us.factors; sd.volume = us.volumes; sd.ok = us.oks;
     sd.align16 = ((reinterpret_cast<uintptr_t>(us.x) % 16) == 0 ? 1 : 0) |
                  ((reinterpret_cast<uintptr_t>(us.p) % 16) == 0 ? 2 : 0);
+    const size_t k = pl.segs.size();
+    sd.e1 = k < pl.beam_e1.size() ? pl.beam_e1[k] : nullptr;
+    sd.e2 = k < pl.beam_e2.size() ? pl.beam_e2[k] : nullptr;
     pl.segs.push_back(sd);
   }
   if (pl.n_banks > 3) return set_error(GPU_PHS_E_TREE, "tree needs more than 3 save banks");
@@ -218,7 +221,7 @@ int plan_build(PlanObj& pl) {
   pl.dp.n_tiles = (int)pl.tiles.size();
   // maxima over all channels (adjoint) and over each class (forward launches)
   auto maxima = [&](DevicePlan& d, int cl) {
-    d.n_banks = 0; d.nx_max = d.nout_max = 0; d.root_banks_only = 1; d.has_friends = 0;
+    d.n_banks = 0; d.nx_max = d.nout_max = 0; d.root_banks_only = 1; d.has_friends = 0; d.per_event_beams = 0;
     for (size_t c = 0; c < pl.chans.size(); c++) {
       if (cl >= 0 && pl.chan_class[c] != cl) continue;
       const ChannelDesc& cd = pl.chans[c];
@@ -238,6 +241,8 @@ int plan_build(PlanObj& pl) {
     pl.dps[c] = pl.dp;
     maxima(pl.dps[c], c);
   }
+  for (const SegDesc& sd : pl.segs)
+    if (sd.e1) pl.dp.per_event_beams = pl.dps[pl.chan_class[sd.channel]].per_event_beams = 1;
   return ok_status();
 }
 
@@ -374,6 +379,22 @@ int gpu_phs_plan_create(const gpu_phs_segment* segments, int n_segments) {
   int h = register_plan(std::move(pl));
   ok_status();
   return h;
+}
+
+int gpu_phs_plan_set_event_beams(int plan, int segment, const double* E1, const double* E2) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (segment < 0 || segment >= (int)pl->user.size()) return set_error(GPU_PHS_E_ARG, "set_event_beams: no such segment");
+  if (!E1 != !E2) return set_error(GPU_PHS_E_ARG, "set_event_beams: give both E1 and E2, or neither");
+  pl->beam_e1.resize(pl->user.size(), nullptr);
+  pl->beam_e2.resize(pl->user.size(), nullptr);
+  pl->beam_e1[segment] = E1;
+  pl->beam_e2[segment] = E2;
+  drop_graph(*pl);
+  pl->adj_valid = false;
+  pl->inv_segs.clear();
+  return plan_build(*pl);
 }
 
 int gpu_phs_plan_destroy(int plan) {

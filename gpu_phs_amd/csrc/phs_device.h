@@ -69,6 +69,93 @@ __device__ __forceinline__ void msq_map(const ChildCmd& c, double x, double s, d
   }
 }
 
+// The beams of one event: the channel's fixed ones, or the event's own energies E1, E2
+// (phs.cu:1313-1317: sqrt(s) = 2 sqrt(E1 E2); phs.cu:1289: bg_lab = (E1 - E2) / (2 sqrt(E1 E2));
+// phs.cu:879: a branch with a friend evaluates its cos-theta map at s = 4 E1 E2).
+struct EvBeams {
+  double sqrts, s, bg_lab, g_lab, s_friend, volume;
+};
+
+__device__ __forceinline__ EvBeams fixed_beams(const ChannelDesc& cd) {
+  return EvBeams{cd.sqrts, cd.s, cd.bg_lab, cd.g_lab, cd.s, cd.volume};
+}
+
+#define GPHS_TWOPI2 39.47841760435743447534   // mappings.h:6
+#define GPHS_TWOPI5 9792.62991312900650440772 // mappings.h:7
+
+__device__ __forceinline__ EvBeams event_beams(const ChannelDesc& cd, double E1, double E2) {
+  EvBeams b;
+  const double r = math::sqrt_phys(E1 * E2);
+  b.sqrts = 2 * r;
+  b.s = b.sqrts * b.sqrts;
+  b.bg_lab = (E1 - E2) / (2 * r);
+  b.g_lab = math::sqrt_phys(1 + b.bg_lab * b.bg_lab);
+  b.s_friend = 4 * E1 * E2;
+  // the phase-space volume (phs_cpu.cpp:195-212): s / (4 (2 pi)^2) per internal branch below
+  // the root, 1 / (4 (2 pi)^5) for the root
+  double v = 1;
+  for (int i = 1; i < cd.n_nodes; i++) v = v * b.sqrts * b.sqrts / (4 * GPHS_TWOPI2);
+  b.volume = v / (4 * GPHS_TWOPI5);
+  return b;
+}
+
+// An internal daughter's map at this event's sqrt(s): the constants that depend on it,
+// recomputed with the expressions of init_mapping_constants_cpu (phs_cpu.cpp:452-511);
+// a0 of the Breit-Wigner, exponential and t/u maps does not depend on sqrt(s).
+__device__ __forceinline__ ChildCmd child_at(const ChildCmd& c, double sqrts, double s, double m_tot) {
+  ChildCmd r = c;
+  if (c.leaf) return r;
+  const double m_max = sqrts - m_tot + c.m_min;
+  const double msq_min = c.msq_min, msq_max = m_max * m_max, d = msq_max - msq_min;
+  r.msq_max = msq_max;
+  r.f_lin = d / s;
+  switch (c.kind) {
+    case MK_LINEAR: r.a1 = d; r.a2 = r.a1 / s; break;
+    case MK_BREIT_WIGNER: {
+      const double msq0 = c.m * c.m;
+      r.a1 = atan((msq_max - msq0) / (c.m * c.w));
+      r.a2 = (r.a1 - c.a0) * (c.m * c.w) / s;
+      break;
+    }
+    case MK_EXP: r.a1 = math::log_bounded(d / c.a0 + 1); r.a2 = r.a1 / s; break;
+    case MK_TU: r.a1 = 2 * math::log_bounded(d / (2 * c.a0) + 1); r.a2 = r.a1 / s; break;
+    case MK_STEP_E: {
+      double a2 = 2 * c.m * c.w / d;
+      a2 = a2 < 0.01 ? 0.01 : a2;
+      const double a1 = math::exp_bounded(-(c.m * c.m - msq_min) / d / a2);
+      r.a2 = a2;
+      r.a1 = a1;
+      r.a0 = 1 - a2 * math::log_bounded((1 + a1 * math::exp_bounded(1 / a2)) / (1 + a1));
+      break;
+    }
+    default: {  // MK_STEP_H
+      const double a2 = (c.m * c.m - msq_min) / d;
+      const double q = 2 * c.m * c.w / d;
+      double a1 = q * q / a2;
+      a1 = a1 < 0.000001 ? 0.000001 : a1;
+      r.a2 = a2;
+      r.a1 = a1;
+      r.a0 = (1 + math::sqrt_phys(1 + 4 * a1 / (1 - a2))) / 2;
+      break;
+    }
+  }
+  return r;
+}
+
+// A branch's kinematic bound and cos-theta constants at this event's beams
+// (phs_cpu.cpp:514-527; mappings_gpu.hpp:186-188 at s = 4 E1 E2 for a friend).
+__device__ __forceinline__ void node_at(const NodeCmd& nc, bool root, const EvBeams& b, double m_tot, double& m_max,
+                                        double& b0, double& b1) {
+  m_max = root ? b.sqrts : b.sqrts - m_tot + nc.m_min;
+  if (nc.ct_kind == CK_PEAKED) {
+    b0 = nc.m * nc.m / (nc.friend_sign != 0 ? b.s_friend : b.s);
+    b1 = math::log_bounded((b0 + 1) / b0);
+  } else {
+    b0 = nc.b0;
+    b1 = nc.b1;
+  }
+}
+
 // cos theta, sin theta and the mapping factor (phs_cpu.cpp:112-146).
 __device__ __forceinline__ void ct_map(int kind, double x, double b0, double b1, double& ct, double& st, double& f) {
   if (kind == CK_FLAT) {

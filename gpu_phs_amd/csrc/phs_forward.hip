@@ -120,12 +120,14 @@ constexpr int kXLds = -1;
 
 // One event of channel `cd` (program `prog`).  xg: the event's randoms; pl: this lane's column
 // of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
-template <int BANK, int XN, bool FR>
-__device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
+// eb: the event's beams (the channel's fixed ones unless PEV: then the map constants that
+// depend on sqrt(s) are recomputed per event as well).
+template <int BANK, int XN, bool FR, bool PEV>
+__device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBeams& eb, const NodeCmd* __restrict__ prog,
                                               const double* __restrict__ xg, double* __restrict__ pl,
                                               double* __restrict__ xl, int S, double* bank_base,
                                               double& factor_out, bool& ok_out) {
-  const double s = cd.s;
+  const double s = eb.s;
   const gdouble* __restrict__ xrow = as_global(xg);
   double xv[XN > 0 ? XN : 1];
   if constexpr (XN > 0) {
@@ -145,20 +147,26 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
   bool ok = true;
   int depth = 0;
 
-  double cur_m = cd.sqrts;  // sqrt(msq) of the branch being visited, carried from its parent
+  double cur_m = eb.sqrts;  // sqrt(msq) of the branch being visited, carried from its parent
   for (int i = 0; i < cd.n_nodes; ++i) {
     const NodeCmd& nc = prog[i];
+    double m_max = nc.m_max, cb0 = nc.b0, cb1 = nc.b1;
+    if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max, cb0, cb1);
     // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547) and masses
     double msq0, f0, m1, msq1, f1, m2;
     if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
-    else { msq_map(nc.k[0], GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = math::sqrt_phys(msq0); }
+    else if constexpr (PEV) {
+      msq_map(child_at(nc.k[0], eb.sqrts, s, cd.m_tot), GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = math::sqrt_phys(msq0);
+    } else { msq_map(nc.k[0], GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = math::sqrt_phys(msq0); }
     if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
-    else { msq_map(nc.k[1], GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = math::sqrt_phys(msq1); }
+    else if constexpr (PEV) {
+      msq_map(child_at(nc.k[1], eb.sqrts, s, cd.m_tot), GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = math::sqrt_phys(msq1);
+    } else { msq_map(nc.k[1], GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = math::sqrt_phys(msq1); }
 
     // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
     const double m = cur_m;
     const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
-    ok = ok && (lda > 0 && m > m1 + m2 && m <= nc.m_max);
+    ok = ok && (lda > 0 && m > m1 + m2 && m <= m_max);
     const double sl = math::sqrt_phys(lda);
     const double pk = sl / (2 * m);
     factor *= f0 * f1 * (sl / cur_msq);
@@ -169,7 +177,7 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
     double sp, cp;
     math::sincos_turns(GPHS_X(nc.x_phi), sp, cp);  // phi = 2 pi x
     double ct, st, fct;
-    ct_map(nc.ct_kind, GPHS_X(nc.x_phi + 1), nc.b0, nc.b1, ct, st, fct);
+    ct_map(nc.ct_kind, GPHS_X(nc.x_phi + 1), cb0, cb1, ct, st, fct);
     factor *= fct;
     // the root's parent boost is the identity (phs_cpu.cpp:568-573): M = L1; after a branch
     // with two final-state daughters the walk pops a saved matrix, so only the columns the
@@ -191,9 +199,9 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const NodeC
         const double q1 = M[4] * E + M[7] * pc;
         const double q2 = M[8] * E + M[11] * pc;
         double q3 = M[12] * E + M[15] * pc;
-        if (cd.bg_lab != 0) {
-          const double b0 = cd.g_lab * q0 + cd.bg_lab * q3;
-          const double b3 = cd.bg_lab * q0 + cd.g_lab * q3;
+        if (eb.bg_lab != 0) {
+          const double b0 = eb.g_lab * q0 + eb.bg_lab * q3;
+          const double b3 = eb.bg_lab * q0 + eb.g_lab * q3;
           q0 = b0; q3 = b3;
         }
         double* o = pl + 4 * nc.k[c].out * S;
@@ -252,7 +260,7 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
 // LDS per workgroup (S = blockDim.x + 1 doubles per row): momenta 4 nout_max rows, the
 // randoms nx_max rows (XN = kXLds), then the save banks (BANK = BK_LDS).  A tile of `count`
 // events is done in sub-tiles of blockDim.x events.
-template <int BANK, int WPE, int XN, bool FR>
+template <int BANK, int WPE, int XN, bool FR, bool PEV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
                     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
@@ -276,11 +284,14 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
       bool ok;
       __syncthreads();   // the previous sub-tile's momentum table has been read
       // lanes past the end repeat the last event and store nothing
-      forward_event<BANK, XN, FR>(cd, prog, sd.x + (e0 + (valid ? lane : cnt - 1)) * cd.n_x, ps + lane, xs + lane, S,
-                              bank_base, factor, ok);
+      const int64_t e = e0 + (valid ? lane : cnt - 1);
+      EvBeams eb;
+      if constexpr (PEV) eb = event_beams(cd, as_global(sd.e1)[e], as_global(sd.e2)[e]);
+      else eb = fixed_beams(cd);
+      forward_event<BANK, XN, FR, PEV>(cd, eb, prog, sd.x + e * cd.n_x, ps + lane, xs + lane, S, bank_base, factor, ok);
       if (valid) {
         as_global(sd.factor)[e0 + lane] = factor;
-        as_global(sd.volume)[e0 + lane] = cd.volume;
+        as_global(sd.volume)[e0 + lane] = eb.volume;
         as_global(sd.ok)[e0 + lane] = ok ? 1 : 0;
       }
       __syncthreads();
@@ -296,22 +307,25 @@ using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const Chan
 
 // Where a plan's randoms live under `lc`: kXLds, or the register row size.
 int x_mode(const DevicePlan& dp, const LaunchConfig& lc) {
-  if ((lc.lds & 2) || dp.nx_max > 16) return kXLds;
+  if ((lc.lds & 2) || dp.nx_max > 16 || dp.per_event_beams) return kXLds;
   return dp.nx_max <= 8 ? 8 : 16;
 }
 
 template <int BANK, int WPE, bool FR>
 FwdKernel pick_xn(int xm) {
-  if (xm == 8) return forward_kernel<BANK, WPE, 8, FR>;
-  if (xm == 16) return forward_kernel<BANK, WPE, 16, FR>;
-  return forward_kernel<BANK, WPE, kXLds, FR>;
+  if (xm == 8) return forward_kernel<BANK, WPE, 8, FR, false>;
+  if (xm == 16) return forward_kernel<BANK, WPE, 16, FR, false>;
+  return forward_kernel<BANK, WPE, kXLds, FR, false>;
 }
 
-// Plans with friend branches run the variant that has the friend-axis composition (at the
-// compiler-chosen occupancy); the others never carry its code.
+// Plans with friend branches run the variant that has the friend-axis composition, plans with
+// per-event beams the one that recomputes the beam-dependent constants (both at the
+// compiler-chosen occupancy, the latter with the randoms in LDS); the others carry neither.
 template <int BANK>
 FwdKernel pick_wpe(const DevicePlan& dp, const LaunchConfig& lc) {
   const int xm = x_mode(dp, lc);
+  if (dp.per_event_beams)
+    return dp.has_friends ? forward_kernel<BANK, 0, kXLds, true, true> : forward_kernel<BANK, 0, kXLds, false, true>;
   if (dp.has_friends) return pick_xn<BANK, 0, true>(xm);
   return lc.waves_per_eu == 3 ? pick_xn<BANK, 3, false>(xm) : pick_xn<BANK, 0, false>(xm);
 }

@@ -105,13 +105,13 @@ __device__ __forceinline__ void leaf_sum(const double* __restrict__ pl, int S, i
 
 // Inverse of one event into channel `cd` (program `prog`): the randoms go to xo[0..n_x),
 // returns the factor.
-template <bool FR>
-__device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
-                                                const double* __restrict__ pl, int S, double* bank, int B,
-                                                gdouble* __restrict__ xo) {
-  const double s = cd.s;
+template <bool FR, bool PEV>
+__device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvBeams& eb,
+                                                const NodeCmd* __restrict__ prog, const double* __restrict__ pl,
+                                                int S, double* bank, int B, gdouble* __restrict__ xo) {
+  const double s = eb.s;
   double G[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  double cur_msq = s, cur_m = cd.sqrts, cur_p = 0, factor = 1;
+  double cur_msq = s, cur_m = eb.sqrts, cur_p = 0, factor = 1;
   int depth = 0;
   for (int i = 0; i < cd.n_nodes; ++i) {
     const NodeCmd& nc = prog[i];
@@ -125,7 +125,8 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const Nod
     for (int d = 0; d < 2; d++) {
       if (!nc.k[d].leaf) {
         double x, f;
-        x_from_msq(nc.k[d], d == 0 ? msq0 : msq1, s, x, f);
+        if constexpr (PEV) x_from_msq(child_at(nc.k[d], eb.sqrts, s, cd.m_tot), d == 0 ? msq0 : msq1, s, x, f);
+        else x_from_msq(nc.k[d], d == 0 ? msq0 : msq1, s, x, f);
         xo[nc.k[d].xi] = x;
         factor *= f;
       }
@@ -179,8 +180,9 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const Nod
     const double cp = rho > 0 ? v0 / rho : 1.0, sp = rho > 0 ? v1 / rho : 0.0;
     const double ct = v2 / vv, st = rho / vv;
     xo[nc.x_phi] = phi / kTwoPi;
-    double xc, fct;
-    x_from_ct(nc.ct_kind, ct, nc.b0, nc.b1, xc, fct);
+    double xc, fct, m_max, cb0 = nc.b0, cb1 = nc.b1;
+    if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max, cb0, cb1);
+    x_from_ct(nc.ct_kind, ct, cb0, cb1, xc, fct);
     xo[nc.x_phi + 1] = xc;
     factor *= fct;
     // G <- R_y(-theta) R_z(-phi) G (rows 1..3)
@@ -223,7 +225,7 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const Nod
 
 // LDS per workgroup: the momenta column (4 n_out rows of blockDim.x + 1 doubles), then the
 // save banks (19 n_banks rows of blockDim.x doubles).
-template <bool FR>
+template <bool FR, bool PEV>
 __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict__ tiles, int n_tiles,
                                                       const SegDesc* __restrict__ segs,
                                                       const InvSeg* __restrict__ isegs,
@@ -237,7 +239,6 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
   double* ps = inv_lds;
   double* bank = ps + 4 * n_out * S + lane;
   double* pl = ps + lane;
-  const double bg_lab = chans[0].bg_lab, g_lab = chans[0].g_lab;
   for (int t = (int)blockIdx.x; t < n_tiles; t += (int)gridDim.x) {
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
@@ -247,13 +248,16 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
       const bool valid = lane < cnt;   // lanes past the end stage the last event and store nothing
       const int64_t e = td.e0 + base + (valid ? lane : cnt - 1);
       const int64_t row = is.row0 + e;
+      EvBeams eb;
+      if constexpr (PEV) eb = event_beams(chans[0], as_global(sd.e1)[e], as_global(sd.e2)[e]);
+      else eb = fixed_beams(chans[0]);
       // momenta back in the centre-of-mass frame (inverse of phs.cu:1286-1301)
       const gdouble* pe = as_global((const double*)sd.p) + e * 4 * n_out;
       for (int j = 0; j < n_out; j++) {
         double q0 = pe[4 * j], q1 = pe[4 * j + 1], q2 = pe[4 * j + 2], q3 = pe[4 * j + 3];
-        if (bg_lab != 0) {
-          const double a = g_lab * q0 - bg_lab * q3;
-          const double b = -bg_lab * q0 + g_lab * q3;
+        if (eb.bg_lab != 0) {
+          const double a = eb.g_lab * q0 - eb.bg_lab * q3;
+          const double b = -eb.bg_lab * q0 + eb.g_lab * q3;
           q0 = a; q3 = b;
         }
         pl[(4 * j) * S] = q0; pl[(4 * j + 1) * S] = q1; pl[(4 * j + 2) * S] = q2; pl[(4 * j + 3) * S] = q3;
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
           continue;
         }
         const ChannelDesc cd = chans[c];
-        if (valid) as_global(factors)[row * n_ch + c] = inverse_event<FR>(cd, nodes + cd.node0, pl, S, bank, B, xo);
+        if (valid) as_global(factors)[row * n_ch + c] = inverse_event<FR, PEV>(cd, eb, nodes + cd.node0, pl, S, bank, B, xo);
       }
     }
   }
@@ -288,7 +292,8 @@ hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_
   const size_t shmem = inverse_lds_bytes(block, it.n_out, it.n_banks);
   if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
   const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
-  auto k = it.has_friends ? inverse_kernel<true> : inverse_kernel<false>;
+  auto k = it.per_event_beams ? (it.has_friends ? inverse_kernel<true, true> : inverse_kernel<false, true>)
+                               : (it.has_friends ? inverse_kernel<true, false> : inverse_kernel<false, false>);
   hipLaunchKernelGGL(k, dim3(g), dim3(block), shmem, stream, tiles, n_tiles, segs, it.isegs, it.chans, it.nodes,
                      it.n_ch, it.n_out, it.n_x, n_rows, factors, x_out);
   return hipGetLastError();

@@ -18,6 +18,7 @@ struct DevicePlan {
   int nout_max = 0;      // largest n_out of the plan's channels
   int root_banks_only = 0;  // 1: no branch but a root defers a subtree, and the root has no friend
   int has_friends = 0;   // 1: some branch takes its polar axis from an incoming beam
+  int per_event_beams = 0;  // 1: some segment carries per-event beam energies
 };
 
 struct LaunchConfig {
@@ -67,7 +68,7 @@ struct InverseTables {
   const InvSeg* isegs = nullptr;
   const ChannelDesc* chans = nullptr;
   const NodeCmd* nodes = nullptr;
-  int n_ch = 0, n_out = 0, n_x = 0, n_banks = 0, has_friends = 0;
+  int n_ch = 0, n_out = 0, n_x = 0, n_banks = 0, has_friends = 0, per_event_beams = 0;
 };
 size_t inverse_lds_bytes(int block, int n_out, int n_banks);
 // factors [n_rows][n_ch] and x_out [n_ch][n_rows][n_x] for the events of tiles [0, n_tiles).

@@ -29,6 +29,7 @@ constexpr int kTileClasses = 2;
 
 struct PlanObj {
   std::vector<gpu_phs_segment> user;       // segments as given
+  std::vector<const double*> beam_e1, beam_e2;   // per-event beam energies of each segment, or null
   std::vector<ChannelDesc> chans;
   std::vector<NodeCmd> nodes;
   std::vector<SegDesc> segs;

@@ -99,6 +99,9 @@ struct SegDesc {
   const double* g_f;     // [n] cotangents of the factor, or null
   double* g_x;           // [n][n_x] gradient w.r.t. the randoms, or null
   double* loss;          // [n] per-event loss (fused loss mode), or null
+  // per-event beam energies [n] (phs.cu:1313-1317), or null: the channel's fixed beams
+  const double* e1;
+  const double* e2;
 };
 
 struct TileDesc {

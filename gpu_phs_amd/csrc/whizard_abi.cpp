@@ -24,6 +24,7 @@ struct WhizardModule {
   double E1 = -1, E2 = -1;
   // device buffers kept between calls (grow-only)
   double* x_d = nullptr; double* p_d = nullptr; double* f_d = nullptr; double* v_d = nullptr;
+  double* e1_d = nullptr; double* e2_d = nullptr;   // per-event beam energies
   uint8_t* ok_d = nullptr;
   size_t cap = 0;                   // events the buffers hold
   int cap_nx = 0, cap_nout = 0;
@@ -40,9 +41,9 @@ WhizardModule& W() {
 
 void free_buffers(WhizardModule& w) {
   (void)hipFree(w.x_d); (void)hipFree(w.p_d); (void)hipFree(w.f_d); (void)hipFree(w.v_d); (void)hipFree(w.ok_d);
-  (void)hipFree(w.fa_d); (void)hipFree(w.xo_d);
+  (void)hipFree(w.fa_d); (void)hipFree(w.xo_d); (void)hipFree(w.e1_d); (void)hipFree(w.e2_d);
   w.x_d = w.p_d = w.f_d = w.v_d = nullptr;
-  w.fa_d = w.xo_d = nullptr;
+  w.fa_d = w.xo_d = w.e1_d = w.e2_d = nullptr;
   w.ok_d = nullptr;
   w.cap = 0;
   w.inv_cap = 0;
@@ -57,6 +58,8 @@ int ensure_buffers(WhizardModule& w, size_t n) {
       (e = hipMalloc(&w.p_d, n * w.n_out * 4 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&w.f_d, n * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&w.v_d, n * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.e1_d, n * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.e2_d, n * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&w.ok_d, n)) != hipSuccess) {
     free_buffers(w);
     return hip_error(e, "hipMalloc(event buffers)");
@@ -266,16 +269,28 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
     set_error(GPU_PHS_E_STATE, "gen_phs_from_x_gpu: init_channel_ids covers fewer events");
     return;
   }
+  // beams: fixed, or per event (phs.cu:1313-1317); the trees are compiled for the first
+  // event's, and with per-event beams the kernels recompute what depends on them
   const double E1 = E1_in[0], E2 = E2_in[0];
-  for (int64_t i = 1; i < n; i++)
-    if (E1_in[i] != E1 || E2_in[i] != E2) {
-      set_error(GPU_PHS_E_ARG, "gen_phs_from_x_gpu: per-event beam energies that differ between events are not supported");
+  bool varying = false;
+  for (int64_t i = 0; i < n; i++) {
+    if (!(E1_in[i] > 0) || !(E2_in[i] > 0)) {
+      set_error(GPU_PHS_E_ARG, "gen_phs_from_x_gpu: beam energies must be positive");
       return;
     }
+    if (E1_in[i] != E1 || E2_in[i] != E2) varying = true;
+  }
   if (compile_for_beams(w, E1, E2)) return;
   if (ensure_buffers(w, (size_t)n)) return;
   hipError_t e = hipMemcpy(w.x_d, x, (size_t)n * w.n_x * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) { hip_error(e, "hipMemcpy(x)"); return; }
+  if (varying) {
+    if ((e = hipMemcpy(w.e1_d, E1_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(w.e2_d, E2_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+      hip_error(e, "hipMemcpy(beam energies)");
+      return;
+    }
+  }
 
   // segments = runs of equal channel id
   std::vector<gpu_phs_segment> segs;
@@ -296,6 +311,12 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
   }
   PlanObj pl;
   pl.user = segs;
+  if (varying) {
+    for (size_t k = 0, r = 0; k < segs.size(); r += segs[k].n_events, k++) {
+      pl.beam_e1.push_back(w.e1_d + r);
+      pl.beam_e2.push_back(w.e2_d + r);
+    }
+  }
   for (int c = 0; c < kTileClasses; c++) pl.lcs[c].block = w.block;
   if (plan_build(pl)) return;
   if (plan_run(pl, nullptr)) return;

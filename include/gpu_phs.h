@@ -80,12 +80,13 @@ void c_whizard_init_gpu_phs(void);
 /* whizard_gpu_phs.cpp:110-112: releases everything the calls above allocated. */
 void c_whizard_finalize_gpu_phs(void);
 /* whizard_gpu_phs.cpp:114-122: generates n_events events from host randoms x (event-major,
- * n_x each) with per-event beam energies E1, E2.  Writes p [n_events][n_out][4], factors at
- * [n_channels * i + c] (the event's own channel c; the other channels' entries, which the
- * reference fills by inverse mapping, are set to 0), volumes [n_events], oks [n_events], and
- * x_out [n_channels][n_events][n_x] (own channel: the input randoms; other channels: 0).
- * requires_boost_to_lab is read but, as in the reference (phs.cu:1559), the lab boost is
- * applied whenever E1 != E2. */
+ * n_x each) with per-event beam energies E1, E2 (equal for all events, or differing: then
+ * each event gets its own sqrt(s), phs.cu:1313-1317).  Writes p [n_events][n_out][4],
+ * factors at [n_channels * i + c] for every channel c (the event's own channel: its factor;
+ * the others: the inverse mapping of its momenta, phs.cu:1491-1554), volumes [n_events], oks
+ * [n_events], and x_out [n_channels][n_events][n_x] (own channel: the input randoms; the
+ * others: the inverse mapping; may be NULL).  requires_boost_to_lab is read but, as in the
+ * reference (phs.cu:1559), the lab boost is applied whenever E1 != E2. */
 void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_boost_to_lab, int* n_events,
                                   int* n_channels, int* n_x, double* x, double* factors, double* volumes,
                                   bool* oks, double* p, double* x_out);
@@ -148,6 +149,12 @@ int gpu_phs_plan_destroy(int plan);
 int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds);
 int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu,
                             int* lds);
+/* Per-event beam energies of segment `segment` (device arrays [n_events], or both NULL for the
+ * process's fixed beams): each event gets sqrt(s) = 2 sqrt(E1 E2), its lab boost and the map
+ * constants that depend on sqrt(s), as the reference does per event (phs.cu:1313-1317,
+ * 1286-1301; mappings_gpu.hpp).  The arrays must outlive the plan's use.  Plans with per-event
+ * beams have no adjoint. */
+int gpu_phs_plan_set_event_beams(int plan, int segment, const double* E1, const double* E2);
 /* Forward pass of every segment in one kernel launch on `stream` (hipStream_t, 0 = null). */
 int gpu_phs_plan_run(int plan, void* stream);
 /* Forward pass with one launch per segment (the plain per-channel form). */

@@ -58,11 +58,12 @@ def threshold_margin(proc, msq, channels, sqrts):
         for b in order:
             k1, k2 = proc.d1[c, b] - 1, proc.d2[c, b] - 1
             b1, b2 = m2[:, k1], m2[:, k2]
-            a = np.full_like(b1, sqrts * sqrts) if b == root else m2[:, b]
+            sq = np.asarray(sqrts, np.float64)[sel] if np.ndim(sqrts) else sqrts   # per-event sqrt(s) allowed
+            a = np.zeros_like(b1) + sq * sq if b == root else m2[:, b]
             lda = (a - b1 - b2) ** 2 - 4 * b1 * b2
             with np.errstate(all="ignore"):
                 m, m1_, m2_ = np.sqrt(a), np.sqrt(b1), np.sqrt(b2)
-                mmax = sqrts if b == root else sqrts - ms[c, root] + ms[c, b]
+                mmax = sq if b == root else sq - ms[c, root] + ms[c, b]
                 cuts = [np.abs(lda) / np.maximum(a * a, 1e-300), np.abs(m - m1_ - m2_) / np.maximum(m, 1e-300)]
                 if b != root:   # the root sits at m = m_max = sqrt(s) by construction
                     cuts.append(np.abs(mmax - m) / mmax)

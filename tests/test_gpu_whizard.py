@@ -145,3 +145,35 @@ def test_reference_samples_on_gpu(name):
     assert dev < 1e-6, dev
     assert np.allclose(volumes[sel], ev["volume"][order][sel], rtol=1e-12)
     L.c_whizard_finalize_gpu_phs()
+
+
+def test_whizard_abi_per_event_beams():
+    """Beam energies that differ from event to event: each event its own sqrt(s)."""
+    proc = W.process_c1()
+    L = setup_module_from(proc)
+    counts = [600, 0, 700]
+    n = sum(counts)
+    rng = np.random.default_rng(8)
+    x = rng.random((n, proc.n_x))
+    E1, E2 = 250.0 * rng.uniform(0.8, 1.3, n), 250.0 * rng.uniform(0.8, 1.3, n)
+    ch = np.repeat(np.arange(3), counts).astype(np.int32)
+    limits = (ctypes.c_int * 3)(*np.cumsum(counts).tolist())
+    L.c_whizard_init_channel_ids(_i(n), _i(3), limits)
+    nc = proc.n_channels
+    factors, volumes = np.zeros(n * nc), np.zeros(n)
+    oks, p = np.zeros(n, np.bool_), np.zeros((n, proc.n_out, 4))
+    x_out = np.zeros(n * nc * proc.n_x)
+    flag = ctypes.c_bool(True)
+    L.c_whizard_gen_phs_from_x_gpu(_p(np.ascontiguousarray(E1)), _p(np.ascontiguousarray(E2)), ctypes.byref(flag),
+                                   _i(n), _i(nc), _i(proc.n_x), _p(x), _p(factors), _p(volumes), _p(oks), _p(p),
+                                   _p(x_out))
+    assert L.gpu_phs_last_error() == 0, L.gpu_phs_last_error_message()
+    factors = factors.reshape(n, nc)
+    ref = oracle.forward(proc, x, ch, E1=E1, E2=E2)
+    r = compare(dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks), ref)
+    assert_parity(r)
+    p_ok = np.where(oks[:, None, None], p, 0.0)
+    f_ref, x_ref = oracle.inverse(proc, p_ok, E1=E1, E2=E2)
+    df, dx = inverse_stats(proc, factors, x_out.reshape(nc, n, proc.n_x), f_ref, x_ref, ch, oks)
+    assert_inverse_parity(df, dx, *sensitivity(proc, p_ok, f_ref, x_ref, ch, oks, E1=E1, E2=E2))
+    L.c_whizard_finalize_gpu_phs()
