@@ -120,6 +120,52 @@ def run_inverse(name, steps, warmup, torch, G, W, scale=1.0, block=0):
                 hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, block=block or 128)
 
 
+def run_whizard_abi(n, reps, G, W):
+    """The Whizard entry point end to end (host arrays in and out, the reference's call
+    sequence), c1 trees: forward pass, all-channel inverse mapping, PCIe copies both ways."""
+    import ctypes
+    import numpy as np
+    proc = W.process_c1()
+    L = G.lib()
+    ci = lambda v: ctypes.byref(ctypes.c_int(v))
+    cp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    nc = proc.n_channels
+    L.c_whizard_set_particle_structure(ci(nc), ci(nc), ci(1), ci(proc.n_x), ci(proc.n_in), ci(proc.n_out))
+    flv = np.ascontiguousarray(proc.flv_masses)
+    L.c_whizard_set_flavors(cp(flv), cp(np.zeros_like(flv)))
+    L.c_whizard_init_mappings()
+    L.c_whizard_init_tree_structures()
+    keep = []
+    for c in range(nc):
+        a = [np.ascontiguousarray(t[c]) for t in (proc.map_ids, proc.map_masses, proc.map_widths, proc.d1, proc.d2,
+                                                  proc.has_children)] + [np.zeros(proc.n_prt, np.int32)]
+        keep.append(a)
+        L.c_whizard_fill_mapping(ci(c + 1), cp(a[0]), cp(a[1]), cp(a[2]))
+        L.c_whizard_fill_tree_structure(ci(c + 1), cp(a[3]), cp(a[4]), cp(a[5]), cp(a[6]))
+    L.c_whizard_init_gpu_phs()
+    counts = [n // 3, n // 3, n - 2 * (n // 3)]
+    limits = (ctypes.c_int * 3)(*np.cumsum(counts).tolist())
+    L.c_whizard_init_channel_ids(ci(n), ci(3), limits)
+    x = np.random.default_rng(1).random((n, proc.n_x))
+    e1, e2 = np.full(n, proc.sqrts / 2), np.full(n, proc.sqrts / 2)
+    factors, volumes = np.zeros(n * nc), np.zeros(n)
+    oks, p, x_out = np.zeros(n, np.bool_), np.zeros((n, proc.n_out, 4)), np.zeros(n * nc * proc.n_x)
+    flag = ctypes.c_bool(True)
+    call = lambda: L.c_whizard_gen_phs_from_x_gpu(cp(e1), cp(e2), ctypes.byref(flag), ci(n), ci(nc), ci(proc.n_x),
+                                                  cp(x), cp(factors), cp(volumes), cp(oks), cp(p), cp(x_out))
+    call()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    dt = (time.perf_counter() - t0) / reps
+    if L.gpu_phs_last_error() != 0:
+        raise RuntimeError(L.gpu_phs_last_error_message().decode())
+    L.c_whizard_finalize_gpu_phs()
+    host_bytes = x.nbytes + e1.nbytes + e2.nbytes + factors.nbytes + volumes.nbytes + oks.nbytes + p.nbytes + x_out.nbytes
+    return dict(n_events=n, channels=nc, ms_per_call=dt * 1e3, events_s=n / dt, host_bytes_per_call=host_bytes,
+                host_gb_s=host_bytes / dt / 1e9)
+
+
 def cpu_baseline(wl_name, W):
     """The reference's own phs_cpu.cpp (oracle/_ref build) on one host core, over a bounded
     sample of the workload's events (repeated until ~10 s of CPU work)."""
@@ -249,6 +295,9 @@ def main():
             r = extra[f"inverse_{name}"]
             log(f"inverse mapping {name}: {r['pairs_s']:.4g} (event, channel) pairs/s, {r['hbm_frac']:.3f} of HBM")
             torch.cuda.empty_cache()
+        log("Whizard entry point (host arrays): setup")
+        extra["whizard_abi_c1"] = run_whizard_abi(1_000_000, 3, G, W)
+        log(f"Whizard entry point: {extra['whizard_abi_c1']['events_s']:.4g} events/s including PCIe copies")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -24,7 +24,8 @@ struct WhizardModule {
   double E1 = -1, E2 = -1;
   // device buffers kept between calls (grow-only)
   double* x_d = nullptr; double* p_d = nullptr; double* f_d = nullptr; double* v_d = nullptr;
-  double* e1_d = nullptr; double* e2_d = nullptr;   // per-event beam energies
+  double* e_d = nullptr;             // beam energies of the last call: E1 [n], then E2 [n] (phs.cu:1361-1363)
+  size_t last_n = 0;                 // events of the last generate call
   uint8_t* ok_d = nullptr;
   size_t cap = 0;                   // events the buffers hold
   int cap_nx = 0, cap_nout = 0;
@@ -41,9 +42,10 @@ WhizardModule& W() {
 
 void free_buffers(WhizardModule& w) {
   (void)hipFree(w.x_d); (void)hipFree(w.p_d); (void)hipFree(w.f_d); (void)hipFree(w.v_d); (void)hipFree(w.ok_d);
-  (void)hipFree(w.fa_d); (void)hipFree(w.xo_d); (void)hipFree(w.e1_d); (void)hipFree(w.e2_d);
+  (void)hipFree(w.fa_d); (void)hipFree(w.xo_d); (void)hipFree(w.e_d);
   w.x_d = w.p_d = w.f_d = w.v_d = nullptr;
-  w.fa_d = w.xo_d = w.e1_d = w.e2_d = nullptr;
+  w.fa_d = w.xo_d = w.e_d = nullptr;
+  w.last_n = 0;
   w.ok_d = nullptr;
   w.cap = 0;
   w.inv_cap = 0;
@@ -58,8 +60,7 @@ int ensure_buffers(WhizardModule& w, size_t n) {
       (e = hipMalloc(&w.p_d, n * w.n_out * 4 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&w.f_d, n * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&w.v_d, n * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&w.e1_d, n * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&w.e2_d, n * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&w.e_d, 2 * n * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&w.ok_d, n)) != hipSuccess) {
     free_buffers(w);
     return hip_error(e, "hipMalloc(event buffers)");
@@ -284,12 +285,12 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
   if (ensure_buffers(w, (size_t)n)) return;
   hipError_t e = hipMemcpy(w.x_d, x, (size_t)n * w.n_x * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) { hip_error(e, "hipMemcpy(x)"); return; }
-  if (varying) {
-    if ((e = hipMemcpy(w.e1_d, E1_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(w.e2_d, E2_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
-      hip_error(e, "hipMemcpy(beam energies)");
-      return;
-    }
+  // the beam energies stay on the device, E1 then E2, as the reference keeps them
+  // (get_in_momentum_device_pointer, phs.cu:1361-1363); per-event beams read them from there
+  if ((e = hipMemcpy(w.e_d, E1_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(w.e_d + n, E2_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+    hip_error(e, "hipMemcpy(beam energies)");
+    return;
   }
 
   // segments = runs of equal channel id
@@ -313,8 +314,8 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
   pl.user = segs;
   if (varying) {
     for (size_t k = 0, r = 0; k < segs.size(); r += segs[k].n_events, k++) {
-      pl.beam_e1.push_back(w.e1_d + r);
-      pl.beam_e2.push_back(w.e2_d + r);
+      pl.beam_e1.push_back(w.e_d + r);
+      pl.beam_e2.push_back(w.e_d + n + r);
     }
   }
   for (int c = 0; c < kTileClasses; c++) pl.lcs[c].block = w.block;
@@ -338,6 +339,7 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
     return;
   }
   w.last_p = w.p_d;
+  w.last_n = (size_t)n;
   ok_status();
 }
 
@@ -349,6 +351,28 @@ void c_whizard_get_momentum_device_pointer(double* p) {
   }
   ok_status();
 }
+
+}  // extern "C"
+
+// transfer_to_whizard.h:1-3 (C++ linkage, as the reference declares them): device pointers of
+// the last generate call's momenta ([n_events][n_out][4], the Whizard layout: the reference
+// returns its internal per-branch array and gathers on the host, phs.cu:1566-1575), beam
+// energies (E1 [n_events], then E2 [n_events]) and ok flags ([n_events] bool).  NULL before
+// the first call.
+double* get_momentum_device_pointer() {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  return W().last_p;
+}
+double* get_in_momentum_device_pointer() {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  return W().last_n ? W().e_d : nullptr;
+}
+bool* get_ok_device_pointer() {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  return W().last_n ? reinterpret_cast<bool*>(W().ok_d) : nullptr;
+}
+
+extern "C" {
 
 void c_whizard_reset_generator(void) {
   std::lock_guard<std::mutex> lk(api_mutex());

@@ -205,6 +205,16 @@ int gpu_phs_plan_inverse(int plan, const int64_t* seg_row0, int64_t n_rows, doub
 
 #ifdef __cplusplus
 }
+
+/* transfer_to_whizard.h:1-3, with C++ linkage as the reference declares them (defined in
+ * phs.cu:1638-1644; get_ok_device_pointer is declared there but never defined): device
+ * pointers of the last c_whizard_gen_phs_from_x_gpu call -- the momenta in the Whizard
+ * layout [n_events][n_out][4] (the reference returns its per-branch array and gathers the
+ * Whizard layout on the host), the beam energies E1 [n_events] followed by E2 [n_events],
+ * and the ok flags [n_events].  NULL before the first call. */
+double* get_momentum_device_pointer();
+double* get_in_momentum_device_pointer();
+bool* get_ok_device_pointer();
 #endif
 
 #endif /* GPU_PHS_H */

@@ -177,3 +177,33 @@ def test_whizard_abi_per_event_beams():
     df, dx = inverse_stats(proc, factors, x_out.reshape(nc, n, proc.n_x), f_ref, x_ref, ch, oks)
     assert_inverse_parity(df, dx, *sensitivity(proc, p_ok, f_ref, x_ref, ch, oks, E1=E1, E2=E2))
     L.c_whizard_finalize_gpu_phs()
+
+
+def test_device_pointers_of_the_last_call():
+    """transfer_to_whizard.h: the momenta (Whizard layout), beam energies and ok flags of the
+    last generate call stay on the device."""
+    proc = W.process_c1()
+    L = setup_module_from(proc)
+    counts = [300, 200, 100]
+    n = sum(counts)
+    x = np.random.default_rng(12).random((n, proc.n_x))
+    E1, E2 = 260.0, 240.0
+    p, factors, volumes, oks, _ = gen(L, proc, x, counts, E1, E2)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+
+    def fetch(sym, shape, dtype):
+        f = getattr(L, sym)
+        f.restype = ctypes.c_void_p
+        addr = f()
+        assert addr
+        out = np.zeros(shape, dtype)
+        assert hip.hipMemcpy(out.ctypes.data, addr, out.nbytes, 2) == 0   # device to host
+        return out
+
+    assert np.array_equal(fetch("_Z27get_momentum_device_pointerv", p.shape, np.float64), p)
+    beams = fetch("_Z30get_in_momentum_device_pointerv", (2, n), np.float64)
+    assert (beams[0] == E1).all() and (beams[1] == E2).all()
+    assert np.array_equal(fetch("_Z21get_ok_device_pointerv", n, np.bool_), oks)
+    L.c_whizard_finalize_gpu_phs()
