@@ -17,6 +17,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -118,6 +120,28 @@ def run_inverse(name, steps, warmup, torch, G, W, scale=1.0, block=0):
     return dict(n_events=n, channels=nc, kernel_ms=ms, events_s=n * steps / wall,
                 pairs_s=n * nc * steps / wall, bytes_per_launch=bytes_,
                 hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, block=block or 128)
+
+
+def run_grid(steps, torch, G, W):
+    """Randoms of the c1 workload (1e7 events x 8) drawn on the device from adaptive grids
+    (Philox4x32-10 + inverse distribution function), and one grid update (histogram of the
+    accepted events)."""
+    wl = W.workload("c1")
+    handles, segs = make_inputs(wl, torch, G)
+    plan = G.Plan(segs)
+    grid = G.Grid(handles[0])
+    w = np.random.default_rng(5).random(grid.shape) + 0.05
+    grid.set_weights(w)
+    wall, ev = time_steps(lambda: grid.generate(plan, 42), steps, 2, torch)
+    plan.run()
+    t0 = time.perf_counter()
+    grid.update(plan)
+    upd = time.perf_counter() - t0
+    n = wl.n_events
+    bytes_ = n * handles[0].n_x * 8
+    ms = ev / steps * 1e3
+    return dict(n_events=n, generate_ms=ms, randoms_s=n * handles[0].n_x / (ms / 1e3),
+                hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, update_ms=upd * 1e3)
 
 
 def run_whizard_abi(n, reps, G, W):
@@ -295,6 +319,10 @@ def main():
             r = extra[f"inverse_{name}"]
             log(f"inverse mapping {name}: {r['pairs_s']:.4g} (event, channel) pairs/s, {r['hbm_frac']:.3f} of HBM")
             torch.cuda.empty_cache()
+        log("adaptive grids: setup")
+        extra["grid_c1"] = run_grid(max(3, args.steps // 4), torch, G, W)
+        log(f"adaptive grids: {extra['grid_c1']['randoms_s']:.4g} randoms/s, {extra['grid_c1']['hbm_frac']:.3f} of HBM")
+        torch.cuda.empty_cache()
         log("Whizard entry point (host arrays): setup")
         extra["whizard_abi_c1"] = run_whizard_abi(1_000_000, 3, G, W)
         log(f"Whizard entry point: {extra['whizard_abi_c1']['events_s']:.4g} events/s including PCIe copies")

@@ -67,6 +67,12 @@ _SIGS = {
     "gpu_phs_plan_inverse": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_set_event_beams": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_phs_grid_create": (ctypes.c_int, [ctypes.c_int]),
+    "gpu_phs_grid_destroy": (ctypes.c_int, [ctypes.c_int]),
+    "gpu_phs_grid_get_weights": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "gpu_phs_grid_set_weights": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "gpu_phs_grid_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]),
+    "gpu_phs_grid_update": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
 }
 
 
@@ -302,6 +308,48 @@ class Plan:
     def close(self):
         if self.handle >= 0 and _lib is not None:
             lib().gpu_phs_plan_destroy(self.handle)
+        self.handle = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grid:
+    """Adaptive grids of a process's randoms: per channel and random, 10 equal bins with
+    weights (uniform at first), as the reference's rng.cpp keeps them.  ``generate`` fills the
+    randoms of every segment of a plan on the device; ``update`` sets the weights to the
+    histogram of the accepted events' randoms."""
+
+    BINS = 10
+
+    def __init__(self, ph: ProcessHandle):
+        self.ph = ph
+        self.handle = _check(lib().gpu_phs_grid_create(ph.handle), "grid_create")
+        self.shape = (ph.proc.n_channels, ph.n_x, self.BINS)
+
+    def weights(self):
+        w = np.zeros(self.shape)
+        _check(lib().gpu_phs_grid_get_weights(self.handle, _ptr(w), w.size), "grid_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        if w.shape != self.shape:
+            raise ValueError(f"weights must have shape {self.shape}")
+        _check(lib().gpu_phs_grid_set_weights(self.handle, _ptr(w), w.size), "grid_set_weights")
+
+    def generate(self, plan, seed, stream=None):
+        _check(lib().gpu_phs_grid_generate(self.handle, plan.handle, seed, _stream_ptr(stream)), "grid_generate")
+
+    def update(self, plan, stream=None):
+        _check(lib().gpu_phs_grid_update(self.handle, plan.handle, _stream_ptr(stream)), "grid_update")
+
+    def close(self):
+        if self.handle > 0 and _lib is not None:
+            lib().gpu_phs_grid_destroy(self.handle)
         self.handle = -1
 
     def __del__(self):

@@ -75,4 +75,13 @@ size_t inverse_lds_bytes(int block, int n_out, int n_banks);
 hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
                           int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream);
 
+// Adaptive grids of the randoms (rng.cpp:5: 10 equal bins per channel and dimension).
+constexpr int kGridBins = 10;
+// Randoms of every segment of the plan drawn from the grids: cdf [plan channel][n_dim][11].
+hipError_t launch_grid_generate(const DevicePlan& dp, const double* cdf, int n_dim, uint64_t seed, hipStream_t stream);
+// counts [plan channel][n_dim][10] histogram of the accepted events' randoms, then the accepted
+// count of every plan channel (n_chans more entries); cleared first.
+hipError_t launch_grid_histogram(const DevicePlan& dp, int n_chans, int n_dim, unsigned long long* counts,
+                                 hipStream_t stream);
+
 }  // namespace gphs

@@ -203,6 +203,25 @@ int gpu_phs_plan_param_map(int plan, int* process, int* channel, int* bincode, i
 int gpu_phs_plan_inverse(int plan, const int64_t* seg_row0, int64_t n_rows, double* factors, double* x_out,
                          int block, void* stream);
 
+/* Adaptive grids of the randoms (replaces rng.cpp:18-93: init_rng, rng_generate,
+ * update_weights, which run on the host).  A grid holds, per channel of a process and per
+ * random, 10 equal bins with weights, uniform at creation (rng.cpp:20-27).  Returns a handle
+ * > 0. */
+int gpu_phs_grid_create(int process);
+int gpu_phs_grid_destroy(int grid);
+/* The weights [n_channels][n_x][10] (n = their count). */
+int gpu_phs_grid_get_weights(int grid, double* weights, int n);
+int gpu_phs_grid_set_weights(int grid, const double* weights, int n);
+/* Fills the randoms x of every segment of `plan` (all of the grid's process) on the device:
+ * random k of event e of segment s is drawn from its channel's piecewise-constant density
+ * (rng.cpp:70-90), with uniform numbers from Philox4x32-10 at counter (e, s, k / 2) and key
+ * `seed` instead of the reference's std::default_random_engine, so a call is reproducible. */
+int gpu_phs_grid_generate(int grid, int plan, uint64_t seed, void* stream);
+/* Sets the weights of every channel that has accepted events (ok) in `plan` to the histogram
+ * of their randoms, bin count / accepted count (rng.cpp:33-67); channels without accepted
+ * events keep theirs (the reference divides by zero there).  Synchronizes with `stream`. */
+int gpu_phs_grid_update(int grid, int plan, void* stream);
+
 #ifdef __cplusplus
 }
 

@@ -232,3 +232,85 @@ def ref_time(proc, x, channels):
                                  *[_ptr(a) for a in t], _D(proc.sqrts), _L(ch.shape[0]), _ptr(ch),
                                  _I(proc.n_x), _ptr(x), ctypes.byref(n_ok))
     return sec, n_ok.value
+
+
+# ---------------------------------------------------------------------------------------------
+# Adaptive grids of the randoms (rng.cpp:18-93), numpy restatement of the device version.
+
+GRID_BINS = 10   # rng.cpp:5
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC 2011): ctr [n, 4] uint32, key (k0, k1)."""
+    c = [np.asarray(ctr[:, i], np.uint64) for i in range(4)]
+    k0, k1 = np.uint64(key[0]), np.uint64(key[1])
+    M0, M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+    W0, W1 = np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0, p1 = M0 * c[0], M1 * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+        k0, k1 = (k0 + W0) & mask, (k1 + W1) & mask
+    return np.stack(c, axis=1).astype(np.uint32)
+
+
+def grid_cdf(w):
+    """Cumulative bin probabilities [..., 11] of weights [..., 10], F[10] = 1."""
+    w = np.asarray(w, np.float64)
+    s = np.zeros(w.shape[:-1])
+    for b in range(GRID_BINS):
+        s = s + w[..., b]
+    F = np.zeros(w.shape[:-1] + (GRID_BINS + 1,))
+    for b in range(GRID_BINS):
+        F[..., b + 1] = F[..., b] + w[..., b] / s
+    F[..., GRID_BINS] = 1.0
+    return F
+
+
+def grid_uniforms(seed, segment, n_events, n_x):
+    """The uniform numbers u [n_events, n_x] of one segment: random k of event e from the
+    Philox block (e, segment, k // 2), two 32-bit words per double."""
+    n_blk = (n_x + 1) // 2
+    e = np.repeat(np.arange(n_events, dtype=np.uint64), n_blk)
+    j = np.tile(np.arange(n_blk, dtype=np.uint64), n_events)
+    ctr = np.stack([e & np.uint64(0xFFFFFFFF), e >> np.uint64(32), np.full_like(e, segment), j], axis=1)
+    out = philox4x32_10(ctr, (seed & 0xFFFFFFFF, seed >> 32)).astype(np.uint64).reshape(n_events, n_blk, 4)
+    u = np.empty((n_events, 2 * n_blk))
+    for h in range(2):
+        bits = ((out[:, :, 2 * h] << np.uint64(32)) | out[:, :, 2 * h + 1]) >> np.uint64(11)
+        u[:, h::2] = bits.astype(np.float64) * 2.0 ** -53
+    return u[:, :n_x]
+
+
+def grid_sample(F, u):
+    """x from u through the inverse distribution function of the piecewise-constant density
+    with cumulative bin probabilities F [n_x, 11] (rng.cpp:74-79: 10 equal bins on [0, 1]):
+    bin b = the first with F[b + 1] > u, x = b / 10 + (u - F[b]) / rho_b."""
+    u = np.asarray(u, np.float64)
+    x = np.empty_like(u)
+    for k in range(u.shape[1]):
+        Fk = F[k]
+        b = np.zeros(u.shape[0], np.int64)
+        for i in range(GRID_BINS - 1):
+            b += (b == i) & ~(Fk[i + 1] > u[:, k])
+        rho = (Fk[b + 1] - Fk[b]) * GRID_BINS
+        x[:, k] = b / GRID_BINS + (u[:, k] - Fk[b]) / rho
+    return x
+
+
+def grid_update(w, x, ok, channels):
+    """rng.cpp:33-67: per channel with accepted events, weight = bin count / accepted count;
+    the bin of x is (int)(x * 10).  Channels without accepted events keep w."""
+    w = np.array(w, np.float64)
+    n_ch, n_dim, _ = w.shape
+    for c in range(n_ch):
+        sel = ok & (channels == c)
+        n_ok = int(sel.sum())
+        if n_ok == 0:
+            continue
+        b = np.clip((x[sel] * GRID_BINS).astype(np.int64), 0, GRID_BINS - 1)
+        for d in range(n_dim):
+            w[c, d] = np.bincount(b[:, d], minlength=GRID_BINS) / n_ok
+    return w
