@@ -175,6 +175,26 @@ def run_grid(steps, torch, G, W):
                 hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, update_ms=upd * 1e3)
 
 
+def cpu_inverse_baseline(name, scale, W):
+    """The oracle's inverse mapping (a C++ restatement of the reference's, single thread) on a
+    bounded sample of the workload's events: (event, channel) pairs per second."""
+    import oracle
+    wl = W.workload(name, scale=scale)
+    proc = wl.processes[0]
+    ch = np.concatenate([np.full(s.n_events, s.channel, np.int32) for s in wl.segments])
+    x = np.random.default_rng(wl.seed).random((ch.shape[0], proc.n_x))
+    p = oracle.forward(proc, x, ch)["p"]
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 3.0 or reps == 0:
+        oracle.inverse(proc, p)
+        reps += 1
+    dt = time.perf_counter() - t0
+    pairs = ch.shape[0] * proc.n_channels * reps
+    return dict(value=pairs / dt, unit="(event, channel) pairs/s", cores=1, kind="port",
+                sample=f"{ch.shape[0]} events of {name} x {proc.n_channels} channels x {reps} passes, {dt:.1f} s")
+
+
 def run_whizard_abi(n, reps, G, W):
     """The Whizard entry point end to end (host arrays in and out, the reference's call
     sequence), c1 trees: forward pass, all-channel inverse mapping, PCIe copies both ways."""
@@ -366,6 +386,10 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(args.workload, W)
+        if not args.no_extra:
+            for name, scale in (("c1", 0.01), ("c2", 0.002)):
+                if f"inverse_{name}" in extra:
+                    extra[f"inverse_{name}"]["cpu_baseline"] = cpu_inverse_baseline(name, scale, W)
     steps = args.steps
     traffic, traffic_config = measured_traffic(args.workload)
     out = {
