@@ -37,17 +37,24 @@ __device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
   return (double)((((uint64_t)a << 32) | b) >> 11) * 0x1.0p-53;
 }
 
-// cdf: [channel][dim][kGridBins + 1] cumulative bin probabilities of the plan's channels.
+// cdf: [channel][dim][kGridBins + 1] cumulative bin probabilities of the plan's channels.  A
+// workgroup copies its channel's table into LDS; the bin is the number of interior bounds
+// F[1..9] at or below u, counted without branches.
 __global__ __launch_bounds__(256) void grid_generate_kernel(const TileDesc* __restrict__ tiles, int n_tiles,
                                                             const SegDesc* __restrict__ segs,
                                                             const ChannelDesc* __restrict__ chans,
                                                             const double* __restrict__ cdf, int n_dim,
                                                             uint64_t seed) {
+  extern __shared__ double cdf_l[];   // n_dim * (kGridBins + 1)
+  constexpr int NB1 = kGridBins + 1;
   for (int t = (int)blockIdx.x; t < n_tiles; t += (int)gridDim.x) {
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
     const int n_x = chans[sd.channel].n_x;
-    const double* F0 = cdf + (size_t)sd.channel * n_dim * (kGridBins + 1);
+    const double* F0 = cdf + (size_t)sd.channel * n_dim * NB1;
+    __syncthreads();   // the previous tile's table has been read
+    for (int i = (int)threadIdx.x; i < n_x * NB1; i += (int)blockDim.x) cdf_l[i] = F0[i];
+    __syncthreads();
     for (int local = (int)threadIdx.x; local < td.count; local += (int)blockDim.x) {
       const int64_t e = td.e0 + local;
       gdouble* xr = as_global(const_cast<double*>(sd.x)) + e * n_x;
@@ -59,9 +66,10 @@ __global__ __launch_bounds__(256) void grid_generate_kernel(const TileDesc* __re
           const int k = 2 * j + h;
           if (k >= n_x) break;
           const double u = u53(c[2 * h], c[2 * h + 1]);
-          const double* F = F0 + k * (kGridBins + 1);
+          const double* F = cdf_l + k * NB1;
           int b = 0;
-          while (b < kGridBins - 1 && !(F[b + 1] > u)) b++;
+#pragma unroll
+          for (int q = 1; q < kGridBins; q++) b += F[q] <= u ? 1 : 0;
           const double rho = (F[b + 1] - F[b]) * kGridBins;   // density of bin b (width 1 / kGridBins)
           xr[k] = (double)b / kGridBins + (u - F[b]) / rho;
         }
@@ -115,7 +123,8 @@ __global__ void fill_u64_kernel(unsigned long long* p, size_t n) {
 
 hipError_t launch_grid_generate(const DevicePlan& dp, const double* cdf, int n_dim, uint64_t seed, hipStream_t stream) {
   if (dp.n_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(grid_generate_kernel, dim3(dp.n_tiles), dim3(256), 0, stream, dp.tiles, dp.n_tiles, dp.segs,
+  const size_t shmem = (size_t)n_dim * (kGridBins + 1) * sizeof(double);
+  hipLaunchKernelGGL(grid_generate_kernel, dim3(dp.n_tiles), dim3(256), shmem, stream, dp.tiles, dp.n_tiles, dp.segs,
                      dp.chans, cdf, n_dim, seed);
   return hipGetLastError();
 }
