@@ -7,18 +7,18 @@
 // the channel's program (NodeCmd records, phs_types.h) is read with scalar loads, so map
 // selection is a wave-uniform branch instead of the reference's per-lane function pointers
 // (mappings_gpu.hpp:312-363).  The boost matrix of the branch being visited stays in
-// registers; the matrix of a deferred subtree waits in one of NBANK save banks.
+// registers; the matrix of a deferred subtree waits in a save bank (registers or LDS).
 //
-// Each lane loads its event's randoms (one row of [n][n_x]) into registers when the event
-// starts, so the branch loop never waits on memory; the row is indexed with the program's
-// wave-uniform random indices (register indexing by a scalar).  The momenta, the bulk of the
+// Each lane loads its event's randoms (one row of [n][n_x]) when the event starts, so the
+// branch loop never waits on memory: into 8 or 16 registers, indexed with the program's
+// wave-uniform random indices (register indexing by a scalar), or into the lane's column of an
+// LDS table (always for more than 16 randoms, n_out > 6).  The momenta, the bulk of the
 // traffic, go through LDS: each lane writes its event's four-vectors into a table
 // p[4 j + mu][event] (consecutive lanes, consecutive words: conflict-free), and the workgroup
 // then writes the sub-tile's momenta ([n][n_out][4]) as one contiguous run of 16-byte words,
 // so the stores leave the CU as whole cache lines instead of 64 scattered 16-byte pieces.
 // Rows are padded to block + 1 doubles so that the gather of the store phase spreads over the
-// LDS banks.  Channels with more than 16 randoms (n_out > 6) read them from memory as they
-// are used instead.
+// LDS banks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -131,7 +131,8 @@ def test_whizard_abi_asymmetric_beams_lab_boost():
 @pytest.mark.parametrize("name", ["ttH_head.ref", "bwbw_head.ref"])
 def test_reference_samples_on_gpu(name):
     """Whizard's own momenta (9-decimal randoms) from the reference's sample files, through the
-    Whizard ABI on the GPU; trees with friends are left out (no CPU counterpart)."""
+    Whizard ABI on the GPU; trees with friends are left out here (the file carries no per-branch
+    friend data; test_gpu_friend.py covers them with the assignment Whizard's factors fix)."""
     proc, ev = read_ref(os.path.join(GOLDEN, name))
     L = setup_module_from(proc)
     ch = ev["channel"]

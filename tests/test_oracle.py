@@ -84,7 +84,8 @@ def test_whizard_momenta_tth():
 
 def test_whizard_momenta_bwbw():
     """samples/bwbw.ref (2->4, 61 channels incl. BW and t-channel maps); channels whose tree has
-    friends are skipped (the reference's CPU algorithm has no friend axis)."""
+    friends are skipped here (the reference's CPU algorithm has no friend axis and the file no
+    per-branch friend data; test_whizard_friends.py covers them)."""
     r, ev, sel, dev = _whizard_check(os.path.join(GOLDEN, "bwbw_head.ref"))
     assert sel.sum() > 50
     assert r["ok"].all()
