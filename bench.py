@@ -297,6 +297,10 @@ def main():
                     help="only run the forward + adjoint workload (c4) and print its figures")
     ap.add_argument("--inverse-only", action="store_true",
                     help="only run the all-channel inverse mapping of c1 and c2 and print its figures")
+    ap.add_argument("--grid-only", action="store_true",
+                    help="only run the adaptive-grid generation and update on c1 and print the figures")
+    ap.add_argument("--beams-only", action="store_true",
+                    help="only run c1 with per-event beam energies and print the figures")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -324,6 +328,12 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(run_adjoint(max(5, args.steps), args.warmup, torch, G, W)))
+        return
+    if args.grid_only:
+        print(json.dumps(run_grid(max(3, args.steps), torch, G, W)))
+        return
+    if args.beams_only:
+        print(json.dumps(run_event_beams(max(3, args.steps), torch, G, W)))
         return
     if args.inverse_only:
         r = {"c1": run_inverse("c1", max(3, args.steps // 2), 1, torch, G, W),
