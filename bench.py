@@ -267,6 +267,30 @@ def cpu_baseline(wl_name, W):
                        f"single thread, {total_t:.1f} s; reference phs_cpu.cpp built -O3")
 
 
+def cpu_all_cores(wl_name, W):
+    """The reference's phs_cpu.cpp (oracle/_ref build) split over the host cores this process
+    may use (contiguous event ranges, one thread each, at most 16), on the same bounded sample
+    as the single-thread baseline; None where the reference build is absent."""
+    import numpy as np
+    import oracle
+    if not oracle.ref_available():
+        return None
+    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    wl = W.workload(wl_name, scale=0.2)
+    proc = wl.processes[0]
+    ch = np.concatenate([np.full(s.n_events, s.channel, np.int32) for s in wl.segments])
+    x = np.random.default_rng(wl.seed).random((ch.shape[0], proc.n_x))
+    total_t, total_n, reps = 0.0, 0, 0
+    while total_t < 3.0 and reps < 40:
+        sec, _ = oracle.ref_time_threads(proc, x, ch, threads)
+        total_t += sec
+        total_n += ch.shape[0]
+        reps += 1
+    return dict(value=total_n / total_t, unit="events/s", cores=threads, kind="reference",
+                sample=f"{ch.shape[0]} events of {wl_name} x {reps} passes, {threads} threads over contiguous "
+                       f"event ranges, {total_t:.1f} s wall; reference phs_cpu.cpp built -O3")
+
+
 def measured_traffic(workload):
     """HBM bytes per launch of the forward kernel from the committed counter profile
     (profiles/forward_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by
@@ -396,6 +420,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(args.workload, W)
+        log("cpu baseline, all cores")
+        extra["cpu_all_cores"] = cpu_all_cores(args.workload, W)
         if not args.no_extra:
             for name, scale in (("c1", 0.01), ("c2", 0.002)):
                 if f"inverse_{name}" in extra:

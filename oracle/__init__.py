@@ -64,6 +64,7 @@ def ref():
         _ref = ctypes.CDLL(REF_LIB)
         _ref.ref_forward.restype = _I
         _ref.ref_time_forward.restype = _D
+        _ref.ref_time_forward_mt.restype = _D
     return _ref
 
 
@@ -231,6 +232,21 @@ def ref_time(proc, x, channels):
     sec = ref().ref_time_forward(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
                                  *[_ptr(a) for a in t], _D(proc.sqrts), _L(ch.shape[0]), _ptr(ch),
                                  _I(proc.n_x), _ptr(x), ctypes.byref(n_ok))
+    return sec, n_ok.value
+
+
+def ref_time_threads(proc, x, channels, threads):
+    """Wall seconds of the reference's generation loop split over `threads` host threads
+    (contiguous event ranges), and its ok count."""
+    x = np.array(x, np.float64, order="C")
+    ch = np.ascontiguousarray(channels, np.int32)
+    n_ok = ctypes.c_long()
+    t = _tree_args(proc)
+    sec = ref().ref_time_forward_mt(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                                    *[_ptr(a) for a in t], _D(proc.sqrts), _L(ch.shape[0]), _ptr(ch),
+                                    _I(proc.n_x), _ptr(x), _I(threads), ctypes.byref(n_ok))
+    if sec < 0:
+        raise RuntimeError("reference timing failed")
     return sec, n_ok.value
 
 

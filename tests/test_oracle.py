@@ -59,6 +59,23 @@ def test_oracle_bitwise_equals_reference_build(name):
     assert np.array_equal(a["msq"], b["msq"], equal_nan=True)
 
 
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference CPU build (oracle/_ref) not present")
+def test_reference_timing_threads_cover_every_event():
+    """The all-cores CPU baseline splits the events over threads; its ok count is the
+    single-thread loop's and the reference build's forward pass's."""
+    proc = PROCS["c1"]()
+    rng = np.random.default_rng(5)
+    n = 20011
+    ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
+    x = rng.random((n, proc.n_x))
+    n_ok = int(oracle.ref_forward(proc, x, ch)["ok"].sum())
+    assert oracle.ref_time(proc, x, ch)[1] == n_ok
+    for threads in (1, 3, 8):
+        assert oracle.ref_time_threads(proc, x, ch, threads)[1] == n_ok
+    with pytest.raises(RuntimeError):
+        oracle.ref_time_threads(proc, x, ch, 0)
+
+
 def _whizard_check(path, perm=None, only_nofriends=True):
     proc, ev = read_ref(path)
     sel = np.ones(len(ev["channel"]), bool)
