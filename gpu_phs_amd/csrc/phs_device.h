@@ -176,8 +176,8 @@ __device__ __forceinline__ void ct_map(int kind, double x, double b0, double b1,
 }
 
 // M <- M * L1 with L1 = B_z(bg) R_z(phi) R_y(theta) (phs_cpu.cpp:314-340); the products with
-// the structural zeros of L1 are left out, every other product and sum keeps the reference's
-// order.
+// the structural zeros of L1 are left out, the others are summed in the reference's order,
+// each after the first as a fused multiply-add (one rounding per term instead of two).
 __device__ __forceinline__ void compose(double (&M)[16], double g, double bg, double ct, double st, double cp, double sp) {
   const double l01 = -bg * st, l03 = bg * ct;
   const double l11 = ct * cp, l12 = -sp, l13 = st * cp;
@@ -186,10 +186,10 @@ __device__ __forceinline__ void compose(double (&M)[16], double g, double bg, do
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const double r0 = M[4 * i], r1 = M[4 * i + 1], r2 = M[4 * i + 2], r3 = M[4 * i + 3];
-    M[4 * i + 0] = r0 * g + r3 * bg;
-    M[4 * i + 1] = r0 * l01 + r1 * l11 + r2 * l21 + r3 * l31;
-    M[4 * i + 2] = r1 * l12 + r2 * l22;
-    M[4 * i + 3] = r0 * l03 + r1 * l13 + r2 * l23 + r3 * l33;
+    M[4 * i + 0] = __builtin_fma(r3, bg, r0 * g);
+    M[4 * i + 1] = __builtin_fma(r3, l31, __builtin_fma(r2, l21, __builtin_fma(r1, l11, r0 * l01)));
+    M[4 * i + 2] = __builtin_fma(r2, l22, r1 * l12);
+    M[4 * i + 3] = __builtin_fma(r3, l33, __builtin_fma(r2, l23, __builtin_fma(r1, l13, r0 * l03)));
   }
 }
 
@@ -201,8 +201,8 @@ __device__ __forceinline__ void compose_cols03(double (&M)[16], double g, double
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const double r0 = M[4 * i], r1 = M[4 * i + 1], r2 = M[4 * i + 2], r3 = M[4 * i + 3];
-    M[4 * i + 0] = r0 * g + r3 * bg;
-    M[4 * i + 3] = r0 * l03 + r1 * l13 + r2 * l23 + r3 * l33;
+    M[4 * i + 0] = __builtin_fma(r3, bg, r0 * g);
+    M[4 * i + 3] = __builtin_fma(r3, l33, __builtin_fma(r2, l23, __builtin_fma(r1, l13, r0 * l03)));
   }
 }
 

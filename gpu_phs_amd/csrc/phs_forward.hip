@@ -167,12 +167,15 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
     const double m = cur_m;
     const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
     ok = ok && (lda > 0 && m > m1 + m2 && m <= m_max);
+    // one reciprocal of m (estimate + two Newton steps) serves p = sqrt(lambda) / (2 m) and
+    // bg = p_parent / m; the results differ from the quotients by an ulp or so
     const double sl = math::sqrt_phys(lda);
-    const double pk = sl / (2 * m);
-    factor *= f0 * f1 * (sl / cur_msq);
+    const double rm = math::recip(m);
+    const double pk = 0.5 * (sl * rm);
+    factor *= f0 * f1 * math::quot(sl, cur_msq);
 
     // angles and the boost of this branch (phs_cpu.cpp:299-340)
-    const double bg = m > 0 ? cur_p / m : 0.0;
+    const double bg = m > 0 ? cur_p * rm : 0.0;
     const double g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
     math::sincos_turns(GPHS_X(nc.x_phi), sp, cp);  // phi = 2 pi x
@@ -195,10 +198,11 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
       if (nc.k[c].leaf) {
         const double pc = c == 0 ? pk : -pk;
         const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
-        double q0 = M[0] * E + M[3] * pc;
-        const double q1 = M[4] * E + M[7] * pc;
-        const double q2 = M[8] * E + M[11] * pc;
-        double q3 = M[12] * E + M[15] * pc;
+        // rows of L (E, 0, 0, pc), the second product fused
+        double q0 = __builtin_fma(M[3], pc, M[0] * E);
+        const double q1 = __builtin_fma(M[7], pc, M[4] * E);
+        const double q2 = __builtin_fma(M[11], pc, M[8] * E);
+        double q3 = __builtin_fma(M[15], pc, M[12] * E);
         if (eb.bg_lab != 0) {
           const double b0 = eb.g_lab * q0 + eb.bg_lab * q3;
           const double b3 = eb.bg_lab * q0 + eb.g_lab * q3;

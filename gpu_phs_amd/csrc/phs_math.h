@@ -99,9 +99,9 @@ GPHS_MATH_FN double fma_sc(double a, double b, double c) {
 #endif
 }
 
-// a / b for derivative formulas, where the last bit does not matter: the reciprocal estimate
-// refined by two Newton steps (r <- r + r (1 - b r)), times a; within a few ulps of the
-// quotient for normal b, and half the instructions of a correctly rounded division.
+// a / b where an ulp or so does not matter: the reciprocal estimate refined by two Newton
+// steps (r <- r + r (1 - b r)), times a; within a few ulps of the quotient for normal b, and
+// half the instructions of a correctly rounded division.
 GPHS_MATH_FN double quot(double a, double b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(b);
@@ -112,6 +112,19 @@ GPHS_MATH_FN double quot(double a, double b) {
   return a * r;
 #else
   return a / b;
+#endif
+}
+
+// 1 / b: the reciprocal estimate refined by two Newton steps, within an ulp or so for normal b.
+GPHS_MATH_FN double recip(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  return __builtin_fma(r, e, r);
+#else
+  return 1.0 / b;
 #endif
 }
 
