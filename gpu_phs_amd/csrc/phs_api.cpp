@@ -408,9 +408,9 @@ int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, i
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
   if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1 ||
-      !(waves_per_eu == 0 || waves_per_eu == 3) || lds < 0 || lds > 3)
+      !(waves_per_eu == 0 || waves_per_eu == 3 || waves_per_eu == 4) || lds < 0 || lds > 3)
     return set_error(GPU_PHS_E_ARG,
-                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3}, "
+                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4}, "
                      "lds in {0,1,2,3}");
   LaunchConfig want;
   want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds = lds;
@@ -556,7 +556,7 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
     const bool banks = pl->dps[cls].n_banks > 0;
     // stage 1: register budget (occupancy target), where deferred boosts wait (lds bit 0) and
     // where the randoms are kept (lds bit 1)
-    for (int wpe : {0, 3})
+    for (int wpe : {0, 3, 4})
       for (int lds : {0, 1, 2, 3})
         if (!rc && (!(lds & 1) || banks)) {
           LaunchConfig lc = best.lc;

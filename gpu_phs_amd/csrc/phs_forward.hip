@@ -331,7 +331,9 @@ FwdKernel pick_wpe(const DevicePlan& dp, const LaunchConfig& lc) {
   if (dp.per_event_beams)
     return dp.has_friends ? forward_kernel<BANK, 0, kXLds, true, true> : forward_kernel<BANK, 0, kXLds, false, true>;
   if (dp.has_friends) return pick_xn<BANK, 0, true>(xm);
-  return lc.waves_per_eu == 3 ? pick_xn<BANK, 3, false>(xm) : pick_xn<BANK, 0, false>(xm);
+  if (lc.waves_per_eu == 3) return pick_xn<BANK, 3, false>(xm);
+  if (lc.waves_per_eu == 4) return pick_xn<BANK, 4, false>(xm);
+  return pick_xn<BANK, 0, false>(xm);
 }
 
 // The bank kind a plan runs with under `lc`.

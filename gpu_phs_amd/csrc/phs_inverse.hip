@@ -11,7 +11,8 @@
 // the reference's _move_factors / _move_x do (phs.cu:1495-1497).
 //
 // The steps per branch follow the GPU reference with its defects corrected (the oracle's
-// inverse lists them): the msq of both daughters from sums of final-state momenta, x and the
+// inverse lists them): the msq of both daughters from sums of final-state momenta (added as
+// the reference adds them, daughter by daughter), x and the
 // map factor of each internal daughter from its msq, lambda and the decay momentum, then the
 // angles of the first daughter in B_z(-bg) G, where G = L_parent^-1 is carried down the tree
 // (a branch with a friend measures them from the friend beam), x_phi = atan2 / 2 pi, x of
@@ -103,6 +104,38 @@ __device__ __forceinline__ void leaf_sum(const double* __restrict__ pl, int S, i
   }
 }
 
+// A daughter's four-momentum: its sum program (ChildCmd::sum_code) over the staged final-state
+// momenta, which adds them in the reference's association, daughter sums first; the index
+// order of `leaves` where the subtree has no program.
+__device__ __forceinline__ void branch_sum(const double* __restrict__ pl, int S, int n_out, const ChildCmd& c,
+                                           double (&q)[4]) {
+  if (c.sum_len <= 0) {
+    leaf_sum(pl, S, n_out, c.leaves, q);
+    return;
+  }
+  double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+  bool second = false;
+  uint64_t code = c.sum_code;
+  for (int t = 0; t < c.sum_len; t++, code >>= 4) {
+    const int op = (int)(code & 15u);
+    if (op == kSumPush) {
+      second = true;
+      b[0] = b[1] = b[2] = b[3] = 0;
+    } else if (op == kSumMerge) {
+      a[0] = a[0] + b[0]; a[1] = a[1] + b[1]; a[2] = a[2] + b[2]; a[3] = a[3] + b[3];
+      second = false;
+    } else {
+      const double* r = pl + 4 * op * S;
+      if (second) {
+        b[0] = b[0] + r[0]; b[1] = b[1] + r[S]; b[2] = b[2] + r[2 * S]; b[3] = b[3] + r[3 * S];
+      } else {
+        a[0] = a[0] + r[0]; a[1] = a[1] + r[S]; a[2] = a[2] + r[2 * S]; a[3] = a[3] + r[3 * S];
+      }
+    }
+  }
+  q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; q[3] = a[3];
+}
+
 // Inverse of one event into channel `cd` (program `prog`): the randoms go to xo[0..n_x),
 // returns the factor.
 template <bool FR, bool PEV>
@@ -116,8 +149,8 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvB
   for (int i = 0; i < cd.n_nodes; ++i) {
     const NodeCmd& nc = prog[i];
     double q0[4], q1[4];
-    leaf_sum(pl, S, cd.n_out, nc.k[0].leaves, q0);
-    leaf_sum(pl, S, cd.n_out, nc.k[1].leaves, q1);
+    branch_sum(pl, S, cd.n_out, nc.k[0], q0);
+    branch_sum(pl, S, cd.n_out, nc.k[1], q1);
     const double msq0 = q0[0] * q0[0] - q0[1] * q0[1] - q0[2] * q0[2] - q0[3] * q0[3];
     const double msq1 = q1[0] * q1[0] - q1[1] * q1[1] - q1[2] * q1[2] - q1[3] * q1[3];
     // x and factor of the internal daughters' msq maps

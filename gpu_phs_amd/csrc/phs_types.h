@@ -44,7 +44,20 @@ struct ChildCmd {
   double da_dw[3];   // d a[k] / d Gamma
   int32_t node;      // visit index of this daughter's NodeCmd (internal), -1 for a leaf
   uint32_t leaves;   // bit j set: final-state particle j lies below this daughter (inverse mapping)
+  // The daughter's four-momentum as the reference forms it, each branch the sum of its two
+  // daughters (phs.cu:1262-1275), as a program of sum_len 4-bit ops (first op in the lowest
+  // bits) over two accumulators: j < kSumPush adds final-state momentum j to the current
+  // accumulator, kSumPush opens the second one, kSumMerge adds it to the first.  sum_len = 0:
+  // the subtree needs more than two accumulators or sixteen ops, and the momenta of `leaves`
+  // are added in index order instead.
+  int32_t sum_len;
+  int32_t pad1;
+  uint64_t sum_code;
 };
+
+constexpr int kSumPush = 14;
+constexpr int kSumMerge = 15;
+constexpr int kSumMaxOps = 16;
 
 struct NodeCmd {
   int32_t ct_kind;   // CtKind of this branch's cos-theta map

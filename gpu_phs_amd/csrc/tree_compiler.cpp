@@ -221,9 +221,38 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
 
     // ---- per-branch records
     auto mid = [&](int b) { return sp.map_ids[c * n_prt_out + b]; };
+    // sum program of branch k's momentum (see ChildCmd::sum_code): a branch whose daughters
+    // are both internal evaluates the one that needs the second accumulator first, then the
+    // other one in the second accumulator, which it must not need; false if that fails
+    std::function<bool(int)> needs_two = [&](int b) -> bool {
+      return HC[b] && ((HC[D1[b] - 1] && HC[D2[b] - 1]) || needs_two(D1[b] - 1) || needs_two(D2[b] - 1));
+    };
+    std::function<bool(int, int, std::vector<int>&)> sum_ops = [&](int b, int level, std::vector<int>& ops) -> bool {
+      if (!HC[b]) { ops.push_back(ilog2(b + 1)); return true; }
+      int k1 = D1[b] - 1, k2 = D2[b] - 1;
+      if (!HC[k1] || !HC[k2]) {
+        // one or two final-state daughters: the internal one (if any) first, then add
+        const int in = HC[k1] ? k1 : k2, lf = HC[k1] ? k2 : k1;
+        if (!sum_ops(in, level, ops)) return false;
+        ops.push_back(ilog2(lf + 1));
+        return true;
+      }
+      if (level > 0 || (needs_two(k1) && needs_two(k2))) return false;
+      const int first = needs_two(k2) ? k2 : k1, second = first == k1 ? k2 : k1;
+      if (!sum_ops(first, 0, ops)) return false;
+      ops.push_back(kSumPush);
+      if (!sum_ops(second, 1, ops)) return false;
+      ops.push_back(kSumMerge);
+      return true;
+    };
     auto fill_child = [&](int k, ChildCmd& cc) -> std::string {
       cc = ChildCmd();
       cc.leaves = (uint32_t)(k + 1);   // a bincode is the set of final-state particles below it
+      std::vector<int> ops;
+      if (sum_ops(k, 0, ops) && (int)ops.size() <= kSumMaxOps) {
+        cc.sum_len = (int32_t)ops.size();
+        for (int t = 0; t < cc.sum_len; t++) cc.sum_code |= (uint64_t)ops[t] << (4 * t);
+      }
       if (!HC[k]) {
         int j = ilog2(k + 1);
         double mj = sp.flv_masses[sp.n_in + j];

@@ -74,7 +74,11 @@ def test_forward_matches_golden(path):
 
 
 @pytest.mark.parametrize("name", ["c0", "c1", "c2", "2to6"])
-@pytest.mark.parametrize("cfg", [dict(block=256, ept=1), dict(block=64, ept=3), dict(block=128, ept=2, grid=7)])
+@pytest.mark.parametrize("cfg", [dict(block=256, ept=1), dict(block=64, ept=3), dict(block=128, ept=2, grid=7),
+                                 dict(block=64, ept=2, waves_per_eu=3, lds=1),
+                                 dict(block=256, ept=1, waves_per_eu=4, lds=2),
+                                 dict(block=128, ept=2, waves_per_eu=4, lds=3),
+                                 dict(block=64, ept=4, waves_per_eu=4)])
 def test_forward_matches_oracle(name, cfg):
     proc = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6}[name]()
     rng = np.random.default_rng(7)
