@@ -199,7 +199,8 @@ GPHS_HD void ct_eval_adj(int kind, double x, double b0, double b1, const CtEval&
   gx = gtmp * r.tmp * 2 * b1 * sg;          // du/dx = sg
 }
 
-// Composition M <- M L1 (phs_cpu.cpp:314-340), the same expression order as the forward kernel.
+// Composition M <- M L1 (phs_cpu.cpp:314-340), the same expression order and fused products as
+// the forward kernel.
 GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double st, double cp, double sp) {
   const double l01 = -bg * st, l03 = bg * ct;
   const double l11 = ct * cp, l12 = -sp, l13 = st * cp;
@@ -208,10 +209,27 @@ GPHS_HD void compose_l1(double (&M)[16], double g, double bg, double ct, double 
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const double r0 = M[4 * i], r1 = M[4 * i + 1], r2 = M[4 * i + 2], r3 = M[4 * i + 3];
-    M[4 * i + 0] = r0 * g + r3 * bg;
-    M[4 * i + 1] = r0 * l01 + r1 * l11 + r2 * l21 + r3 * l31;
-    M[4 * i + 2] = r1 * l12 + r2 * l22;
-    M[4 * i + 3] = r0 * l03 + r1 * l13 + r2 * l23 + r3 * l33;
+    M[4 * i + 0] = __builtin_fma(r3, bg, r0 * g);
+    M[4 * i + 1] = __builtin_fma(r3, l31, __builtin_fma(r2, l21, __builtin_fma(r1, l11, r0 * l01)));
+    M[4 * i + 2] = __builtin_fma(r2, l22, r1 * l12);
+    M[4 * i + 3] = __builtin_fma(r3, l33, __builtin_fma(r2, l23, __builtin_fma(r1, l13, r0 * l03)));
+  }
+}
+
+// Ma = P L1 out of place, the same operations as compose_l1.
+GPHS_HD void compose_l1_to(const double (&P)[16], double (&Ma)[16], double g, double bg, double ct, double st,
+                           double cp, double sp) {
+  const double l01 = -bg * st, l03 = bg * ct;
+  const double l11 = ct * cp, l12 = -sp, l13 = st * cp;
+  const double l21 = ct * sp, l22 = cp, l23 = st * sp;
+  const double l31 = -g * st, l33 = g * ct;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double r0 = P[4 * i], r1 = P[4 * i + 1], r2 = P[4 * i + 2], r3 = P[4 * i + 3];
+    Ma[4 * i + 0] = __builtin_fma(r3, bg, r0 * g);
+    Ma[4 * i + 1] = __builtin_fma(r3, l31, __builtin_fma(r2, l21, __builtin_fma(r1, l11, r0 * l01)));
+    Ma[4 * i + 2] = __builtin_fma(r2, l22, r1 * l12);
+    Ma[4 * i + 3] = __builtin_fma(r3, l33, __builtin_fma(r2, l23, __builtin_fma(r1, l13, r0 * l03)));
   }
 }
 
@@ -305,8 +323,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       if (!nc.k[c].leaf) continue;
       const double pc = c == 0 ? pk : -pk;
       const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
-      double q0 = M[0] * E + M[3] * pc, q1 = M[4] * E + M[7] * pc, q2 = M[8] * E + M[11] * pc,
-             q3 = M[12] * E + M[15] * pc;
+      double q0 = __builtin_fma(M[3], pc, M[0] * E), q1 = __builtin_fma(M[7], pc, M[4] * E), q2 = __builtin_fma(M[11], pc, M[8] * E),
+             q3 = __builtin_fma(M[15], pc, M[12] * E);
       if (cd.bg_lab != 0) {
         const double b0 = cd.g_lab * q0 + cd.bg_lab * q3, b3 = cd.bg_lab * q0 + cd.g_lab * q3;
         q0 = b0; q3 = b3;
@@ -395,9 +413,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       Ma[8] = 0;   Ma[9] = ce.ct * sp;  Ma[10] = cp; Ma[11] = ce.st * sp;
       Ma[12] = bg; Ma[13] = -g * ce.st; Ma[14] = 0;  Ma[15] = g * ce.ct;
     } else {
-#pragma unroll
-      for (int q = 0; q < 16; q++) Ma[q] = P[q];
-      compose_l1(Ma, g, bg, ce.ct, ce.st, cp, sp);
+      compose_l1_to(P, Ma, g, bg, ce.ct, ce.st, cp, sp);
     }
 
     // final-state daughters: q = Ma[:,0] E + Ma[:,3] pc
@@ -409,7 +425,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       // cotangent of the lab-frame momentum, moved back through the (linear) lab boost
       double gq[4];
       if (loss_mode == 1) {
-        const double q1 = Ma[4] * E + Ma[7] * pc, q2 = Ma[8] * E + Ma[11] * pc;
+        const double q1 = __builtin_fma(Ma[7], pc, Ma[4] * E), q2 = __builtin_fma(Ma[11], pc, Ma[8] * E);
         gq[0] = 0; gq[1] = F * 2 * q1 / s; gq[2] = F * 2 * q2 / s; gq[3] = 0;
       } else if (gp) {
 #pragma unroll
@@ -425,10 +441,10 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       double gE = 0, gpc = 0;
 #pragma unroll
       for (int mu = 0; mu < 4; mu++) {
-        G[4 * mu + 0] += gq[mu] * E;
-        G[4 * mu + 3] += gq[mu] * pc;
-        gE += gq[mu] * Ma[4 * mu + 0];
-        gpc += gq[mu] * Ma[4 * mu + 3];
+        G[4 * mu + 0] = __builtin_fma(gq[mu], E, G[4 * mu + 0]);
+        G[4 * mu + 3] = __builtin_fma(gq[mu], pc, G[4 * mu + 3]);
+        gE = __builtin_fma(gq[mu], Ma[4 * mu + 0], gE);
+        gpc = __builtin_fma(gq[mu], Ma[4 * mu + 3], gpc);
       }
       if (E > 0) gpc += math::quot(gE * pc, E);
       g_pk += c == 0 ? gpc : -gpc;
@@ -436,25 +452,25 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros of L1 only)
     const double l01 = -bg * ce.st, l03 = bg * ce.ct, l11 = ce.ct * cp, l12 = -sp, l13 = ce.st * cp;
     const double l21 = ce.ct * sp, l22 = cp, l23 = ce.st * sp, l31 = -g * ce.st, l33 = g * ce.ct;
-    double gP[16];
+    // gP goes straight to Gc, the adjoint handed to the parent (Gc was consumed above)
     if (i > 0) {   // the root has no parent to hand gP to
 #pragma unroll
       for (int ii = 0; ii < 4; ii++) {
         const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
-        gP[4 * ii + 0] = G0 * g + G1 * l01 + G3 * l03;
-        gP[4 * ii + 1] = G1 * l11 + G2 * l12 + G3 * l13;
-        gP[4 * ii + 2] = G1 * l21 + G2 * l22 + G3 * l23;
-        gP[4 * ii + 3] = G0 * bg + G1 * l31 + G3 * l33;
+        Gc[4 * ii + 0] = __builtin_fma(G3, l03, __builtin_fma(G1, l01, G0 * g));
+        Gc[4 * ii + 1] = __builtin_fma(G3, l13, __builtin_fma(G2, l12, G1 * l11));
+        Gc[4 * ii + 2] = __builtin_fma(G3, l23, __builtin_fma(G2, l22, G1 * l21));
+        Gc[4 * ii + 3] = __builtin_fma(G3, l33, __builtin_fma(G1, l31, G0 * bg));
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < 16; q++) gP[q] = 0;
+      for (int q = 0; q < 16; q++) Gc[q] = 0;
     }
     auto gl = [&](int k, int j) {
       if (i == 0) return G[4 * k + j];   // P = identity
-      double acc = 0;
+      double acc = P[k] * G[j];
 #pragma unroll
-      for (int ii = 0; ii < 4; ii++) acc += P[4 * ii + k] * G[4 * ii + j];
+      for (int ii = 1; ii < 4; ii++) acc = __builtin_fma(P[4 * ii + k], G[4 * ii + j], acc);
       return acc;
     };
     const double g00 = gl(0, 0), g01 = gl(0, 1), g03 = gl(0, 3), g11 = gl(1, 1), g12 = gl(1, 2), g13 = gl(1, 3);
@@ -508,8 +524,6 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       gpar[2 * nc.k[c].node + 1] += gW;
     }
     // hand the adjoints of this branch's inputs to its parent
-#pragma unroll
-    for (int q = 0; q < 16; q++) Gc[q] = gP[q];
     gc_msq = g_a; gc_m = g_mnode; gc_p = g_p0;
     if (i >= 1 && prog[i - 1].desc < 0) {   // reached by a pop in the forward pass
 #pragma unroll
