@@ -563,15 +563,29 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
           lc.waves_per_eu = wpe; lc.lds = lds; lc.grid = 0;
           rc = consider(lc);
         }
-    // stage 2: workgroup size x events per thread
+    // stage 2: workgroup size x events per thread, with the randoms where stage 1 put them
+    // and in the other place (registers or LDS: which wins depends on the shape)
     const LaunchConfig base = best.lc;
     for (int block : {64, 128, 256})
       for (int ept : {1, 2, 4})
-        if (!rc) {
-          LaunchConfig lc = base;
-          lc.block = block; lc.ept = ept;
-          rc = consider(lc);
-        }
+        for (int xl : {0, 2})
+          if (!rc) {
+            LaunchConfig lc = base;
+            lc.block = block; lc.ept = ept; lc.lds = base.lds ^ xl;
+            rc = consider(lc);
+          }
+    // stage 1 again at the chosen workgroup shape, if that changed: the best register budget
+    // and placement depend on it
+    if (best.lc.block != base.block || best.lc.ept != base.ept) {
+      const LaunchConfig shape = best.lc;
+      for (int wpe : {0, 3, 4})
+        for (int lds : {0, 1, 2, 3})
+          if (!rc && (!(lds & 1) || banks) && !(wpe == shape.waves_per_eu && lds == shape.lds)) {
+            LaunchConfig lc = shape;
+            lc.waves_per_eu = wpe; lc.lds = lds;
+            rc = consider(lc);
+          }
+    }
     // stage 3: persistent grids around the best shape
     const LaunchConfig b2 = best.lc;
     for (int k : {2, 4, 8})
