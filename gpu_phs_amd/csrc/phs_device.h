@@ -19,6 +19,9 @@ namespace gphs {
 // lets the compiler emit global_* instead of flat_* memory instructions.
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) uint8_t guint8;
+// 16-byte words of global memory (two doubles)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) dvec2 gdouble2;
 __device__ __forceinline__ gdouble* as_global(double* p) { return (gdouble*)p; }
 __device__ __forceinline__ const gdouble* as_global(const double* p) { return (const gdouble*)p; }
 __device__ __forceinline__ guint8* as_global(uint8_t* p) { return (guint8*)p; }

@@ -109,9 +109,6 @@ __device__ __forceinline__ Bank<KIND> make_bank(double* bank_base) {
 
 extern __shared__ double lds[];
 
-// 16-byte words of global memory (two doubles)
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(1))) dvec2 gdouble2;
 
 // Randoms of one event: with XN > 0 the row is held in XN registers (n_x <= XN) and indexed
 // by the program's wave-uniform indices; with XN = kXLds it is copied into the lane's column
