@@ -70,7 +70,8 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   const size_t b_part = al((size_t)std::max(n_t, 1) * pl.adj_sub * pl.adj_part_stride * sizeof(double));
   const size_t b_list = al(std::max(n_t, 1) * sizeof(int)), b_off = al((n_ch + 1) * sizeof(int));
   const size_t b_ranges = al((size_t)std::max(n_ch, 1) * kAdjRanges * pl.adj_part_stride * sizeof(double));
-  const size_t total = b_segs + b_part + b_list + b_off + b_ranges;
+  const size_t b_done = al((size_t)std::max(n_ch, 1) * sizeof(unsigned int));
+  const size_t total = b_segs + b_part + b_list + b_off + b_ranges + b_done;
   if (total > pl.adj_bytes) {
     if (pl.adj_dev) (void)hipFree(pl.adj_dev);
     pl.adj_dev = nullptr;
@@ -84,7 +85,8 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   if ((e = hipMemcpy(base, pl.adj_segs.data(), n_seg * sizeof(SegDesc), hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(base + b_segs + b_part, list.data(), n_t * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(base + b_segs + b_part + b_list, off.data(), (n_ch + 1) * sizeof(int), hipMemcpyHostToDevice)) !=
-          hipSuccess)
+          hipSuccess ||
+      (e = hipMemset(base + b_segs + b_part + b_list + b_off + b_ranges, 0, b_done)) != hipSuccess)
     return hip_error(e, "hipMemcpy(adjoint tables)");
   pl.adj_dp = pl.dp;
   pl.adj_dp.segs = (const SegDesc*)base;
@@ -92,6 +94,7 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   pl.adj_chan_tiles = (int*)(base + b_segs + b_part);
   pl.adj_chan_off = (int*)(base + b_segs + b_part + b_list);
   pl.adj_ranges = (double*)(base + b_segs + b_part + b_list + b_off);
+  pl.adj_done = (unsigned int*)(base + b_segs + b_part + b_list + b_off + b_ranges);
   pl.adj_valid = true;
   return GPU_PHS_OK;
 }
@@ -145,7 +148,8 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
   // one workgroup per 64-event unit: a persistent grid measured slower (its workgroups do not
   // all fit the LDS at once), and units without events would cost a workgroup each
   hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, loss_mode, pl->adj_part, pl->adj_part_stride, pl->adj_sub,
-                                pl->adj_ranges, pl->adj_chan_tiles, pl->adj_chan_off, (int)pl->chans.size(), gparam,
+                                pl->adj_ranges, pl->adj_done, pl->adj_chan_tiles, pl->adj_chan_off,
+                                (int)pl->chans.size(), gparam,
                                 0, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "adjoint launch");
   return ok_status();

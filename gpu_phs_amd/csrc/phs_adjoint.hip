@@ -121,18 +121,21 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
 }
 
 // gparam[2 * slot + w] = sum over the sub-tiles of the tiles of the slot's channel (every slot
-// is written, channels without events get 0).  Two fixed-order stages, so the result does not
-// depend on scheduling: kSplit workgroups per channel each sum a contiguous range of the
-// channel's units (thread t adds units t, t + 256, ..., then a fixed tree), and one workgroup
-// per channel adds the kSplit range sums in order.
+// is written, channels without events get 0).  kSplit workgroups per channel each sum a
+// contiguous range of the channel's units (thread t adds units t, t + 256, ..., then a fixed
+// tree: a butterfly within each wave, the four waves in order), and the workgroup that finishes
+// its channel's last range adds the kSplit range sums in order.  Every sum has a fixed order,
+// so results do not depend on scheduling.  `done` counts the finished ranges per channel; the
+// last workgroup resets it to 0, so it is 0 again when the launch ends.
 constexpr int kReduceThreads = 256;
 constexpr int kSplit = kAdjRanges;
 constexpr int kMaxParams = 2 * (kMaxOut - 1);
-__global__ __launch_bounds__(kReduceThreads) void param_reduce_ranges(
+__global__ __launch_bounds__(kReduceThreads) void param_reduce(
     const double* __restrict__ gpart, int part_stride, int n_sub, const int* __restrict__ chan_tiles,
     const int* __restrict__ chan_tile_off, const ChannelDesc* __restrict__ chans, int n_chans,
-    double* __restrict__ granges) {
-  __shared__ double red[kReduceThreads];
+    double* __restrict__ granges, unsigned int* __restrict__ done, double* __restrict__ gparam) {
+  __shared__ double red[kReduceThreads / 64][kMaxParams];
+  __shared__ int last;
   const int c = blockIdx.x / kSplit, r = blockIdx.x % kSplit;
   if (c >= n_chans) return;
   const int np = 2 * chans[c].n_nodes;
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kReduceThreads) void param_reduce_ranges(
   const int n_units = (chan_tile_off[c + 1] - t0) * n_sub;
   const int chunk = (n_units + kSplit - 1) / kSplit;
   const int u0 = r * chunk, u1 = min(n_units, u0 + chunk);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double sum[kMaxParams];
 #pragma unroll
   for (int v = 0; v < kMaxParams; v++) sum[v] = 0;
@@ -154,34 +157,33 @@ __global__ __launch_bounds__(kReduceThreads) void param_reduce_ranges(
 #pragma unroll
   for (int v = 0; v < kMaxParams; v++) {
     if (v < np) {   // uniform
-      red[tid] = sum[v];
-      __syncthreads();
-      for (int w = kReduceThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) red[tid] += red[tid + w];
-        __syncthreads();
-      }
-      if (tid == 0) granges[(int64_t)blockIdx.x * part_stride + v] = red[0];
-      __syncthreads();
+      const double t = wave_sum(sum[v]);
+      if (lane == 0) red[wv][v] = t;
     }
   }
-}
-
-__global__ void param_reduce_final(const double* __restrict__ granges, int part_stride,
-                                   const ChannelDesc* __restrict__ chans, int n_chans, double* __restrict__ gparam) {
-  const int c = blockIdx.x;
-  if (c >= n_chans) return;
-  const int v = threadIdx.x;
-  if (v >= 2 * chans[c].n_nodes) return;
-  double sum = 0;
-  for (int r = 0; r < kSplit; r++) sum += granges[((int64_t)c * kSplit + r) * part_stride + v];
-  gparam[2 * chans[c].param0 + v] = sum;
+  __syncthreads();
+  if (tid < np) granges[(int64_t)blockIdx.x * part_stride + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+  // publish this range, then count it; the workgroup that counts the channel's last range
+  // sees every range (device-scope release by the writers, acquire by the last one)
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) last = atomicAdd(&done[c], 1u) == (unsigned int)(kSplit - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (tid < np) {
+    double s = 0;
+    for (int q = 0; q < kSplit; q++) s += granges[((int64_t)c * kSplit + q) * part_stride + tid];
+    gparam[2 * chans[c].param0 + tid] = s;
+  }
+  if (tid == 0) done[c] = 0;
 }
 
 }  // namespace
 
 hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
-                          int n_sub, double* granges, const int* chan_tiles, const int* chan_tile_off, int n_chans,
-                          double* gparam, int grid, hipStream_t stream) {
+                          int n_sub, double* granges, unsigned int* done, const int* chan_tiles,
+                          const int* chan_tile_off, int n_chans, double* gparam, int grid, hipStream_t stream) {
   const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
   if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
   const int64_t units = (int64_t)dp.n_tiles * n_sub;
@@ -196,12 +198,8 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_m
     if (e != hipSuccess) return e;
   }
   if (n_chans > 0) {
-    hipLaunchKernelGGL(param_reduce_ranges, dim3(n_chans * kSplit), dim3(kReduceThreads), 0, stream, gpart,
-                       part_stride, n_sub, chan_tiles, chan_tile_off, dp.chans, n_chans, granges);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(param_reduce_final, dim3(n_chans), dim3(64), 0, stream, granges, part_stride, dp.chans, n_chans,
-                       gparam);
+    hipLaunchKernelGGL(param_reduce, dim3(n_chans * kSplit), dim3(kReduceThreads), 0, stream, gpart, part_stride,
+                       n_sub, chan_tiles, chan_tile_off, dp.chans, n_chans, granges, done, gparam);
   }
   return hipGetLastError();
 }

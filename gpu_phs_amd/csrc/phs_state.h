@@ -67,6 +67,7 @@ struct PlanObj {
   int* adj_chan_tiles = nullptr;
   int* adj_chan_off = nullptr;
   double* adj_ranges = nullptr;
+  unsigned int* adj_done = nullptr;   // per-channel count of finished reduction ranges (0 between launches)
   int adj_sub = 1;            // 64-event sub-tiles per tile (one wavefront each)
   bool adj_valid = false;     // the device copies above match adj_segs
   std::vector<int> chan_process, chan_channel;   // plan channel -> (process handle, channel)
