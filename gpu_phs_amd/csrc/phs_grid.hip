@@ -113,9 +113,8 @@ __global__ __launch_bounds__(kGenLanes) void grid_generate_kernel(const TileDesc
 }
 
 // Histogram of the accepted events' randoms per (channel, dim, bin) and the accepted count per
-// channel: a one-wave workgroup histogram in LDS (its tile is one channel), then one global add
-// per bin.
-__global__ __launch_bounds__(kGenLanes) void grid_histogram_kernel(const TileDesc* __restrict__ tiles, int n_tiles,
+// channel: a workgroup histogram in LDS (its tile is one channel), then one global add per bin.
+__global__ __launch_bounds__(256) void grid_histogram_kernel(const TileDesc* __restrict__ tiles, int n_tiles,
                                                              const SegDesc* __restrict__ segs,
                                                              const ChannelDesc* __restrict__ chans, int n_dim,
                                                              unsigned long long* __restrict__ counts,
@@ -173,7 +172,7 @@ hipError_t launch_grid_histogram(const DevicePlan& dp, int n_chans, int n_dim, u
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || dp.n_tiles <= 0) return e;
   const size_t shmem = ((size_t)n_dim * kGridBins + 1) * sizeof(unsigned int);
-  hipLaunchKernelGGL(grid_histogram_kernel, dim3(dp.n_tiles), dim3(kGenLanes), shmem, stream, dp.tiles, dp.n_tiles,
+  hipLaunchKernelGGL(grid_histogram_kernel, dim3(dp.n_tiles), dim3(256), shmem, stream, dp.tiles, dp.n_tiles,
                      dp.segs, dp.chans, n_dim, counts, n_ok);
   return hipGetLastError();
 }
