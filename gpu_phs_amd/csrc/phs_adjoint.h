@@ -86,6 +86,7 @@ GPHS_HD MapEval msq_eval(const ChildCmd& c, double x, double s) {
 // Adjoint of msq_eval: given gmsq, gf, accumulate d/dx and d/d(M, Gamma) of the map.
 GPHS_HD void msq_eval_adj(const ChildCmd& c, double x, double s, const MapEval& r, double gmsq, double gf, double& gx,
                           double& gM, double& gW) {
+#pragma clang fp contract(fast)
   double ga0 = 0, ga1 = 0, ga2 = 0, gm_direct = 0, gw_direct = 0;
   gx = 0;
   switch (c.kind) {
@@ -181,6 +182,7 @@ GPHS_HD CtEval ct_eval(int kind, double x, double b0, double b1) {
 // Adjoint of ct_eval: d/dx and d/d(b0, b1).
 GPHS_HD void ct_eval_adj(int kind, double x, double b0, double b1, const CtEval& r, double gct, double gst, double gf,
                          double& gx, double& gb0, double& gb1) {
+#pragma clang fp contract(fast)
   gx = 0; gb0 = 0; gb1 = 0;
   if (kind == CK_FLAT) {
     // ct = 2x - 1, st = sqrt(tmp (2 - tmp)), tmp = 2 (1 - x)
@@ -353,6 +355,9 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
   for (int k = 0; k < cd.n_x; k++) gx[k] = 0;
   if (loss_out) *loss_out = ok && loss_mode == 1 ? F * S2 / s : 0.0;
   if (!ok) return false;
+  {
+  // derivatives only from here on: products and sums may fuse (the outputs above are final)
+#pragma clang fp contract(fast)
 
   // ---------------- cotangents: of the factor here, of the momenta per leaf in the sweep
   const double gF = loss_mode == 1 ? S2 / s : gf_in;
@@ -531,6 +536,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       S.st_put(depth, 16, gc_msq); S.st_put(depth, 17, gc_m); S.st_put(depth, 18, gc_p);
       depth++;
     }
+  }
   }
   return true;
 }
