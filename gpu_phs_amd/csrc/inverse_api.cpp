@@ -57,6 +57,13 @@ int inverse_prepare(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows) {
   pl.inv.n_out = cp.spec.n_out;
   pl.inv.n_x = cp.spec.n_x();
   pl.inv.n_banks = cp.max_banks;
+  int root_bank = cp.max_banks <= 1 ? 1 : 0;
+  for (const ChannelDesc& c : cp.channels)
+    for (int i = 0; i < c.n_nodes; i++) {
+      const NodeCmd& nd = cp.nodes[c.node0 + i];
+      if ((i > 0 && nd.push >= 0) || (i == 0 && nd.friend_sign != 0)) root_bank = 0;
+    }
+  pl.inv.root_bank = root_bank;
   pl.inv.has_friends = cp.has_friends ? 1 : 0;
   pl.inv.per_event_beams = pl.dp.per_event_beams;
   pl.inv_segs = std::move(is);
@@ -74,7 +81,7 @@ int gphs::plan_inverse(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows, dou
   if (block < 64 || block > 256 || block % 64) return set_error(GPU_PHS_E_ARG, "inverse: block must be 64, 128, 192 or 256");
   int rc = inverse_prepare(pl, seg_row0, n_rows);
   if (rc) return rc;
-  if (inverse_lds_bytes(block, pl.inv.n_out, pl.inv.n_banks) > forward_lds_limit())
+  if (inverse_lds_bytes(block, pl.inv.n_out, pl.inv.n_banks, pl.inv.root_bank) > forward_lds_limit())
     return set_error(GPU_PHS_E_ARG, "inverse: the trees need more LDS than a workgroup of this size may use");
   hipError_t e = launch_inverse(pl.inv, pl.dp.tiles, pl.dp.n_tiles, pl.dp.segs, block, 0, factors, x_out, n_rows, stream);
   if (e != hipSuccess) return hip_error(e, "inverse launch");

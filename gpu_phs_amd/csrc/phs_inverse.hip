@@ -29,6 +29,11 @@ namespace {
 
 constexpr double kTwoPi = 6.28318530717958647693;   // mappings.h:5
 constexpr int kInvBankWords = 19;
+// With RB (every deferred subtree is a daughter of the root, and no root has a friend) a bank
+// entry is the root's cos theta, sin theta, cos phi, sin phi and the subtree's msq, m and
+// p_decay: the root's G is its rotation applied to the identity (the root's bg is 0), so the
+// pop re-forms it bit for bit from the four numbers.
+constexpr int kInvRootBankWords = 7;
 
 extern __shared__ double inv_lds[];
 
@@ -136,9 +141,21 @@ __device__ __forceinline__ void branch_sum(const double* __restrict__ pl, int S,
   q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; q[3] = a[3];
 }
 
+// G <- R_y(-theta) R_z(-phi) G (rows 1..3)
+__device__ __forceinline__ void rotate_back(double (&G)[16], double ct, double st, double cp, double sp) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double w1 = G[4 + j], w2 = G[8 + j], w3 = G[12 + j];
+    const double t1 = cp * w1 + sp * w2;
+    G[4 + j] = ct * t1 - st * w3;
+    G[8 + j] = -sp * w1 + cp * w2;
+    G[12 + j] = st * t1 + ct * w3;
+  }
+}
+
 // Inverse of one event into channel `cd` (program `prog`): the randoms go to xo[0..n_x),
 // returns the factor.
-template <bool FR, bool PEV>
+template <bool FR, bool PEV, bool RB>
 __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvBeams& eb,
                                                 const NodeCmd* __restrict__ prog, const double* __restrict__ pl,
                                                 int S, double* bank, int B, gdouble* __restrict__ xo) {
@@ -218,25 +235,25 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvB
     x_from_ct(nc.ct_kind, ct, cb0, cb1, xc, fct);
     xo[nc.x_phi + 1] = xc;
     factor *= fct;
-    // G <- R_y(-theta) R_z(-phi) G (rows 1..3)
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const double w1 = G[4 + j], w2 = G[8 + j], w3 = G[12 + j];
-      const double t1 = cp * w1 + sp * w2;
-      G[4 + j] = ct * t1 - st * w3;
-      G[8 + j] = -sp * w1 + cp * w2;
-      G[12 + j] = st * t1 + ct * w3;
-    }
+    rotate_back(G, ct, st, cp, sp);
 
     // next branch: a daughter (G stays), or the deferred one from the save bank
     if (nc.push >= 0) {
-      double* b = bank + kInvBankWords * depth * B;
-#pragma unroll
-      for (int j = 0; j < 16; j++) b[j * B] = G[j];
       const double mp = nc.push == 0 ? msq0 : msq1;
-      b[16 * B] = mp;
-      b[17 * B] = math::sqrt_phys(mp);
-      b[18 * B] = nc.push == 0 ? pk : -pk;
+      if constexpr (RB) {   // only the root pushes
+        double* b = bank;
+        b[0] = ct; b[B] = st; b[2 * B] = cp; b[3 * B] = sp;
+        b[4 * B] = mp;
+        b[5 * B] = math::sqrt_phys(mp);
+        b[6 * B] = nc.push == 0 ? pk : -pk;
+      } else {
+        double* b = bank + kInvBankWords * depth * B;
+#pragma unroll
+        for (int j = 0; j < 16; j++) b[j * B] = G[j];
+        b[16 * B] = mp;
+        b[17 * B] = math::sqrt_phys(mp);
+        b[18 * B] = nc.push == 0 ? pk : -pk;
+      }
       depth++;
     }
     if (nc.desc >= 0) {
@@ -245,12 +262,22 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvB
       cur_p = nc.desc == 0 ? pk : -pk;
     } else if (depth > 0) {
       depth--;
-      const double* b = bank + kInvBankWords * depth * B;
+      if constexpr (RB) {
+        const double* b = bank;
 #pragma unroll
-      for (int j = 0; j < 16; j++) G[j] = b[j * B];
-      cur_msq = b[16 * B];
-      cur_m = b[17 * B];
-      cur_p = b[18 * B];
+        for (int j = 0; j < 16; j++) G[j] = (j % 5 == 0) ? 1.0 : 0.0;
+        rotate_back(G, b[0], b[B], b[2 * B], b[3 * B]);
+        cur_msq = b[4 * B];
+        cur_m = b[5 * B];
+        cur_p = b[6 * B];
+      } else {
+        const double* b = bank + kInvBankWords * depth * B;
+#pragma unroll
+        for (int j = 0; j < 16; j++) G[j] = b[j * B];
+        cur_msq = b[16 * B];
+        cur_m = b[17 * B];
+        cur_p = b[18 * B];
+      }
     }
   }
   return factor;
@@ -258,7 +285,7 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvB
 
 // LDS per workgroup: the momenta column (4 n_out rows of blockDim.x + 1 doubles), then the
 // save banks (19 n_banks rows of blockDim.x doubles).
-template <bool FR, bool PEV>
+template <bool FR, bool PEV, bool RB>
 __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict__ tiles, int n_tiles,
                                                       const SegDesc* __restrict__ segs,
                                                       const InvSeg* __restrict__ isegs,
@@ -306,7 +333,7 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
           continue;
         }
         const ChannelDesc cd = chans[c];
-        if (valid) as_global(factors)[row * n_ch + c] = inverse_event<FR, PEV>(cd, eb, nodes + cd.node0, pl, S, bank, B, xo);
+        if (valid) as_global(factors)[row * n_ch + c] = inverse_event<FR, PEV, RB>(cd, eb, nodes + cd.node0, pl, S, bank, B, xo);
       }
     }
   }
@@ -314,19 +341,26 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
 
 }  // namespace
 
-size_t inverse_lds_bytes(int block, int n_out, int n_banks) {
-  return ((size_t)4 * n_out * (block + 1) + (size_t)kInvBankWords * n_banks * block) * sizeof(double);
+size_t inverse_lds_bytes(int block, int n_out, int n_banks, int root_bank) {
+  const size_t words = root_bank ? kInvRootBankWords : kInvBankWords;
+  return ((size_t)4 * n_out * (block + 1) + words * n_banks * block) * sizeof(double);
 }
 
 hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
                           int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream) {
   if (n_tiles <= 0) return hipSuccess;
   if (block < 64 || block > 256 || block % 64) return hipErrorInvalidValue;
-  const size_t shmem = inverse_lds_bytes(block, it.n_out, it.n_banks);
+  const size_t shmem = inverse_lds_bytes(block, it.n_out, it.n_banks, it.root_bank);
   if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
   const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
-  auto k = it.per_event_beams ? (it.has_friends ? inverse_kernel<true, true> : inverse_kernel<false, true>)
-                               : (it.has_friends ? inverse_kernel<true, false> : inverse_kernel<false, false>);
+  using K = decltype(&inverse_kernel<false, false, false>);
+  K k;
+  if (it.root_bank)
+    k = it.per_event_beams ? (it.has_friends ? inverse_kernel<true, true, true> : inverse_kernel<false, true, true>)
+                           : (it.has_friends ? inverse_kernel<true, false, true> : inverse_kernel<false, false, true>);
+  else
+    k = it.per_event_beams ? (it.has_friends ? inverse_kernel<true, true, false> : inverse_kernel<false, true, false>)
+                           : (it.has_friends ? inverse_kernel<true, false, false> : inverse_kernel<false, false, false>);
   hipLaunchKernelGGL(k, dim3(g), dim3(block), shmem, stream, tiles, n_tiles, segs, it.isegs, it.chans, it.nodes,
                      it.n_ch, it.n_out, it.n_x, n_rows, factors, x_out);
   return hipGetLastError();

@@ -70,8 +70,9 @@ struct InverseTables {
   const ChannelDesc* chans = nullptr;
   const NodeCmd* nodes = nullptr;
   int n_ch = 0, n_out = 0, n_x = 0, n_banks = 0, has_friends = 0, per_event_beams = 0;
+  int root_bank = 0;   // every deferred subtree is a daughter of the root and no root has a friend
 };
-size_t inverse_lds_bytes(int block, int n_out, int n_banks);
+size_t inverse_lds_bytes(int block, int n_out, int n_banks, int root_bank);
 // factors [n_rows][n_ch] and x_out [n_ch][n_rows][n_x] for the events of tiles [0, n_tiles).
 hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
                           int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream);
