@@ -17,7 +17,8 @@
 // angles of the first daughter in B_z(-bg) G, where G = L_parent^-1 is carried down the tree
 // (a branch with a friend measures them from the friend beam), x_phi = atan2 / 2 pi, x of
 // cos theta from its map, and G <- R_y(-theta) R_z(-phi) B_z(-bg) G for the daughters.
-// Deferred subtrees wait in an LDS save bank (G and the subtree's msq, m, p_decay).
+// Deferred subtrees wait in an LDS save bank (G and the subtree's msq, m, p_decay; the root's
+// four angle values instead of G where only roots defer).
 #include <hip/hip_runtime.h>
 
 #include "phs_device.h"
