@@ -96,6 +96,35 @@ def process_2to6(n_channels: int = 16) -> Process:
                         name="2->6: six 173 GeV finals")
 
 
+def process_large(n_out: int = 8, n_channels: int = 12) -> Process:
+    """Test process with more final-state particles than the benchmarks: seeded random trees
+    plus the chain and the balanced tree, 10 GeV finals at sqrt(s) = 4 TeV, internal maps
+    drawn from {collinear, infrared, t-channel, Breit-Wigner, step E/H}.  Exercises save banks up to
+    three levels deep, randoms in LDS (n_x > 16) and long daughter sum programs."""
+    rng = np.random.default_rng(seed(7) + n_out)
+    leaves = list(range(n_out))
+    chain = leaves[-1]
+    for j in reversed(leaves[:-1]):
+        chain = (j, chain)
+    def balanced(ls):
+        return ls[0] if len(ls) == 1 else (balanced(ls[:len(ls) // 2]), balanced(ls[len(ls) // 2:]))
+    trees = [chain, balanced(leaves)]
+    while len(trees) < n_channels:
+        trees.append(random_tree(rng, [int(v) for v in rng.permutation(n_out)]))
+    # maps that favour light inner branches, so that a fair share of deep trees is kinematically
+    # allowed (each branch must be heavier than its daughters)
+    choices = [(MAP_COLLINEAR, 10.0, 0.0), (MAP_INFRARED, 10.0, 0.0), (MAP_TCHANNEL, 50.0, 0.0),
+               (MAP_SCHANNEL, 120.0, 5.0), (MAP_STEP_E, 150.0, 10.0), (MAP_STEP_H, 150.0, 10.0)]
+    maps = []
+    for t in trees:
+        m = {"root": ((MAP_NO, MAP_TCHANNEL)[int(rng.integers(2))], 50.0, 0.0)}
+        for node in internal_nodes(t)[1:]:
+            m[node] = choices[int(rng.integers(len(choices)))]
+        maps.append(m)
+    return make_process(2, n_out, trees, maps, [M_E, M_E] + [10.0] * n_out, 4000.0,
+                        name=f"2->{n_out}: 10 GeV finals")
+
+
 def process_friends(n_out: int = 4) -> Process:
     """Trees whose branches take their polar axis from an incoming beam (friends), at
     sqrt(s) = 1 TeV: friend signs -1 and +1, friends on the root and on inner branches, flat

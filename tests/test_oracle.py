@@ -18,7 +18,8 @@ from gpu_phs_amd import workloads as W
 from refio import read_ref
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-PROCS = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6}
+PROCS = {"c0": W.process_c0, "c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6,
+         "2to8": lambda: W.process_large(8)}
 
 
 def load_golden(path):
