@@ -180,7 +180,9 @@ typedef struct {
  * the loss is sum over ok events of factor * sum_j (p_jx^2 + p_jy^2) / s, and g_p, g_f are
  * ignored.  Writes g_x, loss, the forward outputs when write_forward != 0, and gparam
  * [n_params]: d loss / d M at 2 slot, d loss / d Gamma at 2 slot + 1, per (channel, branch)
- * slot of gpu_phs_plan_param_map.  Trees up to 2 -> 6. */
+ * slot of gpu_phs_plan_param_map.  Trees up to 2 -> 6.  The plan owns the partial sums and
+ * per-channel completion counters of the reduction: adjoint calls on one plan must not overlap
+ * (queue them on one stream). */
 int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adjoint, int loss_mode, int write_forward,
                          double* gparam, void* stream);
 /* 2 x the number of (channel, branch) parameter slots of a plan. */
