@@ -91,6 +91,12 @@ def test_adjoint_matches_dual_numbers(name, loss_mode):
         kw = dict(g_p=list(torch.split(gpt, list(counts))), g_f=list(torch.split(gft, list(counts))))
     gparam = plan.adjoint(gx, loss=loss, loss_mode=loss_mode, **kw)
     torch.cuda.synchronize()
+    # a second call gives the same bits: the reduction's order is fixed and its per-channel
+    # completion counters are back at 0 after every launch
+    first = gparam.clone()
+    gparam = plan.adjoint(gx, loss=loss, loss_mode=loss_mode, **kw)
+    torch.cuda.synchronize()
+    assert torch.equal(first, gparam)
     g_gpu = torch.cat(gx).cpu().numpy()
     g_ref, ref = _dual_grads(proc, x, ch, gp, gf, loss_mode)
     ok = ref["ok"]
