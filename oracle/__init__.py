@@ -51,6 +51,8 @@ def lib():
         _lib.oracle_map_ct.restype = None
         _lib.oracle_map_msq.argtypes = [_I, _D, _D, _D, _D, _D, _D, _P, _P, _P]
         _lib.oracle_map_ct.argtypes = [_I, _D, _P, _P, _P, _P]
+        _lib.oracle_set_perturbation.restype = None
+        _lib.oracle_set_perturbation.argtypes = [ctypes.c_uint64]
     return _lib
 
 
@@ -77,12 +79,13 @@ def _tree_args(proc):
     return [d1, d2, hc, mid, mm, mw, flv]
 
 
-def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
+def forward(proc, x, channels, E1=None, E2=None, want_msq=False, perturb=0):
     """Oracle forward pass: returns dict(p, factor, volume, ok, scale, cond[, msq]); ``scale`` is
     the componentwise forward-error scale |L1|...|L1||v| of every momentum component, ``cond``
     the event's condition number (lambda near a threshold, sin theta of peaked maps, the
     friend axis).  Branches with a friend (``proc.friends``) take their polar axis from the
-    incoming beam."""
+    incoming beam.  ``perturb`` != 0 moves every elementary-function result by up to one ulp
+    along that seed's per-event streams (see ``sensitivity``)."""
     x = np.ascontiguousarray(x, np.float64)
     ch = np.ascontiguousarray(channels, np.int32)
     n = ch.shape[0]
@@ -100,16 +103,51 @@ def forward(proc, x, channels, E1=None, E2=None, want_msq=False):
     fr = None
     if proc.friends is not None and np.any(proc.friends):
         fr = np.ascontiguousarray(proc.friends, np.int32)
-    rc = lib().oracle_forward_friends(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
-                                      *[_ptr(a) for a in t[:3]], _ptr(fr), *[_ptr(a) for a in t[3:]],
-                                      _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n), _ptr(ch),
-                                      _I(proc.n_x), _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok),
-                                      _ptr(msq), _ptr(scale), _ptr(cond))
+    lib().oracle_set_perturbation(ctypes.c_uint64(perturb))
+    try:
+        rc = lib().oracle_forward_friends(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
+                                          *[_ptr(a) for a in t[:3]], _ptr(fr), *[_ptr(a) for a in t[3:]],
+                                          _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n), _ptr(ch),
+                                          _I(proc.n_x), _ptr(x), _ptr(p), _ptr(f), _ptr(v), _ptr(ok),
+                                          _ptr(msq), _ptr(scale), _ptr(cond))
+    finally:
+        lib().oracle_set_perturbation(ctypes.c_uint64(0))
     if rc:
         raise ValueError(f"oracle_forward failed ({rc})")
     out = dict(p=p, factor=f, volume=v, ok=ok.astype(bool), scale=scale, cond=cond)
     if want_msq:
         out["msq"] = msq
+    return out
+
+
+def sensitivity(proc, x, channels, E1=None, E2=None, runs=6, base=None, want_msq=False):
+    """Measured one-ulp sensitivity of every event: the oracle is run ``runs`` times with its
+    elementary-function results (sqrt, exp, log, sin, cos, tan, atan) moved by -1, 0 or +1 ulp
+    along independent seeded streams, and compared with the unperturbed run ``base`` (computed
+    if not given).  Returns dict(p [n][n_out][4], factor [n]: the largest absolute change of
+    each output over the runs; ok_flip [n]: the ok flag changed in some run, i.e. the event sits
+    within rounding of a kinematic cut, and its outputs cannot be compared across the runs:
+    their sensitivity is reported as infinite; with ``want_msq`` also msq [n][n_prt_out])."""
+    if base is None or (want_msq and "msq" not in base):
+        base = forward(proc, x, channels, E1=E1, E2=E2, want_msq=want_msq)
+    sp = np.zeros_like(base["p"])
+    sf = np.zeros_like(base["factor"])
+    sm = np.zeros_like(base["msq"]) if want_msq else None
+    flip = np.zeros(base["ok"].shape, bool)
+    for k in range(runs):
+        o = forward(proc, x, channels, E1=E1, E2=E2, want_msq=want_msq, perturb=0x5EED0000 + 7919 * (k + 1))
+        flip |= o["ok"] != base["ok"]
+        both = (o["ok"] & base["ok"])
+        sp = np.maximum(sp, np.where(both[:, None, None], np.abs(o["p"] - base["p"]), 0.0))
+        sf = np.maximum(sf, np.where(both, np.abs(o["factor"] - base["factor"]), 0.0))
+        if want_msq:
+            sm = np.maximum(sm, np.where(both[:, None], np.abs(o["msq"] - base["msq"]), 0.0))
+    sp[flip] = np.inf
+    sf[flip] = np.inf
+    out = dict(p=sp, factor=sf, ok_flip=flip)
+    if want_msq:
+        sm[flip] = np.inf
+        out["msq"] = sm
     return out
 
 

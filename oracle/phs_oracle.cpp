@@ -86,7 +86,43 @@ inline Dual atan(const Dual& a) { return chain(a, std::atan(a.v), 1.0 / (1.0 + a
 inline double val(const Dual& a) { return a.v; }
 inline double val(double a) { return a; }
 
-using std::sqrt; using std::exp; using std::log; using std::sin; using std::cos; using std::tan; using std::atan;
+// Measured sensitivity of the algorithm (used by the parity tests): with a non-zero per-event
+// stream (oracle_set_perturbation), every elementary-function result of the double path --
+// sqrt, exp, log, sin, cos, tan, atan -- moves by -1, 0 or +1 ulp, drawn from a splitmix64
+// sequence.  Two correct FP64 implementations may differ by that much in each of these
+// functions, so the spread of the outputs over a few streams measures how far such
+// implementations can legitimately drift apart at each event.  Zero, infinite and NaN results
+// are left alone (a zero mass stays exactly zero), and so are exact results, which every
+// correct implementation returns: a square root whose square is its argument, the root's mass
+// sqrt(s) (set_msq), and every function at argument 0.  Constants of fixed beams are evaluated once, outside every stream,
+// and never moved (the HIP path takes them from the host as they are); with per-event beams
+// they are evaluated inside the event's stream (the HIP path evaluates them on the device).
+// With the stream at 0 (the default) every function is the plain libm call, bit for bit.
+thread_local uint64_t g_pert = 0;
+uint64_t g_pert_seed = 0;
+inline uint64_t splitmix64(uint64_t& st) {
+  uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline double pert(double r) {
+  if (!g_pert || r == 0 || !std::isfinite(r)) return r;
+  const unsigned k = (unsigned)(splitmix64(g_pert) % 3);
+  return k == 0 ? r : std::nextafter(r, k == 1 ? HUGE_VAL : -HUGE_VAL);
+}
+inline double sqrt(double a) {
+  const double r = std::sqrt(a);
+  return std::fma(r, r, -a) == 0 ? r : pert(r);
+}
+inline double sqrt_unperturbed(double a) { return std::sqrt(a); }
+inline Dual sqrt_unperturbed(const Dual& a) { return sqrt(a); }
+inline double exp(double a) { return a == 0 ? 1.0 : pert(std::exp(a)); }
+inline double log(double a) { return pert(std::log(a)); }
+inline double sin(double a) { return a == 0 ? a : pert(std::sin(a)); }
+inline double cos(double a) { return a == 0 ? 1.0 : pert(std::cos(a)); }
+inline double tan(double a) { return a == 0 ? a : pert(std::tan(a)); }
+inline double atan(double a) { return a == 0 ? a : pert(std::atan(a)); }
 
 template <class T> T tmax(const T& a, double b) { return val(a) < b ? T(b) : a; }  // std::max(a, b) keeps a on ties
 inline double pow2(double q) { return std::pow(q, 2); }  // the reference's pow(q, 2)
@@ -321,7 +357,9 @@ void set_msq(const Proc& P, const MapPar<T>& M, int c, int b, double m_tot, cons
   }
   if (*ok) {
     T this_msq = E.p2[b], msq1 = E.p2[k1], msq2 = E.p2[k2];
-    T m1 = sqrt(msq1), m2 = sqrt(msq2), m = sqrt(this_msq);
+    // the root's mass: sqrt(s) of s = sqrt(s)^2, which a correctly rounded square root returns
+    // exactly and the cut m <= sqrt(s) below relies on; it is never perturbed (see pert())
+    T m1 = sqrt(msq1), m2 = sqrt(msq2), m = b == P.root ? sqrt_unperturbed(this_msq) : sqrt(this_msq);
     T lda = (this_msq - msq1 - msq2) * (this_msq - msq1 - msq2) - 4 * msq1 * msq2;
     // Condition number of lambda: the terms it is formed from over lambda itself.  Near a
     // threshold (lambda -> 0) an ulp of difference in any msq moves sqrt(lambda), the factor
@@ -836,6 +874,10 @@ int oracle_forward_friends(int n_in, int n_out, int n_channels, const int* d1, c
   for (long i = 0; i < n_events; i++) {
     int c = channels[i];
     if (c < 0 || c >= n_channels) return 3;
+    if (g_pert_seed) {   // this event's perturbation stream (per-event beams: their constants too)
+      uint64_t st = g_pert_seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(i + 1));
+      g_pert = splitmix64(st) | 1;
+    }
     double s_ev = sqrts, bg_lab = 0, s_friend = sqrts * sqrts;
     if (E1 && E2) {
       s_friend = 4 * E1[i] * E2[i];
@@ -853,10 +895,14 @@ int oracle_forward_friends(int n_in, int n_out, int n_channels, const int* d1, c
     bool ok = one_event<double>(P, M, c, s_ev, bg_lab, s_friend, &x[(size_t)n_x * i], E, &f, &v, &p[4 * n_out * i],
                                 msq ? &msq[(size_t)P.n_prt_out * i] : nullptr,
                                 scale ? &scale[4 * n_out * i] : nullptr, cond ? &cond[i] : nullptr);
+    g_pert = 0;
     factors[i] = f; volumes[i] = v; oks[i] = ok ? 1 : 0;
   }
   return 0;
 }
+
+// Perturbation streams of oracle_forward (see pert()): 0 turns them off.
+void oracle_set_perturbation(uint64_t seed) { g_pert_seed = seed; }
 
 // Forward-mode derivatives (dual numbers) of the same generation, K <= 24 tangent
 // directions: x_tan [n_events][n_x][K], m_tan / w_tan [n_channels][n_prt_out][K] seed the

@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 from gpu_phs_amd import workloads as W
-from parity import assert_parity, compare, threshold_margin
+from parity import assert_parity, forward_check
 from test_gpu_forward import load_golden
 from test_gpu_inverse import assert_inverse_parity, inverse_stats, sensitivity
 
@@ -67,11 +67,7 @@ def test_per_event_beams_match_oracle(name):
     E1, E2 = _beams(proc, n, rng)
     print("per-event beams", name, flush=True)
     out, plan, _, _ = run_beams(proc, x, ch, E1, E2)
-    ref = oracle.forward(proc, x, ch, E1=E1, E2=E2, want_msq=True)
-    near = threshold_margin(proc, ref["msq"], ch, 2 * np.sqrt(E1 * E2)) < 1e-12
-    r = compare(out, ref, near_threshold=near)
-    print(r)
-    assert_parity(r)
+    assert_parity(forward_check(proc, x, ch, out, E1=E1, E2=E2))
 
 
 def test_per_event_beams_inverse():

@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from gpu_phs_amd import workloads as W
-from parity import assert_parity, compare, threshold_margin
+from parity import assert_parity, forward_check
 from test_gpu_forward import run_gpu
 
 pytestmark = pytest.mark.gpu
@@ -29,13 +29,8 @@ def test_friend_matches_oracle(n_out, beams, cfg):
         E1, E2 = 0.6 * proc.sqrts, proc.sqrts ** 2 / (4 * 0.6 * proc.sqrts)
     print("friend parity", n_out, beams, cfg, flush=True)
     out, _, _ = run_gpu(proc, x, ch, E1=E1, E2=E2, **cfg)
-    e = None if E1 is None else (np.full(len(ch), E1), np.full(len(ch), E2))
-    ref = oracle.forward(proc, x, ch, E1=None if e is None else e[0], E2=None if e is None else e[1],
-                         want_msq=True)
-    near = threshold_margin(proc, ref["msq"], ch, proc.sqrts) < 1e-12
-    r = compare(out, ref, near_threshold=near)
-    print(r)
-    assert_parity(r)
+    e = (None, None) if E1 is None else (np.full(len(ch), E1), np.full(len(ch), E2))
+    assert_parity(forward_check(proc, x, ch, out, E1=e[0], E2=e[1]))
 
 
 def test_root_friend_along_plus_z_equals_plain():

@@ -9,7 +9,7 @@ import pytest
 import gpu_phs_amd as G
 import oracle
 from gpu_phs_amd import workloads as W
-from parity import assert_parity, compare
+from parity import assert_parity, forward_check
 from refio import read_ref
 from test_gpu_inverse import assert_inverse_parity, inverse_stats, sensitivity
 
@@ -77,11 +77,8 @@ def test_whizard_abi_matches_oracle():
     x = rng.random((n, proc.n_x))
     p, factors, volumes, oks, x_out = gen(L, proc, x, counts, 250.0, 250.0)
     ch = np.repeat(np.arange(3), counts).astype(np.int32)
-    ref = oracle.forward(proc, x, ch)
     own = factors[np.arange(n), ch]
-    r = compare(dict(p=p, factor=own, volume=volumes, ok=oks), ref)
-    print(r)
-    assert_parity(r)
+    assert_parity(forward_check(proc, x, ch, dict(p=p, factor=own, volume=volumes, ok=oks)))
     # every other channel's factor and randoms: the inverse mapping of the event's momenta
     for c in range(3):
         sel = ch == c
@@ -105,9 +102,7 @@ def test_whizard_abi_interior_empty_channel():
     x = np.random.default_rng(5).random((n, proc.n_x))
     p, factors, volumes, oks, _ = gen(L, proc, x, counts, 250.0, 250.0)
     ch = np.repeat(np.arange(3), counts).astype(np.int32)
-    ref = oracle.forward(proc, x, ch)
-    r = compare(dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks), ref)
-    assert_parity(r)
+    assert_parity(forward_check(proc, x, ch, dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks)))
     L.c_whizard_finalize_gpu_phs()
 
 
@@ -120,9 +115,8 @@ def test_whizard_abi_asymmetric_beams_lab_boost():
     E1, E2 = 300.0, 200.0
     p, factors, volumes, oks, _ = gen(L, proc, x, counts, E1, E2)
     ch = np.repeat(np.arange(3), counts).astype(np.int32)
-    ref = oracle.forward(proc, x, ch, E1=np.full(n, E1), E2=np.full(n, E2))
-    r = compare(dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks), ref)
-    assert_parity(r)
+    assert_parity(forward_check(proc, x, ch, dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks),
+                                E1=np.full(n, E1), E2=np.full(n, E2)))
     tot = p[oks].sum(axis=1)
     assert np.allclose(tot[:, 0], E1 + E2) and np.allclose(tot[:, 3], E1 - E2, atol=1e-9)
     L.c_whizard_finalize_gpu_phs()
@@ -170,9 +164,8 @@ def test_whizard_abi_per_event_beams():
                                    _p(x_out))
     assert L.gpu_phs_last_error() == 0, L.gpu_phs_last_error_message()
     factors = factors.reshape(n, nc)
-    ref = oracle.forward(proc, x, ch, E1=E1, E2=E2)
-    r = compare(dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks), ref)
-    assert_parity(r)
+    assert_parity(forward_check(proc, x, ch, dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks),
+                                E1=E1, E2=E2))
     p_ok = np.where(oks[:, None, None], p, 0.0)
     f_ref, x_ref = oracle.inverse(proc, p_ok, E1=E1, E2=E2)
     df, dx = inverse_stats(proc, factors, x_out.reshape(nc, n, proc.n_x), f_ref, x_ref, ch, oks)
