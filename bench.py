@@ -1,13 +1,15 @@
 """Benchmark: events/s of the phase-space generator on one MI355X (FP64).
 
-Default workload: the 2->4 double-resonance process (three channels, 1e7 events, eight
-FP64 randoms per event) run as one batched launch over all its channels.  One step = one
-forward pass over all 1e7 events, randoms resident in HBM, outputs (momenta, factor, volume,
-ok) written to HBM.  The JSON line also carries the plain per-channel launches and the
-batched uneven mix as extra fields, the roofline of the dominant kernel and the reference's
-own CPU algorithm timed on the host.
+Default workload: the batched uneven mix (every channel of the 2->2, 2->4 double-resonance and
+120-channel 2->4 processes plus sixteen 2->6 trees, Dirichlet(0.5) event counts, 5e7 events),
+the largest configuration, run as the plan's batched launches.  One step = one forward pass over
+all 5e7 events, randoms resident in HBM, outputs (momenta, factor, volume, ok) written to HBM.
+The JSON line also carries the plain per-segment launches, the other workloads, the adjoint,
+the inverse mapping and the Whizard entry point as extra fields, the HBM and FP64 rooflines of
+the forward kernels and the reference's own CPU algorithm timed on the host's cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1] [--no-extra]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--no-extra]
+                    [--tuned '<json of Plan.tuning()>' | --config block,ept,grid,order,wpe,lds]
 """
 from __future__ import annotations
 
@@ -23,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector, dense
 
 
 def log(*a):
@@ -59,13 +62,20 @@ def time_steps(fn, steps, warmup, torch):
     return wall, start.elapsed_time(end) / 1e3
 
 
-def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None):
+def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None, tuning=None, specialize=True):
+    """fixed: one launch shape for both tile classes; tuning: a Plan.tuning() dict to apply;
+    neither: the autotuner chooses.  specialize=False: the generic kernel for every tree."""
     wl = W.workload(name)
     handles, segs = make_inputs(wl, torch, G)
     plan = G.Plan(segs)
+    if not specialize:
+        plan.set_specialize(False)
     if fixed:
         plan.set_config(**fixed)
         tuned = ""
+    elif tuning:
+        plan.apply_tuning(tuning)
+        tuned = "applied: " + json.dumps(tuning)
     else:
         tuned = plan.autotune(reps=3) if tune else ""
     torch.cuda.synchronize()
@@ -82,8 +92,8 @@ def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None):
     res = dict(workload=name, n_events=n, n_segments=len(segs),
                batched_ev_s=n * steps / wall, batched_kernel_ms=ev_s / steps * 1e3,
                plain_ev_s=n * max(1, steps // 2) / plain_wall, plain_ms=plain_ev / max(1, steps // 2) * 1e3,
-               algorithmic_bytes=wl.algorithmic_bytes(), config=plan.config(), tune=tuned[:200],
-               wall_s=wall, event_s=ev_s, launch=launch)
+               algorithmic_bytes=wl.algorithmic_bytes(), config=plan.config(), tuning=plan.tuning(),
+               tune=tuned[:200], wall_s=wall, event_s=ev_s, launch=launch)
     res["batched_vs_plain"] = res["batched_ev_s"] / res["plain_ev_s"]
     return res, plan, segs, wl
 
@@ -241,70 +251,84 @@ def run_whizard_abi(n, reps, G, W):
                 host_gb_s=host_bytes / dt / 1e9)
 
 
-def cpu_baseline(wl_name, W):
-    """The reference's own phs_cpu.cpp (oracle/_ref build) on one host core, over a bounded
-    sample of the workload's events (repeated until ~10 s of CPU work)."""
-    import numpy as np
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on
+    (sched_getaffinity), limited by the cgroup CPU quota where one is set and by 16, the CPU share
+    of one GPU on a shared GPU host.  Returns (threads, affinity cores, quota cores or None)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, -(-int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return max(1, min(aff, quota or aff, 16)), aff, quota
+
+
+def _process_samples(wl):
+    """Per process of a workload: (process, channels, randoms) of its segments, randoms seeded."""
+    rng = np.random.default_rng(wl.seed)
+    out = []
+    for pi, proc in enumerate(wl.processes):
+        ch = np.concatenate([np.full(s.n_events, s.channel, np.int32) for s in wl.segments if s.process == pi] or
+                            [np.zeros(0, np.int32)])
+        if ch.size:
+            out.append((proc, ch, rng.random((ch.shape[0], proc.n_x))))
+    return out
+
+
+def cpu_baseline(wl_name, W, threads, scale, budget_s):
+    """The reference's own phs_cpu.cpp (oracle/_ref build; the oracle's restatement where that is
+    absent) on a bounded sample of the workload -- the same processes, channels and event shares
+    at `scale` of its size -- split over `threads` host threads (contiguous event ranges), repeated
+    until about `budget_s` seconds of generation time."""
     import oracle
-    wl = W.workload(wl_name, scale=0.2)   # 2e6 events for the 1e7-event workloads
-    proc = wl.processes[0]
-    ch = np.concatenate([np.full(s.n_events, s.channel, np.int32) for s in wl.segments])
-    x = np.random.default_rng(wl.seed).random((ch.shape[0], proc.n_x))
+    wl = W.workload(wl_name, scale=scale)
+    samples = _process_samples(wl)
     kind = "reference" if oracle.ref_available() else "port"
     total_t, total_n, reps = 0.0, 0, 0
-    while total_t < 10.0 and reps < 40:
-        if kind == "reference":
-            sec, _ = oracle.ref_time(proc, x, ch)
-        else:
-            t0 = time.perf_counter()
-            oracle.forward(proc, x, ch)
-            sec = time.perf_counter() - t0
-        total_t += sec
-        total_n += ch.shape[0]
+    while (total_t < budget_s and reps < 40) or reps == 0:
+        for proc, ch, x in samples:
+            if kind == "reference":
+                sec, _ = oracle.ref_time_threads(proc, x, ch, threads) if threads > 1 else oracle.ref_time(proc, x, ch)
+            else:
+                t0 = time.perf_counter()
+                oracle.forward(proc, x, ch)
+                sec = time.perf_counter() - t0
+            total_t += sec
+            total_n += ch.shape[0]
         reps += 1
-    return dict(value=total_n / total_t, unit="events/s", cores=1, kind=kind,
-                sample=f"{ch.shape[0]} events of {wl_name} (seeded U(0,1) randoms, its channel split) x {reps} passes, "
-                       f"single thread, {total_t:.1f} s; reference phs_cpu.cpp built -O3")
+    n = sum(ch.shape[0] for _, ch, _ in samples)
+    cores, aff, quota = cpu_threads()
+    return dict(value=total_n / total_t, unit="events/s", cores=threads if kind == "reference" else 1, kind=kind,
+                sample=f"{n} events of {wl_name} ({scale:g} of it: same processes, channels and shares, seeded U(0,1)) "
+                       f"x {reps} passes, {threads} thread(s) over contiguous event ranges, {total_t:.1f} s; "
+                       f"host: {aff} cores in the affinity mask, cgroup quota {quota or 'none'}; "
+                       f"reference phs_cpu.cpp built -O3")
 
 
-def cpu_all_cores(wl_name, W):
-    """The reference's phs_cpu.cpp (oracle/_ref build) split over the host cores this process
-    may use (contiguous event ranges, one thread each, at most 16), on the same bounded sample
-    as the single-thread baseline; None where the reference build is absent."""
-    import numpy as np
-    import oracle
-    if not oracle.ref_available():
-        return None
-    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
-    wl = W.workload(wl_name, scale=0.2)
-    proc = wl.processes[0]
-    ch = np.concatenate([np.full(s.n_events, s.channel, np.int32) for s in wl.segments])
-    x = np.random.default_rng(wl.seed).random((ch.shape[0], proc.n_x))
-    total_t, total_n, reps = 0.0, 0, 0
-    while total_t < 3.0 and reps < 40:
-        sec, _ = oracle.ref_time_threads(proc, x, ch, threads)
-        total_t += sec
-        total_n += ch.shape[0]
-        reps += 1
-    return dict(value=total_n / total_t, unit="events/s", cores=threads, kind="reference",
-                sample=f"{ch.shape[0]} events of {wl_name} x {reps} passes, {threads} threads over contiguous "
-                       f"event ranges, {total_t:.1f} s wall; reference phs_cpu.cpp built -O3")
-
-
-def measured_traffic(workload):
-    """HBM bytes per launch of the forward kernel from the committed counter profile
-    (profiles/forward_traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by
-    what the counter reads for a plain device copy) and the launch configuration it was taken
-    with, if it was taken on this workload; else (None, None)."""
-    path = os.path.join(ROOT, "profiles", "forward_traffic.json")
+def measured_counters(workload):
+    """The committed counter profile of the forward kernels of one step of `workload`
+    (profiles/forward_counters.json): HBM bytes per step from FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch, summed over the step's dispatches, each scaled by what the counter reads for a plain
+    device copy), FP64 operations per step from the FP64 VALU instruction counters (64 lanes x
+    (2 FMA + MUL + ADD + transcendental)), and the launch tuning they were taken with; None if
+    the profile is of another workload."""
+    path = os.path.join(ROOT, "profiles", "forward_counters.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None, None
+        return None
     if t.get("workload") != workload:
-        return None, None
-    return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024, t.get("config")
+        return None
+    c = t["per_step"]
+    traffic = (c["FETCH_SIZE"] * t["fetch_scale"] + c["WRITE_SIZE"] * t["write_scale"]) * 1024
+    flops = 64 * (2 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_MUL_F64"] + c["SQ_INSTS_VALU_ADD_F64"] +
+                  c["SQ_INSTS_VALU_TRANS_F64"])
+    return dict(traffic=traffic, flops=flops, tuning=t.get("tuning"), kernel_ms=t.get("kernel_ms_per_step"))
 
 
 def main():
@@ -312,8 +336,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--tuned", default="",
+                    help="apply this launch tuning (JSON of Plan.tuning(), as the bench line reports it in "
+                         "config.tuned) instead of autotuning")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-specialize", action="store_true",
+                    help="run every tree on the generic forward kernel (no shape-specialised kernels)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="",
                     help="fixed launch config block,ept,grid,order,waves_per_eu,lds instead of autotuning")
@@ -368,8 +397,11 @@ def main():
     if args.config:
         v = [int(t) for t in args.config.split(",")]
         fixed = dict(zip(("block", "ept", "grid", "order", "waves_per_eu", "lds"), v))
+    tuning = json.loads(args.tuned) if args.tuned else None
     log(f"workload {args.workload}: setup")
-    main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W, fixed=fixed)
+    main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W, fixed=fixed,
+                                            tuning=tuning, specialize=not args.no_specialize)
+    log(f"workload {args.workload}: tuning {json.dumps(main_res['tuning'])}")
     log(f"workload {args.workload}: {main_res['batched_ev_s']:.4g} ev/s batched, "
         f"{main_res['plain_ev_s']:.4g} ev/s plain")
     kernel_s = main_res["batched_kernel_ms"] / 1e3
@@ -385,13 +417,13 @@ def main():
         log(f"workload c4: {extra['c4']['fwd_adjoint_ev_s']:.4g} ev/s fwd+adjoint, "
             f"{extra['c4']['fwd_adjoint_over_forward']:.2f}x forward")
         torch.cuda.empty_cache()
-        for name in ("c0", "c2", "c3"):
+        for name in ("c1", "c0", "c2", "c3"):
             if name == args.workload:
                 continue
             log(f"workload {name}: setup")
             r, p2, s2, _ = run_workload(name, max(3, args.steps // 2), 5, torch, G, W)
             extra[name] = {k: r[k] for k in ("n_events", "batched_ev_s", "plain_ev_s", "batched_vs_plain",
-                                              "batched_kernel_ms", "config")}
+                                              "batched_kernel_ms", "tuning")}
             extra[name]["tune_summary"] = r["tune"].split(" c0:")[0].split(" c1:")[0]
             extra[name]["launch"] = r["launch"]
             extra[name]["hbm_frac"] = r["algorithmic_bytes"] / (r["batched_kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
@@ -418,16 +450,24 @@ def main():
         log(f"Whizard entry point: {extra['whizard_abi_c1']['events_s']:.4g} events/s including PCIe copies")
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline")
-        cpu = cpu_baseline(args.workload, W)
-        log("cpu baseline, all cores")
-        extra["cpu_all_cores"] = cpu_all_cores(args.workload, W)
+        threads, _, _ = cpu_threads()
+        # 2e6 events of the main workload (1e6 of the 5e7-event mix for the single core)
+        scale = 2e6 / wl.n_events
+        log(f"cpu baseline, {threads} threads")
+        cpu = cpu_baseline(args.workload, W, threads, scale, 10.0)
+        log("cpu baseline, one thread")
+        extra["cpu_single_core"] = cpu_baseline(args.workload, W, 1, scale / 2, 10.0)
         if not args.no_extra:
             for name, scale in (("c1", 0.01), ("c2", 0.002)):
                 if f"inverse_{name}" in extra:
                     extra[f"inverse_{name}"]["cpu_baseline"] = cpu_inverse_baseline(name, scale, W)
     steps = args.steps
-    traffic, traffic_config = measured_traffic(args.workload)
+    counters = measured_counters(args.workload)
+    fp64 = None
+    if counters:
+        tflops = counters["flops"] / kernel_s / 1e12
+        fp64 = {"achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP64_PEAK_TFLOPS,
+                "flops_per_step": counters["flops"], "counted_with": counters["tuning"]}
     out = {
         "metric": "events/s @1 GPU (FP64, <=1e-12 rel vs phs_cpu), batched vs plain; fwd+adjoint ev/s",
         "value": main_res["batched_ev_s"],
@@ -442,16 +482,17 @@ def main():
         "dtype": "f64",
         "data": "synthetic: seeded U(0,1) FP64 randoms generated on the device",
         "config": {"workload": f"{args.workload}: {W.DESCRIPTIONS[args.workload]}",
-                   "n_events": main_res["n_events"], "channels": main_res["n_segments"],
-                   "launch": main_res["launch"], "tuned": main_res["config"],
+                   "n_events": main_res["n_events"], "segments": main_res["n_segments"],
+                   "launch": main_res["launch"], "tuned": main_res["tuning"],
                    "tune_summary": main_res["tune"].split(" c0:")[0].split(" c1:")[0]},
         "plain_ev_s": main_res["plain_ev_s"],
         "batched_vs_plain": main_res["batched_vs_plain"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "traffic_config": traffic_config,
-                     "kernel": "forward_kernel", "kernel_ms": main_res["batched_kernel_ms"],
-                     "bytes_per_launch": main_res["algorithmic_bytes"]},
+                     "traffic": counters["traffic"] if counters else None,
+                     "traffic_counted_with": counters["tuning"] if counters else None,
+                     "kernel": "forward_kernel (every launch of one step)", "kernel_ms": main_res["batched_kernel_ms"],
+                     "bytes_per_launch": main_res["algorithmic_bytes"], "fp64": fp64},
         "cpu_baseline": cpu,
         "extra": extra,
     }

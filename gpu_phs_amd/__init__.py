@@ -51,6 +51,7 @@ _SIGS = {
     "gpu_phs_process_destroy": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_process_info": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 3),
     "gpu_phs_process_trees": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 7),
+    "gpu_phs_process_shapes": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "gpu_phs_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gpu_phs_plan_destroy": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_plan_set_config": (ctypes.c_int, [ctypes.c_int] * 7),
@@ -67,6 +68,12 @@ _SIGS = {
     "gpu_phs_plan_inverse": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_set_event_beams": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "gpu_phs_plan_set_msq_output": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "gpu_phs_plan_set_class_config": (ctypes.c_int, [ctypes.c_int] * 7),
+    "gpu_phs_plan_get_class_config": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5),
+    "gpu_phs_plan_set_schedule": (ctypes.c_int, [ctypes.c_int] * 3),
+    "gpu_phs_plan_get_schedule": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 2),
+    "gpu_phs_plan_set_specialize": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "gpu_phs_grid_create": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_grid_destroy": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_grid_get_weights": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
@@ -136,6 +143,12 @@ class ProcessHandle:
     @property
     def n_out(self):
         return self.proc.n_out
+
+    def shapes(self):
+        """Per channel: the tree shape its forward kernel is specialised for, or -1."""
+        out = np.zeros(self.proc.n_channels, np.int32)
+        _check(lib().gpu_phs_process_shapes(self.handle, _ptr(out), out.size), "process_shapes")
+        return out
 
     def close(self):
         if self.handle >= 0 and _lib is not None:
@@ -211,6 +224,26 @@ class Plan:
         return dict(block=v[0].value, ept=v[1].value, grid=v[2].value, order=v[3].value,
                     waves_per_eu=v[4].value, lds=v[5].value)
 
+    def tuning(self):
+        """The launch shape of both tile classes and the schedule, as the autotuner left them:
+        dict(classes=[{block, ept, grid, waves_per_eu, lds}] * 2, order, serial)."""
+        classes = []
+        for c in range(2):
+            v = [ctypes.c_int() for _ in range(5)]
+            _check(lib().gpu_phs_plan_get_class_config(self.handle, c, *[ctypes.byref(q) for q in v]),
+                   "plan_get_class_config")
+            classes.append(dict(zip(("block", "ept", "grid", "waves_per_eu", "lds"), (q.value for q in v))))
+        o, se = ctypes.c_int(), ctypes.c_int()
+        _check(lib().gpu_phs_plan_get_schedule(self.handle, ctypes.byref(o), ctypes.byref(se)), "plan_get_schedule")
+        return dict(classes=classes, order=o.value, serial=se.value)
+
+    def apply_tuning(self, t):
+        """Set what ``tuning()`` returned (e.g. from an earlier autotuner run)."""
+        for c, lc in enumerate(t["classes"]):
+            _check(lib().gpu_phs_plan_set_class_config(self.handle, c, lc["block"], lc["ept"], lc["grid"],
+                                                       lc["waves_per_eu"], lc["lds"]), "plan_set_class_config")
+        _check(lib().gpu_phs_plan_set_schedule(self.handle, t["order"], t["serial"]), "plan_set_schedule")
+
     def info(self):
         v = [ctypes.c_int() for _ in range(3)]
         _check(lib().gpu_phs_plan_info(self.handle, *[ctypes.byref(q) for q in v]), "plan_info")
@@ -232,6 +265,25 @@ class Plan:
                                                   None if E1 is None else ctypes.c_void_p(E1.data_ptr()),
                                                   None if E2 is None else ctypes.c_void_p(E2.data_ptr())),
                "plan_set_event_beams")
+
+    def set_msq_output(self, segment, msq=None):
+        """Per-branch msq of every event of the segment into ``msq`` (a cuda float64 tensor
+        [n_events][2^n_out - 1], index bincode - 1; slots of other bincodes are left as they
+        are), or None to turn it off.  The tensor must stay alive while the plan is used."""
+        if msq is not None:
+            buf = self._keep[segment][1]
+            n_out = buf.p.shape[1]
+            if msq.dtype != buf.x.dtype or not msq.is_contiguous() or tuple(msq.shape) != (buf.x.shape[0], 2 ** n_out - 1):
+                raise ValueError("msq must be a contiguous float64 tensor [n_events][2^n_out - 1]")
+            self._msq = getattr(self, "_msq", {})
+            self._msq[segment] = msq
+        _check(lib().gpu_phs_plan_set_msq_output(self.handle, segment,
+                                                 None if msq is None else ctypes.c_void_p(msq.data_ptr())),
+               "plan_set_msq_output")
+
+    def set_specialize(self, on=True):
+        """Shape-specialised kernels for trees of up to four final-state particles (default on)."""
+        _check(lib().gpu_phs_plan_set_specialize(self.handle, 1 if on else 0), "plan_set_specialize")
 
     def run(self, stream=None):
         _check(lib().gpu_phs_plan_run(self.handle, _stream_ptr(stream)), "plan_run")

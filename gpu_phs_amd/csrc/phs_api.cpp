@@ -5,6 +5,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "phs_shapes.h"
 #include "phs_state.h"
 
 namespace gphs {
@@ -125,6 +126,7 @@ int plan_build(PlanObj& pl) {
     const size_t k = pl.segs.size();
     sd.e1 = k < pl.beam_e1.size() ? pl.beam_e1[k] : nullptr;
     sd.e2 = k < pl.beam_e2.size() ? pl.beam_e2[k] : nullptr;
+    sd.msq = k < pl.seg_msq.size() ? pl.seg_msq[k] : nullptr;
     pl.segs.push_back(sd);
   }
   if (pl.n_banks > 3) return set_error(GPU_PHS_E_TREE, "tree needs more than 3 save banks");
@@ -161,8 +163,14 @@ int plan_build(PlanObj& pl) {
   if (tiles.size() > (size_t)0x7fffffff) return set_error(GPU_PHS_E_ARG, "too many tiles");
   std::vector<int> idx(tiles.size());
   std::iota(idx.begin(), idx.end(), 0);
+  // class 0 grouped by tree shape (one specialised launch per shape); shapeless channels last
+  auto shape_key = [&](int t) {
+    const int sh = pl.chans[pl.segs[tiles[t].seg].channel].shape;
+    return sh < 0 ? kNumShapes : sh;
+  };
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
     if (tile_class[a] != tile_class[b]) return tile_class[a] < tile_class[b];
+    if (tile_class[a] == 0 && shape_key(a) != shape_key(b)) return shape_key(a) < shape_key(b);
     // order 1: heaviest programs first, so the tail of the launch is made of short tiles
     return pl.order == 1 && tile_depth[a] > tile_depth[b];
   });
@@ -243,11 +251,34 @@ int plan_build(PlanObj& pl) {
   }
   for (const SegDesc& sd : pl.segs)
     if (sd.e1) pl.dp.per_event_beams = pl.dps[pl.chan_class[sd.channel]].per_event_beams = 1;
+  // class 0 by shape, when its kernels may be specialised (fixed beams)
+  pl.shape_runs.clear();
+  if (pl.specialize && !pl.dps[0].per_event_beams) {
+    for (int t = pl.class_begin[0]; t < pl.class_end[0]; t++) {
+      int sh = pl.chans[pl.segs[pl.tiles[t].seg].channel].shape;
+      if (pl.dps[0].has_friends) sh = -1;
+      if (pl.shape_runs.empty() || pl.shape_runs.back().sid != sh) pl.shape_runs.push_back(ShapeRun{sh, t, t + 1});
+      else pl.shape_runs.back().end = t + 1;
+    }
+  }
   return ok_status();
 }
 
+// Tiles [b, e) of class c: per shape run of class 0 the specialised kernel, else the generic one.
+hipError_t launch_range(PlanObj& pl, int c, int b, int e, const LaunchConfig& lc, hipStream_t stream) {
+  if (c != 0 || pl.shape_runs.empty()) return launch_forward(pl.dps[c], b, e, lc, stream);
+  for (const ShapeRun& r : pl.shape_runs) {
+    const int lo = std::max(b, r.begin), hi = std::min(e, r.end);
+    if (lo >= hi) continue;
+    hipError_t err = r.sid >= 0 ? launch_forward_shape(pl.dps[0], r.sid, lo, hi, lc, stream)
+                                : launch_forward(pl.dps[0], lo, hi, lc, stream);
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_class(PlanObj& pl, int c, hipStream_t stream) {
-  return launch_forward(pl.dps[c], pl.class_begin[c], pl.class_end[c], pl.lcs[c], stream);
+  return launch_range(pl, c, pl.class_begin[c], pl.class_end[c], pl.lcs[c], stream);
 }
 
 // Both classes on `stream`'s timeline: class 0 on `stream`, class 1 (when both have tiles) on
@@ -284,7 +315,7 @@ int plan_run_plain(PlanObj& pl, hipStream_t stream) {
     const int cl = pl.chan_class[pl.segs[s].channel];
     LaunchConfig lc = pl.lcs[cl];
     lc.grid = 0;
-    hipError_t e = launch_forward(pl.dps[cl], pl.seg_tile_begin[s], pl.seg_tile_end[s], lc, stream);
+    hipError_t e = launch_range(pl, cl, pl.seg_tile_begin[s], pl.seg_tile_end[s], lc, stream);
     if (e != hipSuccess) return hip_error(e, "forward launch (plain)");
   }
   return ok_status();
@@ -347,6 +378,15 @@ int gpu_phs_process_info(int process, int* n_out, int* n_x, int* n_channels) {
   return ok_status();
 }
 
+int gpu_phs_process_shapes(int process, int* shapes, int n) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  ProcessObj* po = find_process(process);
+  if (!po) return set_error(GPU_PHS_E_HANDLE, "unknown process");
+  if (!shapes || n < (int)po->cp.channels.size()) return set_error(GPU_PHS_E_ARG, "shapes: need one entry per channel");
+  for (size_t c = 0; c < po->cp.channels.size(); c++) shapes[c] = po->cp.channels[c].shape;
+  return ok_status();
+}
+
 int gpu_phs_process_trees(int process, int* d1, int* d2, int* has_children, int* map_ids, double* map_masses,
                           double* map_widths, double* flv_masses) {
   std::lock_guard<std::mutex> lk(api_mutex());
@@ -397,6 +437,28 @@ int gpu_phs_plan_set_event_beams(int plan, int segment, const double* E1, const 
   return plan_build(*pl);
 }
 
+int gpu_phs_plan_set_msq_output(int plan, int segment, double* msq) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (segment < 0 || segment >= (int)pl->user.size()) return set_error(GPU_PHS_E_ARG, "set_msq_output: no such segment");
+  pl->seg_msq.resize(pl->user.size(), nullptr);
+  pl->seg_msq[segment] = msq;
+  drop_graph(*pl);
+  pl->adj_valid = false;
+  pl->inv_segs.clear();
+  return plan_build(*pl);
+}
+
+int gpu_phs_plan_set_specialize(int plan, int on) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  pl->specialize = on ? 1 : 0;
+  drop_graph(*pl);
+  return plan_build(*pl);
+}
+
 int gpu_phs_plan_destroy(int plan) {
   std::lock_guard<std::mutex> lk(api_mutex());
   if (drop_plan(plan) != GPU_PHS_OK) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
@@ -427,6 +489,63 @@ int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, i
   }
   pl->order = order;
   if (rebuild) return plan_build(*pl);
+  return ok_status();
+}
+
+int gpu_phs_plan_set_class_config(int plan, int cls, int block, int ept, int grid, int waves_per_eu, int lds) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (cls < 0 || cls >= kTileClasses || block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 ||
+      !(waves_per_eu == 0 || waves_per_eu == 3 || waves_per_eu == 4) || lds < 0 || lds > 3)
+    return set_error(GPU_PHS_E_ARG, "class in {0,1}, block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, "
+                                    "waves_per_eu in {0,3,4}, lds in {0,1,2,3}");
+  LaunchConfig want;
+  want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds = lds;
+  if (pl->class_end[cls] > pl->class_begin[cls] && forward_lds_bytes(pl->dps[cls], want) > forward_lds_limit())
+    return set_error(GPU_PHS_E_ARG, "the staging tables of this block size (and LDS save banks) exceed the LDS a "
+                                    "workgroup may use on this device; choose a smaller block");
+  const bool rebuild = block != pl->lcs[cls].block || ept != pl->lcs[cls].ept;
+  pl->lcs[cls] = want;
+  drop_graph(*pl);
+  if (rebuild) return plan_build(*pl);
+  return ok_status();
+}
+
+int gpu_phs_plan_get_class_config(int plan, int cls, int* block, int* ept, int* grid, int* waves_per_eu, int* lds) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (cls < 0 || cls >= kTileClasses) return set_error(GPU_PHS_E_ARG, "class in {0,1}");
+  const LaunchConfig& lc = pl->lcs[cls];
+  if (block) *block = lc.block;
+  if (ept) *ept = lc.ept;
+  if (grid) *grid = lc.grid;
+  if (waves_per_eu) *waves_per_eu = lc.waves_per_eu;
+  if (lds) *lds = lc.lds;
+  return ok_status();
+}
+
+int gpu_phs_plan_set_schedule(int plan, int order, int serial) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (order < 0 || order > 1 || serial < 0 || serial > 1) return set_error(GPU_PHS_E_ARG, "order and serial in {0,1}");
+  pl->serial_classes = serial;
+  drop_graph(*pl);
+  if (order != pl->order) {
+    pl->order = order;
+    return plan_build(*pl);
+  }
+  return ok_status();
+}
+
+int gpu_phs_plan_get_schedule(int plan, int* order, int* serial) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (order) *order = pl->order;
+  if (serial) *serial = pl->serial_classes;
   return ok_status();
 }
 
@@ -554,11 +673,15 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
       return GPU_PHS_OK;
     };
     const bool banks = pl->dps[cls].n_banks > 0;
+    // the shape-specialised kernels keep randoms and banks in registers: where only they run,
+    // the placement flags change nothing
+    bool placement = cls != 0 || pl->shape_runs.empty();
+    for (const ShapeRun& r : pl->shape_runs) placement = placement || r.sid < 0;
     // stage 1: register budget (occupancy target), where deferred boosts wait (lds bit 0) and
     // where the randoms are kept (lds bit 1)
     for (int wpe : {0, 3, 4})
       for (int lds : {0, 1, 2, 3})
-        if (!rc && (!(lds & 1) || banks)) {
+        if (!rc && (!(lds & 1) || banks) && (placement || lds == 0)) {
           LaunchConfig lc = best.lc;
           lc.waves_per_eu = wpe; lc.lds = lds; lc.grid = 0;
           rc = consider(lc);
@@ -569,7 +692,7 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
     for (int block : {64, 128, 256})
       for (int ept : {1, 2, 4})
         for (int xl : {0, 2})
-          if (!rc) {
+          if (!rc && (placement || xl == 0)) {
             LaunchConfig lc = base;
             lc.block = block; lc.ept = ept; lc.lds = base.lds ^ xl;
             rc = consider(lc);
@@ -580,7 +703,7 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
       const LaunchConfig shape = best.lc;
       for (int wpe : {0, 3, 4})
         for (int lds : {0, 1, 2, 3})
-          if (!rc && (!(lds & 1) || banks) && !(wpe == shape.waves_per_eu && lds == shape.lds)) {
+          if (!rc && (!(lds & 1) || banks) && (placement || lds == 0) && !(wpe == shape.waves_per_eu && lds == shape.lds)) {
             LaunchConfig lc = shape;
             lc.waves_per_eu = wpe; lc.lds = lds;
             rc = consider(lc);

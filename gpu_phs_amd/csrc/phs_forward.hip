@@ -25,6 +25,7 @@
 
 #include "phs_device.h"
 #include "phs_kernels.h"
+#include "phs_shapes.h"
 
 namespace gphs {
 
@@ -115,19 +116,52 @@ extern __shared__ double lds[];
 // of an LDS table x[k][event] when the event starts.
 constexpr int kXLds = -1;
 
+// The walk's structure: SID < 0 reads it from the program records (generic kernel); SID >= 0
+// takes it from the compile-time shape table (phs_shapes.h), so after the branch loop is
+// unrolled every flag and random index below is a constant.
+template <int SID>
+struct Walk {
+  static constexpr int kS = SID < 0 ? 0 : SID;
+  static __device__ __forceinline__ bool leaf(const NodeCmd& nc, int i, int c) {
+    if constexpr (SID >= 0) return c == 0 ? kShapes[kS].node[i].leaf0 : kShapes[kS].node[i].leaf1;
+    else return nc.k[c].leaf;
+  }
+  static __device__ __forceinline__ int desc(const NodeCmd& nc, int i) {
+    if constexpr (SID >= 0) return kShapes[kS].node[i].desc;
+    else return nc.desc;
+  }
+  static __device__ __forceinline__ int push(const NodeCmd& nc, int i) {
+    if constexpr (SID >= 0) return kShapes[kS].node[i].push;
+    else return nc.push;
+  }
+  static __device__ __forceinline__ int xi(const NodeCmd& nc, int i, int c) {
+    if constexpr (SID >= 0) return c == 0 ? kShapes[kS].node[i].xi0 : kShapes[kS].node[i].xi1;
+    else return nc.k[c].xi;
+  }
+  static __device__ __forceinline__ int xphi(const NodeCmd& nc, int i) {
+    if constexpr (SID >= 0) return kShapes[kS].node[i].xphi;
+    else return nc.x_phi;
+  }
+};
+
 // One event of channel `cd` (program `prog`).  xg: the event's randoms; pl: this lane's column
 // of the momentum table (word 4 j + mu at pl[(4 j + mu) * S]).
 // eb: the event's beams (the channel's fixed ones unless PEV: then the map constants that
-// depend on sqrt(s) are recomputed per event as well).
-template <int BANK, int XN, bool FR, bool PEV>
+// depend on sqrt(s) are recomputed per event as well).  msq_row: this event's row of the
+// per-branch msq output when want_msq (a wave-uniform flag), else unused.
+template <int BANK, int XN, bool FR, bool PEV, int SID>
 __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBeams& eb, const NodeCmd* __restrict__ prog,
                                               const double* __restrict__ xg, double* __restrict__ pl,
-                                              double* __restrict__ xl, int S, double* bank_base,
-                                              double& factor_out, bool& ok_out) {
+                                              double* __restrict__ xl, int S, double* bank_base, bool want_msq,
+                                              gdouble* __restrict__ msq_row, double& factor_out, bool& ok_out) {
+  using W = Walk<SID>;
   const double s = eb.s;
   const gdouble* __restrict__ xrow = as_global(xg);
   double xv[XN > 0 ? XN : 1];
-  if constexpr (XN > 0) {
+  if constexpr (SID >= 0) {
+#pragma unroll
+    for (int k = 0; k < XN; k++) xv[k] = xrow[k];   // XN is the shape's n_x
+  } else if constexpr (XN > 0) {
 #pragma unroll
     for (int k = 0; k < XN; k++) xv[k] = xrow[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
   } else {
@@ -143,22 +177,28 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
   double factor = 1.0;
   bool ok = true;
   int depth = 0;
-
   double cur_m = eb.sqrts;  // sqrt(msq) of the branch being visited, carried from its parent
-  for (int i = 0; i < cd.n_nodes; ++i) {
+  if (want_msq) msq_row[(1 << cd.n_out) - 2] = s;
+
+  auto branch = [&](const int i) {
     const NodeCmd& nc = prog[i];
+    const bool leaf0 = W::leaf(nc, i, 0), leaf1 = W::leaf(nc, i, 1);
     double m_max = nc.m_max, cb0 = nc.b0, cb1 = nc.b1;
     if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max, cb0, cb1);
     // daughters' msq (phs_cpu.cpp:180-191, 207-208; leaves phs_cpu.cpp:541-547) and masses
     double msq0, f0, m1, msq1, f1, m2;
-    if (nc.k[0].leaf) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
+    if (leaf0) { msq0 = nc.k[0].msq_leaf; f0 = 1.0; m1 = nc.k[0].m_leaf; }
     else if constexpr (PEV) {
-      msq_map(child_at(nc.k[0], eb.sqrts, s, cd.m_tot), GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = math::sqrt_phys(msq0);
-    } else { msq_map(nc.k[0], GPHS_X(nc.k[0].xi), s, msq0, f0); m1 = math::sqrt_phys(msq0); }
-    if (nc.k[1].leaf) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
+      msq_map(child_at(nc.k[0], eb.sqrts, s, cd.m_tot), GPHS_X(W::xi(nc, i, 0)), s, msq0, f0); m1 = math::sqrt_phys(msq0);
+    } else { msq_map(nc.k[0], GPHS_X(W::xi(nc, i, 0)), s, msq0, f0); m1 = math::sqrt_phys(msq0); }
+    if (leaf1) { msq1 = nc.k[1].msq_leaf; f1 = 1.0; m2 = nc.k[1].m_leaf; }
     else if constexpr (PEV) {
-      msq_map(child_at(nc.k[1], eb.sqrts, s, cd.m_tot), GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = math::sqrt_phys(msq1);
-    } else { msq_map(nc.k[1], GPHS_X(nc.k[1].xi), s, msq1, f1); m2 = math::sqrt_phys(msq1); }
+      msq_map(child_at(nc.k[1], eb.sqrts, s, cd.m_tot), GPHS_X(W::xi(nc, i, 1)), s, msq1, f1); m2 = math::sqrt_phys(msq1);
+    } else { msq_map(nc.k[1], GPHS_X(W::xi(nc, i, 1)), s, msq1, f1); m2 = math::sqrt_phys(msq1); }
+    if (want_msq) {
+      msq_row[nc.k[0].leaves - 1] = msq0;
+      msq_row[nc.k[1].leaves - 1] = msq1;
+    }
 
     // 1 -> 2 decay of this branch (phs_cpu.cpp:218-233)
     const double m = cur_m;
@@ -175,9 +215,9 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
     const double bg = m > 0 ? cur_p * rm : 0.0;
     const double g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
-    math::sincos_turns(GPHS_X(nc.x_phi), sp, cp);  // phi = 2 pi x
+    math::sincos_turns(GPHS_X(W::xphi(nc, i)), sp, cp);  // phi = 2 pi x
     double ct, st, fct;
-    ct_map(nc.ct_kind, GPHS_X(nc.x_phi + 1), cb0, cb1, ct, st, fct);
+    ct_map(nc.ct_kind, GPHS_X(W::xphi(nc, i) + 1), cb0, cb1, ct, st, fct);
     factor *= fct;
     // the root's parent boost is the identity (phs_cpu.cpp:568-573): M = L1; after a branch
     // with two final-state daughters the walk pops a saved matrix, so only the columns the
@@ -185,14 +225,14 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
     // an incoming beam (phs.cu:853-910).
     if (FR && nc.friend_sign != 0) compose_friend(M, g, bg, ct, st, cp, sp, (double)nc.friend_sign);
     else if (i == 0) set_l1(M, g, bg, ct, st, cp, sp);
-    else if (nc.k[0].leaf && nc.k[1].leaf) compose_cols03(M, g, bg, ct, st, cp, sp);
+    else if (leaf0 && leaf1) compose_cols03(M, g, bg, ct, st, cp, sp);
     else compose(M, g, bg, ct, st, cp, sp);
 
     // final-state daughters: p = L (E, 0, 0, +-p) (phs_cpu.cpp:286-291), then the lab boost
     // of the final leaves (phs.cu:1286-1301); E = sqrt(0 + p^2) is |p| exactly
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (nc.k[c].leaf) {
+      if (c == 0 ? leaf0 : leaf1) {
         const double pc = c == 0 ? pk : -pk;
         const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
         // rows of L (E, 0, 0, pc), the second product fused
@@ -211,18 +251,25 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
     }
 
     // next branch: a daughter (matrix stays), or the deferred one from the save bank
-    const double msq_push = nc.push == 0 ? msq0 : msq1;
-    const double m_push = nc.push == 0 ? m1 : m2;
-    const double p_push = nc.push == 0 ? pk : -pk;
-    if (nc.push >= 0) { bank.push(depth, M, L1Par{g, bg, ct, st, cp, sp}, msq_push, m_push, p_push); depth++; }
-    if (nc.desc >= 0) {
-      cur_msq = nc.desc == 0 ? msq0 : msq1;
-      cur_m = nc.desc == 0 ? m1 : m2;
-      cur_p = nc.desc == 0 ? pk : -pk;
+    const int push = W::push(nc, i), desc = W::desc(nc, i);
+    const double msq_push = push == 0 ? msq0 : msq1;
+    const double m_push = push == 0 ? m1 : m2;
+    const double p_push = push == 0 ? pk : -pk;
+    if (push >= 0) { bank.push(depth, M, L1Par{g, bg, ct, st, cp, sp}, msq_push, m_push, p_push); depth++; }
+    if (desc >= 0) {
+      cur_msq = desc == 0 ? msq0 : msq1;
+      cur_m = desc == 0 ? m1 : m2;
+      cur_p = desc == 0 ? pk : -pk;
     } else if (depth > 0) {
       depth--;
       bank.pop(depth, M, cur_msq, cur_m, cur_p);
     }
+  };
+  if constexpr (SID >= 0) {
+#pragma unroll
+    for (int i = 0; i < kShapes[Walk<SID>::kS].n_nodes; ++i) branch(i);
+  } else {
+    for (int i = 0; i < cd.n_nodes; ++i) branch(i);
   }
   factor_out = factor;
   ok_out = ok;
@@ -261,7 +308,7 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
 // LDS per workgroup (S = blockDim.x + 1 doubles per row): momenta 4 nout_max rows, the
 // randoms nx_max rows (XN = kXLds), then the save banks (BANK = BK_LDS).  A tile of `count`
 // events is done in sub-tiles of blockDim.x events.
-template <int BANK, int WPE, int XN, bool FR, bool PEV>
+template <int BANK, int WPE, int XN, bool FR, bool PEV, int SID>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
                     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
@@ -289,7 +336,12 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
       EvBeams eb;
       if constexpr (PEV) eb = event_beams(cd, as_global(sd.e1)[e], as_global(sd.e2)[e]);
       else eb = fixed_beams(cd);
-      forward_event<BANK, XN, FR, PEV>(cd, eb, prog, sd.x + e * cd.n_x, ps + lane, xs + lane, S, bank_base, factor, ok);
+      // the msq output is wave-uniform; lanes past the end write their (repeated) last event's
+      // values again, identical ones
+      const bool want_msq = sd.msq != nullptr;
+      gdouble* msq_row = want_msq ? as_global(sd.msq) + e * ((1 << cd.n_out) - 1) : nullptr;
+      forward_event<BANK, XN, FR, PEV, SID>(cd, eb, prog, sd.x + e * cd.n_x, ps + lane, xs + lane, S, bank_base,
+                                            want_msq, msq_row, factor, ok);
       if (valid) {
         as_global(sd.factor)[e0 + lane] = factor;
         as_global(sd.volume)[e0 + lane] = eb.volume;
@@ -314,9 +366,36 @@ int x_mode(const DevicePlan& dp, const LaunchConfig& lc) {
 
 template <int BANK, int WPE, bool FR>
 FwdKernel pick_xn(int xm) {
-  if (xm == 8) return forward_kernel<BANK, WPE, 8, FR, false>;
-  if (xm == 16) return forward_kernel<BANK, WPE, 16, FR, false>;
-  return forward_kernel<BANK, WPE, kXLds, FR, false>;
+  if (xm == 8) return forward_kernel<BANK, WPE, 8, FR, false, -1>;
+  if (xm == 16) return forward_kernel<BANK, WPE, 16, FR, false, -1>;
+  return forward_kernel<BANK, WPE, kXLds, FR, false, -1>;
+}
+
+// Kernels specialised for one tree shape: randoms in the shape's n_x registers, the root's
+// boost parameters as the save bank when the root defers a daughter, fixed beams, no friends.
+template <int SID, int WPE>
+FwdKernel shape_kernel() {
+  constexpr int bank = kShapes[SID].banks ? BK_ROOT : BK_NONE;
+  return forward_kernel<bank, WPE, kShapes[SID].n_x, false, false, SID>;
+}
+template <int SID>
+FwdKernel pick_shape_wpe(int wpe) {
+  if (wpe == 3) return shape_kernel<SID, 3>();
+  if (wpe == 4) return shape_kernel<SID, 4>();
+  return shape_kernel<SID, 0>();
+}
+FwdKernel pick_shape(int sid, int wpe) {
+  static_assert(kNumShapes == 8, "one case per shape");
+  switch (sid) {
+    case 0: return pick_shape_wpe<0>(wpe);
+    case 1: return pick_shape_wpe<1>(wpe);
+    case 2: return pick_shape_wpe<2>(wpe);
+    case 3: return pick_shape_wpe<3>(wpe);
+    case 4: return pick_shape_wpe<4>(wpe);
+    case 5: return pick_shape_wpe<5>(wpe);
+    case 6: return pick_shape_wpe<6>(wpe);
+    default: return pick_shape_wpe<7>(wpe);
+  }
 }
 
 // Plans with friend branches run the variant that has the friend-axis composition, plans with
@@ -326,7 +405,7 @@ template <int BANK>
 FwdKernel pick_wpe(const DevicePlan& dp, const LaunchConfig& lc) {
   const int xm = x_mode(dp, lc);
   if (dp.per_event_beams)
-    return dp.has_friends ? forward_kernel<BANK, 0, kXLds, true, true> : forward_kernel<BANK, 0, kXLds, false, true>;
+    return dp.has_friends ? forward_kernel<BANK, 0, kXLds, true, true, -1> : forward_kernel<BANK, 0, kXLds, false, true, -1>;
   if (dp.has_friends) return pick_xn<BANK, 0, true>(xm);
   if (lc.waves_per_eu == 3) return pick_xn<BANK, 3, false>(xm);
   if (lc.waves_per_eu == 4) return pick_xn<BANK, 4, false>(xm);
@@ -367,6 +446,21 @@ size_t forward_lds_limit() {
     limit[dev] = (size_t)std::min(v, 160 * 1024);
   }
   return limit[dev];
+}
+
+hipError_t launch_forward_shape(const DevicePlan& dp, int sid, int t_begin, int t_end, const LaunchConfig& lc,
+                                hipStream_t stream) {
+  if (t_end <= t_begin) return hipSuccess;
+  if (sid < 0 || sid >= kNumShapes || lc.block < 64 || lc.block > 256 || lc.block % 64) return hipErrorInvalidValue;
+  const int n_out = kShapes[sid].n_out;
+  const size_t shmem = 4 * (size_t)n_out * (size_t)(lc.block + 1) * sizeof(double);
+  if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
+  const int n = t_end - t_begin;
+  const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
+  FwdKernel k = pick_shape(sid, lc.waves_per_eu);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,
+                     dp.nodes, (int)kShapes[sid].n_x, n_out);
+  return hipGetLastError();
 }
 
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream) {

@@ -38,6 +38,10 @@ size_t forward_lds_limit();
 
 // Forward pass over tiles [t_begin, t_end).
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream);
+// The same for tiles whose channels all have tree shape `sid` (phs_shapes.h), with the kernel
+// specialised for it (fixed beams, no friends; lc.lds is not used: randoms in registers).
+hipError_t launch_forward_shape(const DevicePlan& dp, int sid, int t_begin, int t_end, const LaunchConfig& lc,
+                                hipStream_t stream);
 
 // Per-lane LDS layout of the adjoint (in doubles, lane-strided): boost parameters of every
 // branch [0, off_in), branch inputs [off_in, off_st), stash [off_st, off_acc), (M, Gamma)

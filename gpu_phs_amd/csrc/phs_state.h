@@ -27,9 +27,18 @@ struct ProcessObj {
 
 constexpr int kTileClasses = 2;
 
+// A run of consecutive tiles of tile class 0 whose channels share one tree shape (or -1:
+// channels without a shape, or the generic kernel asked for).
+struct ShapeRun {
+  int sid, begin, end;
+};
+
 struct PlanObj {
   std::vector<gpu_phs_segment> user;       // segments as given
   std::vector<const double*> beam_e1, beam_e2;   // per-event beam energies of each segment, or null
+  std::vector<double*> seg_msq;            // per-branch msq output of each segment, or null
+  int specialize = 1;                      // 1: tile class 0 runs the shape-specialised kernels
+  std::vector<ShapeRun> shape_runs;        // class 0 split by shape (empty: one generic launch)
   std::vector<ChannelDesc> chans;
   std::vector<NodeCmd> nodes;
   std::vector<SegDesc> segs;

@@ -88,7 +88,7 @@ struct ChannelDesc {
   int32_t n_tape;    // adjoint tape slots the program needs
   uint32_t div_nx;   // floor(2^32 / n_x) + 1: i / n_x = umulhi(i, div_nx) for i < 2^32 / n_x
   uint32_t div_p;    // floor(2^32 / (4 n_out)) + 1, likewise for the momenta of an event
-  int32_t pad1;
+  int32_t shape;     // entry of the compile-time shape table (phs_shapes.h), -1: none
   double sqrts, s;
   double volume;     // phase-space volume, evaluated in the reference's order
   double bg_lab;     // lab-frame boost beta*gamma = (E1 - E2) / (2 sqrt(E1 E2)), phs.cu:1289
@@ -115,6 +115,9 @@ struct SegDesc {
   // per-event beam energies [n] (phs.cu:1313-1317), or null: the channel's fixed beams
   const double* e1;
   const double* e2;
+  // msq of every branch [n][2^n_out - 1] at bincode - 1 (leaves m^2, the root s), as the
+  // reference keeps them (phs.cu:570-624), or null; slots of other bincodes are not written
+  double* msq;
 };
 
 struct TileDesc {

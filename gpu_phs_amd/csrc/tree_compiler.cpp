@@ -1,6 +1,8 @@
 // Host work-list compiler (see tree_compiler.h).
 #include "tree_compiler.h"
 
+#include "phs_shapes.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -359,6 +361,7 @@ std::string compile_process(const ProcessSpec& sp, CompiledProcess& out) {
     if (!e.empty()) return e;
     cd.n_banks = max_depth;
     cd.n_tape = n_tape;
+    cd.shape = shape_of(cd, &out.nodes[cd.node0]);
     out.max_banks = std::max(out.max_banks, max_depth);
     if ((int)out.nodes.size() - cd.node0 != cd.n_nodes) return "internal error: node count";
     out.channels.push_back(cd);

@@ -125,6 +125,43 @@ def process_large(n_out: int = 8, n_channels: int = 12) -> Process:
                         name=f"2->{n_out}: 10 GeV finals")
 
 
+def ordered_trees(leaves):
+    """Every ordered binary tree over the given leaves in that order."""
+    if len(leaves) == 1:
+        return [leaves[0]]
+    out = []
+    for k in range(1, len(leaves)):
+        for a in ordered_trees(leaves[:k]):
+            for b in ordered_trees(leaves[k:]):
+                out.append((a, b))
+    return out
+
+
+def process_shapes(n_out: int, per_shape: int = 3) -> Process:
+    """Every ordered tree shape with n_out final-state particles (2, 3 or 4), each with
+    ``per_shape`` seeded labellings of the particles and seeded maps from the whole set
+    (Breit-Wigner, step, exponential, t/u-channel, linear), particles 1 and 4 massive (4.8 and
+    1.5 GeV), the others massless, at sqrt(s) = 1 TeV: the trees of the shape-specialised
+    forward kernels."""
+    rng = np.random.default_rng(seed(9) + n_out)
+    choices = [(MAP_NO, 0.0, 0.0), (MAP_SCHANNEL, 91.2, 2.5), (MAP_SCHANNEL, 300.0, 15.0),
+               (MAP_COLLINEAR, 10.0, 0.0), (MAP_INFRARED, 10.0, 0.0), (MAP_TCHANNEL, 20.0, 0.0),
+               (MAP_UCHANNEL, 20.0, 0.0), (MAP_STEP_E, 250.0, 20.0), (MAP_STEP_H, 250.0, 20.0)]
+    trees, maps = [], []
+    for shape in ordered_trees(list(range(n_out))):
+        for _ in range(per_shape):
+            perm = [int(v) for v in rng.permutation(n_out)]
+            relabel = lambda t: perm[t] if isinstance(t, int) else (relabel(t[0]), relabel(t[1]))
+            t = relabel(shape)
+            m = {"root": ((MAP_NO, MAP_TCHANNEL, MAP_UCHANNEL)[int(rng.integers(3))], 30.0, 0.0)}
+            for node in internal_nodes(t)[1:]:
+                m[node] = choices[int(rng.integers(len(choices)))]
+            trees.append(t)
+            maps.append(m)
+    flv = [M_E, M_E] + [4.8, 0.0, 0.0, 1.5][:n_out]
+    return make_process(2, n_out, trees, maps, flv, 1000.0, name=f"2->{n_out}: every tree shape")
+
+
 def process_friends(n_out: int = 4) -> Process:
     """Trees whose branches take their polar axis from an incoming beam (friends), at
     sqrt(s) = 1 TeV: friend signs -1 and +1, friends on the root and on inner branches, flat

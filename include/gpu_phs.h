@@ -117,6 +117,10 @@ int gpu_phs_process_load_ref(const char* path, double E1, double E2);
 int gpu_phs_process_destroy(int process);
 /* n_out, n_x, n_channels of a process. */
 int gpu_phs_process_info(int process, int* n_out, int* n_x, int* n_channels);
+/* The compile-time tree shape each channel's forward kernel is specialised for (an index of
+ * the library's shape table: the ordered trees of 2, 3 or 4 final-state particles), or -1 where
+ * the generic kernel runs it (larger trees, friends).  n >= n_channels. */
+int gpu_phs_process_shapes(int process, int* shapes, int n);
 /* The process's tree arrays as created (sizes from gpu_phs_process_info); any pointer may
  * be NULL. */
 int gpu_phs_process_trees(int process, int* d1, int* d2, int* has_children, int* map_ids,
@@ -149,12 +153,34 @@ int gpu_phs_plan_destroy(int plan);
 int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, int waves_per_eu, int lds);
 int gpu_phs_plan_get_config(int plan, int* block, int* ept, int* grid, int* order, int* waves_per_eu,
                             int* lds);
+/* Launch shape of one tile class (0: trees of up to four final-state particles whose only
+ * deferred subtree is the root's; 1: the others), arguments as gpu_phs_plan_set_config, which
+ * sets both classes alike.  The autotuner sets them separately; these reproduce its choice. */
+int gpu_phs_plan_set_class_config(int plan, int cls, int block, int ept, int grid, int waves_per_eu, int lds);
+int gpu_phs_plan_get_class_config(int plan, int cls, int* block, int* ept, int* grid, int* waves_per_eu, int* lds);
+/* Tile order (as in gpu_phs_plan_set_config) and whether the two tile classes run one after the
+ * other on the caller's stream (serial = 1) or share the GPU (0: the second class on a side
+ * stream, joined back by events). */
+int gpu_phs_plan_set_schedule(int plan, int order, int serial);
+int gpu_phs_plan_get_schedule(int plan, int* order, int* serial);
 /* Per-event beam energies of segment `segment` (device arrays [n_events], or both NULL for the
  * process's fixed beams): each event gets sqrt(s) = 2 sqrt(E1 E2), its lab boost and the map
  * constants that depend on sqrt(s), as the reference does per event (phs.cu:1313-1317,
  * 1286-1301; mappings_gpu.hpp).  The arrays must outlive the plan's use.  Plans with per-event
  * beams have no adjoint. */
 int gpu_phs_plan_set_event_beams(int plan, int segment, const double* E1, const double* E2);
+/* Per-branch msq output of segment `segment`: a device array [n_events][2^n_out - 1] that
+ * receives, at index bincode - 1, the invariant mass squared of every branch of the event's
+ * tree -- m_j^2 for the final-state particles, s for the root, the mapped msq of the others --
+ * as the reference's per-branch array holds them after its decay stage (phs.cu:570-624,
+ * phs_cpu.cpp:174-234).  Slots of bincodes that are not branches of the tree are not written.
+ * NULL (the default) turns it off; it costs no memory traffic then.  The array must outlive
+ * the plan's use. */
+int gpu_phs_plan_set_msq_output(int plan, int segment, double* msq);
+/* 1 (the default): channels whose trees have at most four final-state particles (and no
+ * friends, fixed beams) run forward kernels specialised for their tree shape; 0: every channel
+ * runs the generic kernel.  Both give identical results. */
+int gpu_phs_plan_set_specialize(int plan, int on);
 /* Forward pass of every segment in one kernel launch on `stream` (hipStream_t, 0 = null). */
 int gpu_phs_plan_run(int plan, void* stream);
 /* Forward pass with one launch per segment (the plain per-channel form). */

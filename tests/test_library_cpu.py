@@ -67,6 +67,26 @@ def test_process_create(name):
     assert (n_out.value, n_x.value, n_ch.value) == (proc.n_out, proc.n_x, proc.n_channels)
 
 
+@pytest.mark.parametrize("n_out", [2, 3, 4])
+def test_every_small_tree_has_a_specialised_kernel(n_out):
+    """Each ordered tree shape of 2-4 final-state particles matches its own entry of the
+    compile-time shape table (phs_shapes.h), whatever the particle labels and maps."""
+    proc = W.process_shapes(n_out)
+    shapes = G.ProcessHandle(proc).shapes()
+    n_shapes = len(W.ordered_trees(list(range(n_out))))
+    per = proc.n_channels // n_shapes
+    assert (shapes >= 0).all()
+    by_tree = shapes.reshape(n_shapes, per)
+    assert (by_tree == by_tree[:, :1]).all()              # labels and maps do not matter
+    assert len(set(by_tree[:, 0].tolist())) == n_shapes   # distinct shapes, distinct kernels
+
+
+def test_larger_trees_and_friends_run_the_generic_kernel():
+    for proc in (W.process_2to6(), W.process_friends(4)):
+        assert (G.ProcessHandle(proc).shapes() == -1).all()
+    assert (G.ProcessHandle(W.process_c2()).shapes() >= 0).all()
+
+
 def _expect_tree_error(proc, fragment):
     with pytest.raises(G.GpuPhsError) as e:
         G.ProcessHandle(proc)
