@@ -163,14 +163,14 @@ int plan_build(PlanObj& pl) {
   if (tiles.size() > (size_t)0x7fffffff) return set_error(GPU_PHS_E_ARG, "too many tiles");
   std::vector<int> idx(tiles.size());
   std::iota(idx.begin(), idx.end(), 0);
-  // class 0 grouped by tree shape (one specialised launch per shape); shapeless channels last
+  // each class grouped by tree shape (one specialised launch per shape); shapeless channels last
   auto shape_key = [&](int t) {
     const int sh = pl.chans[pl.segs[tiles[t].seg].channel].shape;
     return sh < 0 ? kNumShapes : sh;
   };
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
     if (tile_class[a] != tile_class[b]) return tile_class[a] < tile_class[b];
-    if (tile_class[a] == 0 && shape_key(a) != shape_key(b)) return shape_key(a) < shape_key(b);
+    if (shape_key(a) != shape_key(b)) return shape_key(a) < shape_key(b);
     // order 1: heaviest programs first, so the tail of the launch is made of short tiles
     return pl.order == 1 && tile_depth[a] > tile_depth[b];
   });
@@ -251,14 +251,15 @@ int plan_build(PlanObj& pl) {
   }
   for (const SegDesc& sd : pl.segs)
     if (sd.e1) pl.dp.per_event_beams = pl.dps[pl.chan_class[sd.channel]].per_event_beams = 1;
-  // class 0 by shape, when its kernels may be specialised (fixed beams)
-  pl.shape_runs.clear();
-  if (pl.specialize && !pl.dps[0].per_event_beams) {
-    for (int t = pl.class_begin[0]; t < pl.class_end[0]; t++) {
-      int sh = pl.chans[pl.segs[pl.tiles[t].seg].channel].shape;
-      if (pl.dps[0].has_friends) sh = -1;
-      if (pl.shape_runs.empty() || pl.shape_runs.back().sid != sh) pl.shape_runs.push_back(ShapeRun{sh, t, t + 1});
-      else pl.shape_runs.back().end = t + 1;
+  // each class by shape, when its kernels may be specialised (fixed beams)
+  for (int c = 0; c < kTileClasses; c++) {
+    std::vector<ShapeRun>& runs = pl.shape_runs[c];
+    runs.clear();
+    if (!pl.specialize || pl.dps[c].per_event_beams) continue;
+    for (int t = pl.class_begin[c]; t < pl.class_end[c]; t++) {
+      const int sh = pl.chans[pl.segs[pl.tiles[t].seg].channel].shape;
+      if (runs.empty() || runs.back().sid != sh) runs.push_back(ShapeRun{sh, t, t + 1});
+      else runs.back().end = t + 1;
     }
   }
   return ok_status();
@@ -266,12 +267,12 @@ int plan_build(PlanObj& pl) {
 
 // Tiles [b, e) of class c: per shape run of class 0 the specialised kernel, else the generic one.
 hipError_t launch_range(PlanObj& pl, int c, int b, int e, const LaunchConfig& lc, hipStream_t stream) {
-  if (c != 0 || pl.shape_runs.empty()) return launch_forward(pl.dps[c], b, e, lc, stream);
-  for (const ShapeRun& r : pl.shape_runs) {
+  if (pl.shape_runs[c].empty()) return launch_forward(pl.dps[c], b, e, lc, stream);
+  for (const ShapeRun& r : pl.shape_runs[c]) {
     const int lo = std::max(b, r.begin), hi = std::min(e, r.end);
     if (lo >= hi) continue;
-    hipError_t err = r.sid >= 0 ? launch_forward_shape(pl.dps[0], r.sid, lo, hi, lc, stream)
-                                : launch_forward(pl.dps[0], lo, hi, lc, stream);
+    hipError_t err = r.sid >= 0 ? launch_forward_shape(pl.dps[c], r.sid, lo, hi, lc, stream)
+                                : launch_forward(pl.dps[c], lo, hi, lc, stream);
     if (err != hipSuccess) return err;
   }
   return hipSuccess;
@@ -470,9 +471,9 @@ int gpu_phs_plan_set_config(int plan, int block, int ept, int grid, int order, i
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
   if (block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 || order < 0 || order > 1 ||
-      !(waves_per_eu == 0 || waves_per_eu == 3 || waves_per_eu == 4) || lds < 0 || lds > 3)
+      !(waves_per_eu == 0 || (waves_per_eu >= 3 && waves_per_eu <= 5)) || lds < 0 || lds > 3)
     return set_error(GPU_PHS_E_ARG,
-                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4}, "
+                     "block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, order in {0,1}, waves_per_eu in {0,3,4,5}, "
                      "lds in {0,1,2,3}");
   LaunchConfig want;
   want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds = lds;
@@ -497,9 +498,9 @@ int gpu_phs_plan_set_class_config(int plan, int cls, int block, int ept, int gri
   PlanObj* pl = find_plan(plan);
   if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
   if (cls < 0 || cls >= kTileClasses || block < 64 || block > 256 || block % 64 || ept < 1 || ept > 16 || grid < 0 ||
-      !(waves_per_eu == 0 || waves_per_eu == 3 || waves_per_eu == 4) || lds < 0 || lds > 3)
+      !(waves_per_eu == 0 || (waves_per_eu >= 3 && waves_per_eu <= 5)) || lds < 0 || lds > 3)
     return set_error(GPU_PHS_E_ARG, "class in {0,1}, block in {64,128,192,256}, 1 <= ept <= 16, grid >= 0, "
-                                    "waves_per_eu in {0,3,4}, lds in {0,1,2,3}");
+                                    "waves_per_eu in {0,3,4,5}, lds in {0,1,2,3}");
   LaunchConfig want;
   want.block = block; want.ept = ept; want.grid = grid; want.waves_per_eu = waves_per_eu; want.lds = lds;
   if (pl->class_end[cls] > pl->class_begin[cls] && forward_lds_bytes(pl->dps[cls], want) > forward_lds_limit())
@@ -674,14 +675,19 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
     };
     const bool banks = pl->dps[cls].n_banks > 0;
     // the shape-specialised kernels keep randoms and banks in registers: where only they run,
-    // the placement flags change nothing
-    bool placement = cls != 0 || pl->shape_runs.empty();
-    for (const ShapeRun& r : pl->shape_runs) placement = placement || r.sid < 0;
+    // the placement flags change nothing; those of up to four particles also have a budget for
+    // 5 waves per SIMD, the larger ones only the compiler's
+    bool placement = pl->shape_runs[cls].empty(), small = false;
+    for (const ShapeRun& r : pl->shape_runs[cls]) {
+      placement = placement || r.sid < 0;
+      small = small || (r.sid >= 0 && kShapes[r.sid].n_out <= 4);
+    }
+    const int wpe_max = placement ? 4 : (small ? 5 : 0);
     // stage 1: register budget (occupancy target), where deferred boosts wait (lds bit 0) and
     // where the randoms are kept (lds bit 1)
-    for (int wpe : {0, 3, 4})
+    for (int wpe : {0, 3, 4, 5})
       for (int lds : {0, 1, 2, 3})
-        if (!rc && (!(lds & 1) || banks) && (placement || lds == 0)) {
+        if (!rc && wpe <= wpe_max && (!(lds & 1) || banks) && (placement || lds == 0)) {
           LaunchConfig lc = best.lc;
           lc.waves_per_eu = wpe; lc.lds = lds; lc.grid = 0;
           rc = consider(lc);
@@ -701,9 +707,10 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
     // and placement depend on it
     if (best.lc.block != base.block || best.lc.ept != base.ept) {
       const LaunchConfig shape = best.lc;
-      for (int wpe : {0, 3, 4})
+      for (int wpe : {0, 3, 4, 5})
         for (int lds : {0, 1, 2, 3})
-          if (!rc && (!(lds & 1) || banks) && (placement || lds == 0) && !(wpe == shape.waves_per_eu && lds == shape.lds)) {
+          if (!rc && wpe <= wpe_max && (!(lds & 1) || banks) && (placement || lds == 0) &&
+              !(wpe == shape.waves_per_eu && lds == shape.lds)) {
             LaunchConfig lc = shape;
             lc.waves_per_eu = wpe; lc.lds = lds;
             rc = consider(lc);

@@ -1,5 +1,5 @@
 // Compile-time tree shapes: the kernel programs (phs_types.h NodeCmd sequences) of every
-// ordered binary tree with 2, 3 or 4 final-state particles, reduced to what does not depend on
+// ordered binary tree with 2 to 6 final-state particles, reduced to what does not depend on
 // the maps, masses or particle labels -- which daughters are final-state particles, which one
 // the walk descends into and which one it defers, and the random index of every msq and angle.
 //
@@ -10,19 +10,19 @@
 // channel's program with this table entry by entry (shape_of); a channel that matches none runs
 // the generic kernel, which reads the same fields from its program.
 //
-// The entries are the programs tree_compiler.cpp emits for the trees named in the comments
-// (preorder: 1 = branch, 0 = final-state particle): random indices follow the reference's
-// consumption order (msq randoms of the internal non-root branches in post-order,
-// phs_cpu.cpp:180-209, then phi and cos theta per branch in pre-order, phs_cpu.cpp:301-311),
-// and the walk descends into the daughter with fewer final-state particles.
+// The table (phs_shape_table.h) is written by gen_shapes.py: the programs tree_compiler.cpp
+// emits for the trees named in its comments (preorder: 1 = branch, 0 = final-state particle):
+// random indices follow the reference's consumption order (msq randoms of the internal
+// non-root branches in post-order, phs_cpu.cpp:180-209, then phi and cos theta per branch in
+// pre-order, phs_cpu.cpp:301-311), and the walk descends into the daughter with fewer
+// final-state particles, so no tree of up to seven particles defers more than one subtree at a
+// time (banks <= 1).
 #pragma once
 #include <stdint.h>
 
 #include "phs_types.h"
 
 namespace gphs {
-
-constexpr int kShapeMaxNodes = 3;
 
 struct ShapeNode {
   int8_t leaf0, leaf1;  // daughter 0 / 1 is a final-state particle
@@ -32,29 +32,17 @@ struct ShapeNode {
 };
 
 struct ShapeProg {
-  int8_t n_out, n_nodes, n_x, banks;
-  ShapeNode node[kShapeMaxNodes];
+  int8_t n_out, n_nodes, n_x, banks, root_only;
+  ShapeNode node[5];
 };
 
-constexpr int kNumShapes = 8;
-constexpr ShapeProg kShapes[kNumShapes] = {
-    // 100: (1 2)
-    {2, 1, 2, 0, {{1, 1, -1, -1, -1, -1, 0}}},
-    // 11000: ((1 2) 3)
-    {3, 2, 5, 0, {{0, 1, 0, -1, 0, -1, 1}, {1, 1, -1, -1, -1, -1, 3}}},
-    // 10100: (1 (2 3))
-    {3, 2, 5, 0, {{1, 0, 1, -1, -1, 0, 1}, {1, 1, -1, -1, -1, -1, 3}}},
-    // 1110000: (((1 2) 3) 4)
-    {4, 3, 8, 0, {{0, 1, 0, -1, 1, -1, 2}, {0, 1, 0, -1, 0, -1, 4}, {1, 1, -1, -1, -1, -1, 6}}},
-    // 1101000: ((1 (2 3)) 4)
-    {4, 3, 8, 0, {{0, 1, 0, -1, 1, -1, 2}, {1, 0, 1, -1, -1, 0, 4}, {1, 1, -1, -1, -1, -1, 6}}},
-    // 1100100: ((1 2) (3 4)): the root defers its second daughter
-    {4, 3, 8, 1, {{0, 0, 0, 1, 0, 1, 2}, {1, 1, -1, -1, -1, -1, 4}, {1, 1, -1, -1, -1, -1, 6}}},
-    // 1011000: (1 ((2 3) 4))
-    {4, 3, 8, 0, {{1, 0, 1, -1, -1, 1, 2}, {0, 1, 0, -1, 0, -1, 4}, {1, 1, -1, -1, -1, -1, 6}}},
-    // 1010100: (1 (2 (3 4)))
-    {4, 3, 8, 0, {{1, 0, 1, -1, -1, 1, 2}, {1, 0, 1, -1, -1, 0, 4}, {1, 1, -1, -1, -1, -1, 6}}},
-};
+}  // namespace gphs
+
+#include "phs_shape_table.h"
+
+namespace gphs {
+
+static_assert(kShapeMaxNodes <= 5, "ShapeProg holds five branches");
 
 // The shape of a channel's program, or -1 (trees with friends always run the generic kernel).
 inline int shape_of(const ChannelDesc& cd, const NodeCmd* prog) {

@@ -27,8 +27,8 @@ struct ProcessObj {
 
 constexpr int kTileClasses = 2;
 
-// A run of consecutive tiles of tile class 0 whose channels share one tree shape (or -1:
-// channels without a shape, or the generic kernel asked for).
+// A run of consecutive tiles of one tile class whose channels share one tree shape (or -1:
+// channels without a shape).
 struct ShapeRun {
   int sid, begin, end;
 };
@@ -37,8 +37,8 @@ struct PlanObj {
   std::vector<gpu_phs_segment> user;       // segments as given
   std::vector<const double*> beam_e1, beam_e2;   // per-event beam energies of each segment, or null
   std::vector<double*> seg_msq;            // per-branch msq output of each segment, or null
-  int specialize = 1;                      // 1: tile class 0 runs the shape-specialised kernels
-  std::vector<ShapeRun> shape_runs;        // class 0 split by shape (empty: one generic launch)
+  int specialize = 1;                      // 1: trees with a shape run the shape-specialised kernels
+  std::vector<ShapeRun> shape_runs[2];     // each tile class split by shape (empty: one generic launch)
   std::vector<ChannelDesc> chans;
   std::vector<NodeCmd> nodes;
   std::vector<SegDesc> segs;

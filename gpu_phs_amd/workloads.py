@@ -138,7 +138,7 @@ def ordered_trees(leaves):
 
 
 def process_shapes(n_out: int, per_shape: int = 3) -> Process:
-    """Every ordered tree shape with n_out final-state particles (2, 3 or 4), each with
+    """Every ordered tree shape with n_out final-state particles (2 to 6), each with
     ``per_shape`` seeded labellings of the particles and seeded maps from the whole set
     (Breit-Wigner, step, exponential, t/u-channel, linear), particles 1 and 4 massive (4.8 and
     1.5 GeV), the others massless, at sqrt(s) = 1 TeV: the trees of the shape-specialised
@@ -158,7 +158,7 @@ def process_shapes(n_out: int, per_shape: int = 3) -> Process:
                 m[node] = choices[int(rng.integers(len(choices)))]
             trees.append(t)
             maps.append(m)
-    flv = [M_E, M_E] + [4.8, 0.0, 0.0, 1.5][:n_out]
+    flv = [M_E, M_E] + [4.8, 0.0, 0.0, 1.5, 0.0, 0.0][:n_out]
     return make_process(2, n_out, trees, maps, flv, 1000.0, name=f"2->{n_out}: every tree shape")
 
 

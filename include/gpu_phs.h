@@ -144,7 +144,8 @@ int gpu_phs_plan_destroy(int plan);
 /* Launch shape: workgroup size (64..256, multiple of 64), events per thread per tile
  * (1..16), persistent grid (0 = one workgroup per tile), tile order (0 = segment order,
  * 1 = tiles of the deepest trees first), kernel variant: waves_per_eu = register budget for
- * at least that many waves per SIMD (0 = compiler's choice, 3 or 4), lds = bit flags: 1 keeps
+ * at least that many waves per SIMD (0 = compiler's choice, 3, 4, or 5: shape-specialised
+ * kernels only, others take 5 as 0), lds = bit flags: 1 keeps
  * the boosts of deferred subtrees in LDS instead of VGPRs (trees that defer two levels always
  * use LDS), 2 keeps each event's randoms in LDS instead of VGPRs (always for n_x > 16).  Changing block, ept or order
  * rebuilds the tile table.  A block whose LDS tables (4 n_out rows of block + 1 doubles, n_x

@@ -99,14 +99,14 @@ def test_forward_matches_oracle(name, cfg):
     assert_parity(r)
 
 
-@pytest.mark.parametrize("n_out", [2, 3, 4])
+@pytest.mark.parametrize("n_out", [2, 3, 4, 5, 6])
 def test_shape_kernels_match_generic_and_oracle(n_out):
-    """Every ordered tree of 2-4 final-state particles: the kernel specialised for its shape
+    """Every ordered tree of 2-6 final-state particles: the kernel specialised for its shape
     gives bit for bit the generic kernel's momenta, factors, volumes, ok flags and branch msq,
     and both meet the forward criterion against the oracle."""
     proc = W.process_shapes(n_out)
     rng = np.random.default_rng(17 + n_out)
-    n = 20011
+    n = 20011 if n_out <= 4 else 60011   # a fair share of the deeper trees is kinematically closed
     ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
     x = rng.random((n, proc.n_x))
     print("shapes", n_out, flush=True)
@@ -117,7 +117,7 @@ def test_shape_kernels_match_generic_and_oracle(n_out):
             assert np.array_equal(spec[k], gen[k], equal_nan=k != "ok"), (k, cfg)
     r = forward_check(proc, x, ch, spec)
     print(f"2->{n_out} shapes", end=": ")
-    assert r["n_ok"] > n // 10
+    assert r["n_ok"] > 1000
     assert_parity(r)
 
 

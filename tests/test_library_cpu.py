@@ -67,10 +67,11 @@ def test_process_create(name):
     assert (n_out.value, n_x.value, n_ch.value) == (proc.n_out, proc.n_x, proc.n_channels)
 
 
-@pytest.mark.parametrize("n_out", [2, 3, 4])
+@pytest.mark.parametrize("n_out", [2, 3, 4, 5, 6])
 def test_every_small_tree_has_a_specialised_kernel(n_out):
-    """Each ordered tree shape of 2-4 final-state particles matches its own entry of the
-    compile-time shape table (phs_shapes.h), whatever the particle labels and maps."""
+    """Each ordered tree shape of 2-6 final-state particles matches its own entry of the
+    compile-time shape table (phs_shape_table.h, written by gen_shapes.py), whatever the
+    particle labels and maps: the generated programs agree with the tree compiler's."""
     proc = W.process_shapes(n_out)
     shapes = G.ProcessHandle(proc).shapes()
     n_shapes = len(W.ordered_trees(list(range(n_out))))
@@ -82,9 +83,10 @@ def test_every_small_tree_has_a_specialised_kernel(n_out):
 
 
 def test_larger_trees_and_friends_run_the_generic_kernel():
-    for proc in (W.process_2to6(), W.process_friends(4)):
+    for proc in (W.process_large(8), W.process_friends(4), W.process_friends(6)):
         assert (G.ProcessHandle(proc).shapes() == -1).all()
-    assert (G.ProcessHandle(W.process_c2()).shapes() >= 0).all()
+    for proc in (W.process_c2(), W.process_2to6()):
+        assert (G.ProcessHandle(proc).shapes() >= 0).all()
 
 
 def _expect_tree_error(proc, fragment):
