@@ -218,6 +218,20 @@ int gpu_phs_plan_n_params(int plan);
  * width it refers to.  Arrays hold at least n_params / 2 entries. */
 int gpu_phs_plan_param_map(int plan, int* process, int* channel, int* bincode, int n);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).  The forward pass of
+ * n_events events of one channel (0-based) of a process, as a one-segment plan would compute them
+ * (same kernels, same results), with device buffers x [n_events][n_x], p [n_events][n_out][4],
+ * factors, volumes [n_events] and oks [n_events], on `stream`.  It allocates nothing: the call's
+ * tables go into a static 1 MiB device array that every GPU the library runs on holds until the
+ * process ends, which limits a call to 64,512 tiles (66,060,288 events of a tree of up to four
+ * final-state particles, 16,515,072 of a larger one); larger calls are rejected with
+ * GPU_PHS_E_ARG.
+ * All calls on one GPU share that array and overwrite it: calls that overlap on one GPU race, so
+ * order them on one stream. */
+int gpu_phs_generate_noalloc(int process, int channel, int64_t n_events, const double* x, double* p,
+                             double* factors, double* volumes, uint8_t* oks, void* stream);
+
 /* Inverse mapping (replaces the do_inverse_mapping branch of gen_phs_from_x_gpu,
  * phs.cu:1491-1554, and its kernels phs.cu:626-675, 924-1284): for every event of the
  * plan's segments and every channel c of their process, the randoms x_out and the factor
