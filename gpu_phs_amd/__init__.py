@@ -16,7 +16,8 @@ import numpy as np
 from .process import Process  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpu_phs.so")
+# GPU_PHS_LIB selects another build of the library (e.g. the sanitizer build of csrc/Makefile)
+LIB_PATH = os.environ.get("GPU_PHS_LIB") or os.path.join(HERE, "libgpu_phs.so")
 CSRC = os.path.join(HERE, "csrc")
 
 GPU_PHS_OK = 0

@@ -5,6 +5,7 @@
 // module object, so the calls keep their order-dependent semantics.
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <cstdio>
 #include <cstring>
 
@@ -33,7 +34,18 @@ struct WhizardModule {
   double* fa_d = nullptr; double* xo_d = nullptr;
   size_t inv_cap = 0;               // doubles of xo_d
   double* last_p = nullptr;         // device momenta of the last generate call
+  // the plan of the last call, reused (tables, inverse tables) while its segments repeat
+  std::unique_ptr<PlanObj> plan;
+  std::vector<gpu_phs_segment> plan_segs;
+  bool plan_varying = false;
+  hipStream_t stream = nullptr;     // the module's own stream (non-blocking, default priority)
+  // pinned staging of the host copies: two slots, so the host-side copy of one chunk overlaps
+  // the DMA of the other
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
+
+constexpr size_t kPinChunk = (size_t)8 << 20;
 
 WhizardModule& W() {
   static WhizardModule w;
@@ -41,6 +53,8 @@ WhizardModule& W() {
 }
 
 void free_buffers(WhizardModule& w) {
+  w.plan.reset();   // its segments point into the buffers
+  w.plan_segs.clear();
   (void)hipFree(w.x_d); (void)hipFree(w.p_d); (void)hipFree(w.f_d); (void)hipFree(w.v_d); (void)hipFree(w.ok_d);
   (void)hipFree(w.fa_d); (void)hipFree(w.xo_d); (void)hipFree(w.e_d);
   w.x_d = w.p_d = w.f_d = w.v_d = nullptr;
@@ -83,6 +97,68 @@ int ensure_inverse_buffers(WhizardModule& w, size_t n) {
   }
   w.inv_cap = need;
   return GPU_PHS_OK;
+}
+
+void free_staging(WhizardModule& w) {
+  if (w.stream) (void)hipStreamSynchronize(w.stream);
+  for (int k = 0; k < 2; k++) {
+    if (w.pin[k]) (void)hipHostFree(w.pin[k]);
+    if (w.pin_ev[k]) (void)hipEventDestroy(w.pin_ev[k]);
+    w.pin[k] = nullptr;
+    w.pin_ev[k] = nullptr;
+  }
+  if (w.stream) (void)hipStreamDestroy(w.stream);
+  w.stream = nullptr;
+}
+
+int ensure_staging(WhizardModule& w) {
+  if (w.stream) return GPU_PHS_OK;
+  hipError_t e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { w.stream = nullptr; return hip_error(e, "hipStreamCreate"); }
+  for (int k = 0; k < 2; k++) {
+    if ((e = hipHostMalloc(&w.pin[k], kPinChunk, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&w.pin_ev[k], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventRecord(w.pin_ev[k], w.stream)) != hipSuccess) {
+      free_staging(w);
+      return hip_error(e, "pinned staging buffers");
+    }
+  }
+  return GPU_PHS_OK;
+}
+
+// Host -> device through the pinned slots: each chunk is copied into a free slot on the host and
+// sent by DMA on the module's stream while the next chunk is copied into the other slot.
+int h2d(WhizardModule& w, void* dst, const void* src, size_t bytes) {
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += kPinChunk, k ^= 1) {
+    const size_t n = std::min(kPinChunk, bytes - off);
+    hipError_t e = hipEventSynchronize(w.pin_ev[k]);   // the slot's previous DMA is done
+    if (e != hipSuccess) return hip_error(e, "hipEventSynchronize");
+    std::memcpy(w.pin[k], (const char*)src + off, n);
+    if ((e = hipMemcpyAsync((char*)dst + off, w.pin[k], n, hipMemcpyHostToDevice, w.stream)) != hipSuccess ||
+        (e = hipEventRecord(w.pin_ev[k], w.stream)) != hipSuccess)
+      return hip_error(e, "hipMemcpyAsync(host to device)");
+  }
+  return GPU_PHS_OK;
+}
+
+// Device -> host (after the work queued on the module's stream): chunk i + 1 is sent by DMA into
+// one slot while chunk i is copied out of the other on the host.
+int d2h(WhizardModule& w, void* dst, const void* src, size_t bytes) {
+  const size_t n_chunks = (bytes + kPinChunk - 1) / kPinChunk;
+  auto issue = [&](size_t i) -> hipError_t {
+    const size_t off = i * kPinChunk, n = std::min(kPinChunk, bytes - off);
+    hipError_t e = hipMemcpyAsync(w.pin[i % 2], (const char*)src + off, n, hipMemcpyDeviceToHost, w.stream);
+    return e != hipSuccess ? e : hipEventRecord(w.pin_ev[i % 2], w.stream);
+  };
+  hipError_t e = n_chunks ? issue(0) : hipSuccess;
+  for (size_t i = 0; i < n_chunks && e == hipSuccess; i++) {
+    if (i + 1 < n_chunks && (e = issue(i + 1)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(w.pin_ev[i % 2])) != hipSuccess) break;
+    const size_t off = i * kPinChunk;
+    std::memcpy((char*)dst + off, w.pin[i % 2], std::min(kPinChunk, bytes - off));
+  }
+  return e == hipSuccess ? GPU_PHS_OK : hip_error(e, "device to host copy");
 }
 
 int compile_for_beams(WhizardModule& w, double E1, double E2) {
@@ -248,6 +324,7 @@ void c_whizard_finalize_gpu_phs(void) {
   std::lock_guard<std::mutex> lk(api_mutex());
   WhizardModule& w = W();
   free_buffers(w);
+  free_staging(w);
   if (w.process >= 0) drop_process(w.process);
   w = WhizardModule();
   ok_status();
@@ -282,16 +359,13 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
     if (E1_in[i] != E1 || E2_in[i] != E2) varying = true;
   }
   if (compile_for_beams(w, E1, E2)) return;
+  if (w.plan && !w.plan_segs.empty() && w.plan_segs[0].process != w.process) { w.plan.reset(); w.plan_segs.clear(); }
   if (ensure_buffers(w, (size_t)n)) return;
-  hipError_t e = hipMemcpy(w.x_d, x, (size_t)n * w.n_x * sizeof(double), hipMemcpyHostToDevice);
-  if (e != hipSuccess) { hip_error(e, "hipMemcpy(x)"); return; }
+  if (ensure_staging(w)) return;
+  if (h2d(w, w.x_d, x, (size_t)n * w.n_x * sizeof(double))) return;
   // the beam energies stay on the device, E1 then E2, as the reference keeps them
   // (get_in_momentum_device_pointer, phs.cu:1361-1363); per-event beams read them from there
-  if ((e = hipMemcpy(w.e_d, E1_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(w.e_d + n, E2_in, (size_t)n * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
-    hip_error(e, "hipMemcpy(beam energies)");
-    return;
-  }
+  if (h2d(w, w.e_d, E1_in, (size_t)n * sizeof(double)) || h2d(w, w.e_d + n, E2_in, (size_t)n * sizeof(double))) return;
 
   // segments = runs of equal channel id
   std::vector<gpu_phs_segment> segs;
@@ -310,34 +384,49 @@ void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_b
     segs.push_back(s);
     i = j;
   }
-  PlanObj pl;
-  pl.user = segs;
-  if (varying) {
-    for (size_t k = 0, r = 0; k < segs.size(); r += segs[k].n_events, k++) {
-      pl.beam_e1.push_back(w.e_d + r);
-      pl.beam_e2.push_back(w.e_d + n + r);
+  // the plan (and its inverse tables) is kept while the segments and the beam mode repeat
+  const bool same = w.plan && w.plan_varying == varying && w.plan_segs.size() == segs.size() &&
+                    std::memcmp(w.plan_segs.data(), segs.data(), segs.size() * sizeof(gpu_phs_segment)) == 0;
+  if (!same) {
+    if (!w.plan) w.plan = std::make_unique<PlanObj>();
+    PlanObj& pl = *w.plan;
+    pl.user = segs;
+    pl.beam_e1.clear();
+    pl.beam_e2.clear();
+    if (varying) {
+      for (size_t k = 0, r = 0; k < segs.size(); r += segs[k].n_events, k++) {
+        pl.beam_e1.push_back(w.e_d + r);
+        pl.beam_e2.push_back(w.e_d + n + r);
+      }
     }
+    // each tile class keeps its default shape with the block capped at set_threads' value,
+    // and smaller where the LDS tables of the deepest trees would not fit
+    const PlanObj defaults;
+    for (int c = 0; c < kTileClasses; c++) pl.lcs[c].block = std::min(defaults.lcs[c].block, w.block);
+    if (plan_build(pl)) { w.plan.reset(); return; }
+    for (int c = 0; c < kTileClasses; c++)
+      while (pl.lcs[c].block > 64 && forward_lds_bytes(pl.dps[c], pl.lcs[c]) > forward_lds_limit()) {
+        pl.lcs[c].block -= 64;
+        if (plan_build(pl)) { w.plan.reset(); return; }
+      }
+    w.plan_segs = segs;
+    w.plan_varying = varying;
   }
-  for (int c = 0; c < kTileClasses; c++) pl.lcs[c].block = w.block;
-  if (plan_build(pl)) return;
-  if (plan_run(pl, nullptr)) return;
+  PlanObj& pl = *w.plan;
+  if (plan_run(pl, w.stream)) return;
   // the inverse mapping of every event into every channel (the reference always asks for it,
   // whizard_gpu_phs.cpp:118): factors [n][n_channels] at [n_channels i + c] (phs.cu:207) and
   // x_out channel-major [n_x n c + n_x i + k] (phs.cu:222), own channel included
   if (ensure_inverse_buffers(w, (size_t)n)) return;
   std::vector<int64_t> rows(segs.size());
   for (size_t k = 0, r = 0; k < segs.size(); r += segs[k].n_events, k++) rows[k] = (int64_t)r;
-  if (plan_inverse(pl, rows.data(), n, w.fa_d, w.xo_d, 0, nullptr)) return;
+  if (plan_inverse(pl, rows.data(), n, w.fa_d, w.xo_d, 0, w.stream)) return;
   const int nc = w.n_channels;
-  if ((e = hipMemcpy(p, w.p_d, (size_t)n * w.n_out * 4 * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-      (e = hipMemcpy(factors, w.fa_d, (size_t)n * nc * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-      (e = hipMemcpy(volumes, w.v_d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess ||
-      (e = hipMemcpy(oks, w.ok_d, (size_t)n, hipMemcpyDeviceToHost)) != hipSuccess ||
-      (x_out && (e = hipMemcpy(x_out, w.xo_d, (size_t)n * nc * w.n_x * sizeof(double), hipMemcpyDeviceToHost)) !=
-                    hipSuccess)) {
-    hip_error(e, "hipMemcpy(outputs)");
+  if (d2h(w, p, w.p_d, (size_t)n * w.n_out * 4 * sizeof(double)) ||
+      d2h(w, factors, w.fa_d, (size_t)n * nc * sizeof(double)) ||
+      d2h(w, volumes, w.v_d, (size_t)n * sizeof(double)) || d2h(w, oks, w.ok_d, (size_t)n) ||
+      (x_out && d2h(w, x_out, w.xo_d, (size_t)n * nc * w.n_x * sizeof(double))))
     return;
-  }
   w.last_p = w.p_d;
   w.last_n = (size_t)n;
   ok_status();
@@ -378,6 +467,7 @@ void c_whizard_reset_generator(void) {
   std::lock_guard<std::mutex> lk(api_mutex());
   WhizardModule& w = W();
   free_buffers(w);
+  free_staging(w);
   std::fill(w.spec.flv_masses.begin(), w.spec.flv_masses.end(), 0.0);
   std::fill(w.spec.flv_widths.begin(), w.spec.flv_widths.end(), 0.0);
   if (w.process >= 0) { drop_process(w.process); w.process = -1; }

@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+LIB = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")   # ORACLE_LIB: the sanitizer build
 REF_LIB = os.path.join(HERE, "_ref", "libphs_ref.so")
 REF_SRC = "/root/reference"
 

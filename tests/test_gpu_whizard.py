@@ -201,3 +201,41 @@ def test_device_pointers_of_the_last_call():
     assert (beams[0] == E1).all() and (beams[1] == E2).all()
     assert np.array_equal(fetch("_Z21get_ok_device_pointerv", n, np.bool_), oks)
     L.c_whizard_finalize_gpu_phs()
+
+
+def test_whizard_abi_large_trees_at_block_256():
+    """2->8 trees (two levels of LDS save banks) through the C ABI with set_threads(256): each
+    tile class keeps a workgroup its LDS tables fit into."""
+    proc = W.process_large(8)
+    L = setup_module_from(proc, block=256)
+    rng = np.random.default_rng(31)
+    counts = [900] * proc.n_channels
+    n = sum(counts)
+    x = rng.random((n, proc.n_x))
+    h = proc.sqrts / 2
+    p, factors, volumes, oks, _ = gen(L, proc, x, counts, h, h)
+    ch = np.repeat(np.arange(proc.n_channels), counts).astype(np.int32)
+    r = forward_check(proc, x, ch, dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks))
+    assert r["n_ok"] > 100
+    assert_parity(r)
+    L.c_whizard_finalize_gpu_phs()
+
+
+def test_whizard_abi_repeated_calls_reuse_and_resplit():
+    """Calls with the same batch split reuse the module's plan; a new split rebuilds it; every
+    call matches the oracle, and a repeated call gives identical results."""
+    proc = W.process_c1()
+    L = setup_module_from(proc)
+    rng = np.random.default_rng(41)
+    outs = []
+    for counts in ([400, 300, 500], [400, 300, 500], [10, 1000, 3], [400, 300, 500]):
+        n = sum(counts)
+        x = rng.random((n, proc.n_x)) if len(outs) != 1 else outs[0][0]
+        p, factors, volumes, oks, x_out = gen(L, proc, x, counts, 250.0, 250.0)
+        ch = np.repeat(np.arange(3), counts).astype(np.int32)
+        assert_parity(forward_check(proc, x, ch, dict(p=p, factor=factors[np.arange(n), ch], volume=volumes, ok=oks)))
+        outs.append((x, p, factors, oks, x_out))
+    # the second call repeated the first one's randoms and split
+    for a, b in zip(outs[0][1:], outs[1][1:]):
+        assert np.array_equal(a, b, equal_nan=True)
+    L.c_whizard_finalize_gpu_phs()
