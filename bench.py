@@ -98,12 +98,14 @@ def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None, tuning
     return res, plan, segs, wl
 
 
-def run_adjoint(steps, warmup, torch, G, W):
+def run_adjoint(steps, warmup, torch, G, W, specialize=True):
     """c4: forward + reverse mode on the c1 trees, 1e6 events, loss = sum ok * factor * sum pT^2 / s.
     One adjoint launch produces the forward outputs and d loss / d (x, M, Gamma)."""
     wl = W.workload("c4")
     handles, segs = make_inputs(wl, torch, G)
     plan = G.Plan(segs)
+    if not specialize:
+        plan.set_specialize(False)
     plan.autotune(reps=3)
     gx = [torch.empty_like(b.x) for _, _, b in segs]
     loss = [torch.empty(b.x.shape[0], dtype=torch.float64, device="cuda") for _, _, b in segs]
@@ -380,7 +382,7 @@ def main():
                           "write_expected_kib": n * 8 / 1024}))
         return
     if args.adjoint_only:
-        print(json.dumps(run_adjoint(max(5, args.steps), args.warmup, torch, G, W)))
+        print(json.dumps(run_adjoint(max(5, args.steps), args.warmup, torch, G, W, specialize=not args.no_specialize)))
         return
     if args.grid_only:
         print(json.dumps(run_grid(max(3, args.steps), torch, G, W)))

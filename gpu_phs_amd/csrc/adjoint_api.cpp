@@ -145,12 +145,19 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
       return set_error(GPU_PHS_E_ARG, "adjoint: the built-in loss assumes symmetric beams");
   int rc = adjoint_prepare(*pl, adj, write_forward);
   if (rc) return rc;
-  // one workgroup per 64-event unit: a persistent grid measured slower (its workgroups do not
-  // all fit the LDS at once), and units without events would cost a workgroup each
-  hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, loss_mode, pl->adj_part, pl->adj_part_stride, pl->adj_sub,
-                                pl->adj_ranges, pl->adj_done, pl->adj_chan_tiles, pl->adj_chan_off,
-                                (int)pl->chans.size(), gparam,
-                                0, (hipStream_t)stream);
+  // the tile table by class and, within a class, by tree shape (the forward launches' runs):
+  // runs of a shape of up to four final-state particles take the adjoint specialised for it.
+  // One workgroup per 64-event unit: a persistent grid measured slower (its workgroups do not
+  // all fit the LDS at once), and units without events would cost a workgroup each.
+  std::vector<TileRun> runs;
+  for (int c = 0; c < kTileClasses; c++) {
+    if (pl->class_end[c] <= pl->class_begin[c]) continue;
+    if (pl->shape_runs[c].empty()) runs.push_back(TileRun{-1, pl->class_begin[c], pl->class_end[c]});
+    else runs.insert(runs.end(), pl->shape_runs[c].begin(), pl->shape_runs[c].end());
+  }
+  hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, runs.data(), (int)runs.size(), loss_mode, pl->adj_part,
+                                pl->adj_part_stride, pl->adj_sub, pl->adj_ranges, pl->adj_done, pl->adj_chan_tiles,
+                                pl->adj_chan_off, (int)pl->chans.size(), gparam, (hipStream_t)stream);
   if (e != hipSuccess) return hip_error(e, "adjoint launch");
   return ok_status();
 }

@@ -17,6 +17,7 @@
 #include "phs_kernels.h"
 #include "phs_math.h"
 #include "phs_types.h"
+#include "phs_walk.h"
 
 namespace gphs {
 
@@ -252,19 +253,21 @@ GPHS_HD constexpr int compact_stash_slot(int k) {
 
 
 // P = product of the L1 of the ancestors of branch j (root first): the boost its own L1 is
-// composed onto.  The ancestors are found by walking the parent links (a few scalar loads).
-template <class Store>
+// composed onto.  The ancestors are found by walking the parent links (a few scalar loads, or
+// constants in a kernel specialised for the tree's shape).
+template <int SID, class Store>
 GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16]) {
+  using W = Walk<SID>;
   int d = 0;
-  for (int a = prog[j].parent; a >= 0; a = prog[a].parent) d++;
+  for (int a = W::parent(prog, j); a >= 0; a = W::parent(prog, a)) d++;
   if (d == 0) {
 #pragma unroll
     for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
     return;
   }
   // the root's L1 itself (the identity times L1: the same values), then the others
-  int r = prog[j].parent;
-  for (int u = 0; u < d - 1; u++) r = prog[r].parent;
+  int r = W::parent(prog, j);
+  for (int u = 0; u < d - 1; u++) r = W::parent(prog, r);
   {
     const double g = S.T(r, 0), bg = S.T(r, 1), ct = S.T(r, 2), st = S.T(r, 3), cp = S.T(r, 4), sp = S.T(r, 5);
     P[0] = g;   P[1] = -bg * st; P[2] = 0;   P[3] = bg * ct;
@@ -273,8 +276,8 @@ GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16])
     P[12] = bg; P[13] = -g * st; P[14] = 0;  P[15] = g * ct;
   }
   for (int level = 1; level < d; level++) {
-    int a = prog[j].parent;
-    for (int u = 0; u < d - 1 - level; u++) a = prog[a].parent;
+    int a = W::parent(prog, j);
+    for (int u = 0; u < d - 1 - level; u++) a = W::parent(prog, a);
     compose_l1(P, S.T(a, 0), S.T(a, 1), S.T(a, 2), S.T(a, 3), S.T(a, 4), S.T(a, 5));
   }
 }
@@ -286,25 +289,33 @@ GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16])
 //   (mode 1), gx [n_x] (zero for a non-ok event), and adds d/d(M, Gamma) of every branch to
 //   gpar[2 * node], gpar[2 * node + 1] (visit index; `GPar` is any indexable accumulator).
 //   Returns ok.
-template <class Store, class GPar, class XR>
-GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR& xr, int loss_mode,
-                           const double* gp, double gf_in, double* p_out, double* f_out, double* loss_out,
-                           double* gx, GPar& gpar, Store& S) {
+//   SID >= 0: the program has that compile-time tree shape (phs_walk.h); both sweeps are then
+//   unrolled and every link of the walk is a constant.  LM is the loss mode; with LM = 1 the
+//   beams are symmetric (the entry point checks it), so no lab boost, and the cotangents of
+//   every leaf's energy and longitudinal momentum are zero, which the sweep leaves out.
+template <int SID, int LM, class Store, class GPar, class XR>
+GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR& xr, const double* gp,
+                           double gf_in, double* p_out, double* f_out, double* loss_out, double* gx, GPar& gpar,
+                           Store& S) {
+  constexpr int loss_mode = LM;
+  using Wk = Walk<SID>;
   const double s = cd.s;
-  const int N = cd.n_nodes;
+  const int N = Wk::n_nodes(cd);
   // ---------------- forward (phs_forward.hip, plus the tape)
   double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   double cur_msq = s, cur_m = cd.sqrts, cur_p = 0, F = 1, S2 = 0;
   bool ok = true;
   int depth = 0;
-  for (int i = 0; i < N; i++) {
+  auto fwd_branch = [&](const int i) {
     const NodeCmd& nc = prog[i];
+    const bool leaf[2] = {Wk::leaf(nc, i, 0), Wk::leaf(nc, i, 1)};
+    const int push = Wk::push(nc, i), desc = Wk::desc(nc, i);
     S.in(i, 0) = cur_msq; S.in(i, 1) = cur_p;
     double msq[2], f[2], mk[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (nc.k[c].leaf) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
-      else { MapEval r = msq_eval(nc.k[c], xr[nc.k[c].xi], s); msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq); }
+      if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
+      else { MapEval r = msq_eval(nc.k[c], xr[Wk::xi(nc, i, c)], s); msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq); }
     }
     const double m = cur_m;
     const double lda = (cur_msq - msq[0] - msq[1]) * (cur_msq - msq[0] - msq[1]) - 4 * msq[0] * msq[1];
@@ -315,19 +326,19 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     F *= f[0] * f[1] * math::quot(sl, cur_msq);
     const double bg = m > 0 ? math::quot(cur_p, m) : 0.0, g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
-    math::sincos_turns(xr[nc.x_phi], sp, cp);
-    const CtEval ce = ct_eval(nc.ct_kind, xr[nc.x_phi + 1], nc.b0, nc.b1);
+    math::sincos_turns(xr[Wk::xphi(nc, i)], sp, cp);
+    const CtEval ce = ct_eval(nc.ct_kind, xr[Wk::xphi(nc, i) + 1], nc.b0, nc.b1);
     F *= ce.f;
     compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
     S.T(i, 0) = g; S.T(i, 1) = bg; S.T(i, 2) = ce.ct; S.T(i, 3) = ce.st; S.T(i, 4) = cp; S.T(i, 5) = sp;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (!nc.k[c].leaf) continue;
+      if (!leaf[c]) continue;
       const double pc = c == 0 ? pk : -pk;
       const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
       double q0 = __builtin_fma(M[3], pc, M[0] * E), q1 = __builtin_fma(M[7], pc, M[4] * E), q2 = __builtin_fma(M[11], pc, M[8] * E),
              q3 = __builtin_fma(M[15], pc, M[12] * E);
-      if (cd.bg_lab != 0) {
+      if (LM == 0 && cd.bg_lab != 0) {
         const double b0 = cd.g_lab * q0 + cd.bg_lab * q3, b3 = cd.bg_lab * q0 + cd.g_lab * q3;
         q0 = b0; q3 = b3;
       }
@@ -337,19 +348,25 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
         o[0] = q0; o[1] = q1; o[2] = q2; o[3] = q3;
       }
     }
-    if (nc.push >= 0) {   // the deferred daughter's kinematics; its boost is re-formed at the pop
-      const int j = nc.k[nc.push].node;
-      S.in(j, 0) = msq[nc.push];
-      S.in(j, 1) = nc.push == 0 ? pk : -pk;
+    if (push >= 0) {   // the deferred daughter's kinematics; its boost is re-formed at the pop
+      const int j = Wk::child(nc, i, push);
+      S.in(j, 0) = msq[push];
+      S.in(j, 1) = push == 0 ? pk : -pk;
       depth++;
     }
-    if (nc.desc >= 0) {
-      cur_msq = msq[nc.desc]; cur_m = mk[nc.desc]; cur_p = nc.desc == 0 ? pk : -pk;
+    if (desc >= 0) {
+      cur_msq = msq[desc]; cur_m = mk[desc]; cur_p = desc == 0 ? pk : -pk;
     } else if (depth > 0 && i + 1 < N) {   // the next branch in visit order is the deferred one
       depth--;
-      parent_boost(prog, i + 1, S, M);
+      parent_boost<SID>(prog, i + 1, S, M);
       cur_msq = S.in(i + 1, 0); cur_p = S.in(i + 1, 1); cur_m = math::sqrt_phys(cur_msq);
     }
+  };
+  if constexpr (SID >= 0) {
+#pragma unroll
+    for (int i = 0; i < Wk::n_nodes(cd); i++) fwd_branch(i);
+  } else {
+    for (int i = 0; i < N; i++) fwd_branch(i);
   }
   *f_out = F;
   for (int k = 0; k < cd.n_x; k++) gx[k] = 0;
@@ -366,39 +383,41 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
   // ---------------- reverse sweep
   double Gc[16] = {0};       // adjoint of the parent boost, from the branch just reversed
   double gc_msq = 0, gc_m = 0, gc_p = 0;
-  for (int i = N - 1; i >= 0; i--) {
+  auto rev_branch = [&](const int i) {
     const NodeCmd& nc = prog[i];
+    const bool leaf[2] = {Wk::leaf(nc, i, 0), Wk::leaf(nc, i, 1)};
+    const int push = Wk::push(nc, i), desc = Wk::desc(nc, i);
     double G[16] = {0};
     double g_msq[2] = {0, 0}, g_m[2] = {0, 0}, g_pk = 0;
-    if (nc.desc >= 0) {
+    if (desc >= 0) {
 #pragma unroll
       for (int q = 0; q < 16; q++) G[q] += Gc[q];
-      g_msq[nc.desc] += gc_msq; g_m[nc.desc] += gc_m; g_pk += nc.desc == 0 ? gc_p : -gc_p;
+      g_msq[desc] += gc_msq; g_m[desc] += gc_m; g_pk += desc == 0 ? gc_p : -gc_p;
     }
-    if (nc.push >= 0) {
+    if (push >= 0) {
       depth--;
 #pragma unroll
       for (int q = 0; q < 16; q++) G[q] += S.st_get(depth, q);
-      g_msq[nc.push] += S.st_get(depth, 16); g_m[nc.push] += S.st_get(depth, 17);
-      g_pk += nc.push == 0 ? S.st_get(depth, 18) : -S.st_get(depth, 18);
+      g_msq[push] += S.st_get(depth, 16); g_m[push] += S.st_get(depth, 17);
+      g_pk += push == 0 ? S.st_get(depth, 18) : -S.st_get(depth, 18);
     }
     // the branch's local quantities: boost parameters from the tape, maps recomputed
     const double a = S.in(i, 0), p0 = S.in(i, 1);
     const double m = i == 0 ? cd.sqrts : math::sqrt_phys(a);
     double P[16];
-    parent_boost(prog, i, S, P);
+    parent_boost<SID>(prog, i, S, P);
     MapEval r[2];
     double msq[2], f[2], mk[2], xk[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (nc.k[c].leaf) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
-      else { xk[c] = xr[nc.k[c].xi]; r[c] = msq_eval(nc.k[c], xk[c], s); msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]); }
+      if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
+      else { xk[c] = xr[Wk::xi(nc, i, c)]; r[c] = msq_eval(nc.k[c], xk[c], s); msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]); }
     }
     const double d = a - msq[0] - msq[1];
     const double lda = d * d - 4 * msq[0] * msq[1];
     const double sl = math::sqrt_phys(lda), pk = math::quot(sl, 2 * m);
     const double g = S.T(i, 0), bg = S.T(i, 1), cp = S.T(i, 4), sp = S.T(i, 5);
-    const double xct = xr[nc.x_phi + 1];
+    const double xct = xr[Wk::xphi(nc, i) + 1];
     CtEval ce;
     if (nc.ct_kind == CK_FLAT) {   // from the tape: sin theta as the forward sweep formed it
       ce.tmp = 2 * (1 - xct);
@@ -424,7 +443,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     // final-state daughters: q = Ma[:,0] E + Ma[:,3] pc
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (!nc.k[c].leaf) continue;
+      if (!leaf[c]) continue;
       const double pc = c == 0 ? pk : -pk;
       const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
       // cotangent of the lab-frame momentum, moved back through the (linear) lab boost
@@ -438,7 +457,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       } else {
         gq[0] = gq[1] = gq[2] = gq[3] = 0;
       }
-      if (cd.bg_lab != 0) {
+      if (LM == 0 && cd.bg_lab != 0) {
         const double a0 = gq[0], a3 = gq[3];
         gq[0] = cd.g_lab * a0 + cd.bg_lab * a3;
         gq[3] = cd.bg_lab * a0 + cd.g_lab * a3;
@@ -446,6 +465,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       double gE = 0, gpc = 0;
 #pragma unroll
       for (int mu = 0; mu < 4; mu++) {
+        if (LM == 1 && (mu == 0 || mu == 3)) continue;   // zero cotangents
         G[4 * mu + 0] = __builtin_fma(gq[mu], E, G[4 * mu + 0]);
         G[4 * mu + 3] = __builtin_fma(gq[mu], pc, G[4 * mu + 3]);
         gE = __builtin_fma(gq[mu], Ma[4 * mu + 0], gE);
@@ -487,12 +507,12 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     const double gcp = g11 * ce.ct + g13 * ce.st + g22;
     const double gsp = -g12 + g21 * ce.ct + g23 * ce.st;
     // phi = 2 pi x
-    gx[nc.x_phi] = (gsp * cp - gcp * sp) * kTwoPi;
+    gx[Wk::xphi(nc, i)] = (gsp * cp - gcp * sp) * kTwoPi;
     // cos theta map of this branch (its own M through b0, b1)
     const double gfct = ce.f != 0 ? math::quot(W, ce.f) : 0.0;
     double gxc, gb0, gb1;
     ct_eval_adj(nc.ct_kind, xct, nc.b0, nc.b1, ce, gct, gst, gfct, gxc, gb0, gb1);
-    gx[nc.x_phi + 1] = gxc;
+    gx[Wk::xphi(nc, i) + 1] = gxc;
     gpar[2 * i] += gb0 * nc.db_dm[0] + gb1 * nc.db_dm[1];
     // gamma = sqrt(1 + bg^2), bg = p / m
     gbg += math::quot(gg * bg, g);
@@ -520,22 +540,28 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     // internal daughters: m_k = sqrt(msq_k), then their msq maps
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      if (nc.k[c].leaf) continue;
+      if (leaf[c]) continue;
       if (mk[c] > 0) g_msq[c] += math::quot(g_m[c], 2 * mk[c]);
       double gxk = 0, gM = 0, gW = 0;
       msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
-      gx[nc.k[c].xi] = gxk;
-      gpar[2 * nc.k[c].node] += gM;
-      gpar[2 * nc.k[c].node + 1] += gW;
+      gx[Wk::xi(nc, i, c)] = gxk;
+      gpar[2 * Wk::child(nc, i, c)] += gM;
+      gpar[2 * Wk::child(nc, i, c) + 1] += gW;
     }
     // hand the adjoints of this branch's inputs to its parent
     gc_msq = g_a; gc_m = g_mnode; gc_p = g_p0;
-    if (i >= 1 && prog[i - 1].desc < 0) {   // reached by a pop in the forward pass
+    if (i >= 1 && Wk::desc(prog[i - 1], i - 1) < 0) {   // reached by a pop in the forward pass
 #pragma unroll
       for (int q = 0; q < 16; q++) S.st_put(depth, q, Gc[q]);
       S.st_put(depth, 16, gc_msq); S.st_put(depth, 17, gc_m); S.st_put(depth, 18, gc_p);
       depth++;
     }
+  };
+  if constexpr (SID >= 0) {
+#pragma unroll
+    for (int i = Wk::n_nodes(cd) - 1; i >= 0; i--) rev_branch(i);
+  } else {
+    for (int i = N - 1; i >= 0; i--) rev_branch(i);
   }
   }
   return true;

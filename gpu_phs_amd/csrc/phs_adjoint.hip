@@ -6,18 +6,24 @@
 // fixed order, so results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "phs_adjoint.h"
 #include "phs_kernels.h"
+#include "phs_shapes.h"
 
 namespace gphs {
 
 namespace {
 
-// COMPACT: the stash keeps 15 of its 19 words (only the root defers; phs_adjoint.h).
+// COMPACT: the stash keeps 15 of its 19 words (only the root defers; phs_adjoint.h).  Word w
+// of a lane is at base[w * 64] (one wavefront per workgroup: consecutive lanes, consecutive
+// words).
 template <bool COMPACT>
 struct LdsStore {
+  static constexpr int stride = 64;
   double* base;
-  int stride, off_in, off_st;
+  int off_in, off_st;
   __device__ double& T(int node, int k) { return base[(node * 6 + k) * stride]; }
   __device__ double& in(int node, int k) { return base[(off_in + node * 2 + k) * stride]; }
   __device__ __forceinline__ void st_put(int level, int k, double v) {
@@ -40,8 +46,7 @@ struct LdsStore {
 
 struct LdsAcc {
   double* base;
-  int stride;
-  __device__ double& operator[](int i) { return base[i * stride]; }
+  __device__ double& operator[](int i) { return base[i * 64]; }
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -54,22 +59,25 @@ constexpr int kAdjXReg = 8;             // n_x <= 8 <=> n_out <= 4
 constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_out = 4 tree
 
 // XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
-template <int XN, bool COMPACT>
-__global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict__ tiles, int n_units, int n_sub,
-                                                     const SegDesc* __restrict__ segs,
-                                                     const ChannelDesc* __restrict__ chans,
-                                                     const NodeCmd* __restrict__ nodes, int loss_mode,
-                                                     double* __restrict__ gpart, int part_stride, AdjLayout lay) {
+// SID >= 0: every channel has that compile-time tree shape (XN is then its n_x).  The launch
+// covers tiles [t_begin, t_begin + n_units / n_sub); partials go to the tile's global unit.
+// The register budget keeps two waves per SIMD (at most 256 VGPRs).
+template <int XN, bool COMPACT, int SID, int LM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(
+    const TileDesc* __restrict__ tiles, int t_begin, int n_units, int n_sub, const SegDesc* __restrict__ segs,
+    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, double* __restrict__ gpart,
+    int part_stride, AdjLayout lay) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
-  LdsStore<COMPACT> S{lds + lane, 64, lay.off_in, lay.off_st};
+  LdsStore<COMPACT> S{lds + lane, lay.off_in, lay.off_st};
   // the (M, Gamma) accumulators: in registers for the small programs (XN > 0: n_out <= 4, at
   // most kSmallParams), else in LDS after the tape
   double accv[XN > 0 ? kSmallParams : 1];
-  LdsAcc acc_lds{lds + lay.off_acc * 64 + lane, 64};
-  // unit u = sub-tile (u % n_sub) of tile (u / n_sub): events [64 sub, 64 sub + 64) of the tile
+  LdsAcc acc_lds{lds + lay.off_acc * 64 + lane};
+  // unit u = sub-tile (u % n_sub) of tile t_begin + u / n_sub: events [64 sub, 64 sub + 64) of
+  // the tile
   for (int u = (int)blockIdx.x; u < n_units; u += (int)gridDim.x) {
-    const int t = u / n_sub, sub = u - t * n_sub;
+    const int tl = u / n_sub, sub = u - tl * n_sub, t = t_begin + tl;
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
     const ChannelDesc cd = chans[sd.channel];
@@ -93,10 +101,10 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
       if constexpr (XN > 0) {
         double xv[XN];
 #pragma unroll
-        for (int k = 0; k < XN; k++) xv[k] = xg[k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
-        ok = event_adjoint(cd, prog, xv, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
+        for (int k = 0; k < XN; k++) xv[k] = xg[SID >= 0 || k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
+        ok = event_adjoint<SID, LM>(cd, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
       } else {
-        ok = event_adjoint(cd, prog, xg, loss_mode, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
+        ok = event_adjoint<SID, LM>(cd, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
       }
       if (sd.factor) sd.factor[e] = F;
       if (sd.volume) sd.volume[e] = cd.volume;
@@ -108,13 +116,13 @@ __global__ __launch_bounds__(64) void adjoint_kernel(const TileDesc* __restrict_
       for (int v = 0; v < kSmallParams; v++) {
         if (v < np) {   // uniform
           const double tot = wave_sum(accv[v]);
-          if (lane == 0) gpart[(int64_t)u * part_stride + v] = tot;
+          if (lane == 0) gpart[((int64_t)t * n_sub + sub) * part_stride + v] = tot;
         }
       }
     } else {
       for (int v = 0; v < np; v++) {
         const double tot = wave_sum(acc_lds[v]);
-        if (lane == 0) gpart[(int64_t)u * part_stride + v] = tot;
+        if (lane == 0) gpart[((int64_t)t * n_sub + sub) * part_stride + v] = tot;
       }
     }
   }
@@ -179,21 +187,52 @@ __global__ __launch_bounds__(kReduceThreads) void param_reduce(
   if (tid == 0) done[c] = 0;
 }
 
+using AdjKernel = void (*)(const TileDesc*, int, int, int, const SegDesc*, const ChannelDesc*, const NodeCmd*,
+                           double*, int, AdjLayout);
+
+// The adjoint specialised for a tree shape of up to four final-state particles (n_x <= 8): the
+// randoms in the shape's n_x registers, the accumulators in registers; larger shapes run the
+// generic kernel.
+template <int SID, int LM>
+AdjKernel adj_shape(bool compact) {
+  if constexpr (kShapes[SID].n_out <= 4) {
+    constexpr int xn = kShapes[SID].n_x;
+    return compact ? adjoint_kernel<xn, true, SID, LM> : adjoint_kernel<xn, false, SID, LM>;
+  } else {
+    return nullptr;
+  }
+}
+template <int... S>
+AdjKernel adj_pick_seq(int sid, bool compact, int lm, std::integer_sequence<int, S...>) {
+  AdjKernel k = nullptr;
+  (void)((sid == S ? (k = lm ? adj_shape<S, 1>(compact) : adj_shape<S, 0>(compact), true) : false) || ...);
+  return k;
+}
+template <int LM>
+AdjKernel adj_generic(const DevicePlan& dp, const AdjLayout& lay) {
+  return dp.nx_max > kAdjXReg ? adjoint_kernel<0, false, -1, LM>
+         : lay.compact ? adjoint_kernel<kAdjXReg, true, -1, LM> : adjoint_kernel<kAdjXReg, false, -1, LM>;
+}
+
 }  // namespace
 
-hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
-                          int n_sub, double* granges, unsigned int* done, const int* chan_tiles,
-                          const int* chan_tile_off, int n_chans, double* gparam, int grid, hipStream_t stream) {
+hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const TileRun* runs, int n_runs, int loss_mode,
+                          double* gpart, int part_stride, int n_sub, double* granges, unsigned int* done,
+                          const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam,
+                          hipStream_t stream) {
   const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
   if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
-  const int64_t units = (int64_t)dp.n_tiles * n_sub;
-  if (units > 0x7fffffff) return hipErrorInvalidValue;
-  if (units > 0) {
-    const int g = grid > 0 && grid < units ? grid : (int)units;
-    auto k = dp.nx_max > kAdjXReg ? adjoint_kernel<0, false>
-             : lay.compact ? adjoint_kernel<kAdjXReg, true> : adjoint_kernel<kAdjXReg, false>;
-    hipLaunchKernelGGL(k, dim3(g), dim3(64), shmem, stream, dp.tiles, (int)units, n_sub, dp.segs, dp.chans, dp.nodes,
-                       loss_mode, gpart, part_stride, lay);
+  if ((int64_t)dp.n_tiles * n_sub > 0x7fffffff) return hipErrorInvalidValue;
+  AdjKernel generic = loss_mode ? adj_generic<1>(dp, lay) : adj_generic<0>(dp, lay);
+  for (int r = 0; r < n_runs; r++) {
+    const int units = (runs[r].end - runs[r].begin) * n_sub;
+    if (units <= 0) continue;
+    AdjKernel k = nullptr;
+    if (runs[r].sid >= 0)
+      k = adj_pick_seq(runs[r].sid, lay.compact != 0, loss_mode, std::make_integer_sequence<int, kNumShapes>{});
+    if (!k) k = generic;
+    hipLaunchKernelGGL(k, dim3(units), dim3(64), shmem, stream, dp.tiles, runs[r].begin, units, n_sub, dp.segs,
+                       dp.chans, dp.nodes, gpart, part_stride, lay);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -30,6 +30,7 @@
 #include "phs_device.h"
 #include "phs_kernels.h"
 #include "phs_shapes.h"
+#include "phs_walk.h"
 
 namespace gphs {
 
@@ -120,34 +121,6 @@ extern __shared__ double lds[];
 // by the program's wave-uniform indices; with XN = kXLds it is copied into the lane's column
 // of an LDS table x[k][event] when the event starts.
 constexpr int kXLds = -1;
-
-// The walk's structure: SID < 0 reads it from the program records (generic kernel); SID >= 0
-// takes it from the compile-time shape table (phs_shapes.h), so after the branch loop is
-// unrolled every flag and random index below is a constant.
-template <int SID>
-struct Walk {
-  static constexpr int kS = SID < 0 ? 0 : SID;
-  static __device__ __forceinline__ bool leaf(const NodeCmd& nc, int i, int c) {
-    if constexpr (SID >= 0) return c == 0 ? kShapes[kS].node[i].leaf0 : kShapes[kS].node[i].leaf1;
-    else return nc.k[c].leaf;
-  }
-  static __device__ __forceinline__ int desc(const NodeCmd& nc, int i) {
-    if constexpr (SID >= 0) return kShapes[kS].node[i].desc;
-    else return nc.desc;
-  }
-  static __device__ __forceinline__ int push(const NodeCmd& nc, int i) {
-    if constexpr (SID >= 0) return kShapes[kS].node[i].push;
-    else return nc.push;
-  }
-  static __device__ __forceinline__ int xi(const NodeCmd& nc, int i, int c) {
-    if constexpr (SID >= 0) return c == 0 ? kShapes[kS].node[i].xi0 : kShapes[kS].node[i].xi1;
-    else return nc.k[c].xi;
-  }
-  static __device__ __forceinline__ int xphi(const NodeCmd& nc, int i) {
-    if constexpr (SID >= 0) return kShapes[kS].node[i].xphi;
-    else return nc.x_phi;
-  }
-};
 
 // One event of channel `cd` (program `prog`).  xv / xl: the event's randoms (registers, or the
 // lane's LDS column); pl: this lane's column of the momentum table (word 4 j + mu at
@@ -279,7 +252,7 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
   };
   if constexpr (SID >= 0) {
 #pragma unroll
-    for (int i = 0; i < kShapes[Walk<SID>::kS].n_nodes; ++i) branch(i);
+    for (int i = 0; i < Walk<SID>::n_nodes(cd); ++i) branch(i);
   } else {
     for (int i = 0; i < cd.n_nodes; ++i) branch(i);
   }

@@ -56,10 +56,17 @@ constexpr int kAdjStashWords = 19, kAdjCompactStashWords = 15;
 // per-channel reduction of the parameter partials (through granges, kAdjRanges * part_stride
 // doubles per channel) into gparam[2 * (chans[c].param0 + node) + {0: M, 1: Gamma}].  done:
 // n_chans counters that are 0 before the launch (and are 0 again after it).
+//
+// The tiles go in runs of [begin, end) (together every tile once), each run on the kernel
+// specialised for tree shape `sid` when there is one (phs_shapes.h), else the generic kernel.
+struct TileRun {
+  int sid, begin, end;
+};
 constexpr int kAdjRanges = 16;
-hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, int loss_mode, double* gpart, int part_stride,
-                          int n_sub, double* granges, unsigned int* done, const int* chan_tiles,
-                          const int* chan_tile_off, int n_chans, double* gparam, int grid, hipStream_t stream);
+hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const TileRun* runs, int n_runs, int loss_mode,
+                          double* gpart, int part_stride, int n_sub, double* granges, unsigned int* done,
+                          const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam,
+                          hipStream_t stream);
 
 // Inverse mapping.  Per segment of the plan: the channel that generated its events (its
 // factor and randoms are copied) and the output row of its first event.

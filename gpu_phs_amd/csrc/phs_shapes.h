@@ -29,6 +29,8 @@ struct ShapeNode {
   int8_t desc, push;    // daughter visited next / deferred (-1: none)
   int8_t xi0, xi1;      // msq random of an internal daughter (-1: final-state particle)
   int8_t xphi;          // phi random; cos theta is the next one
+  int8_t parent;        // visit index of the parent branch (-1: the root)
+  int8_t node0, node1;  // visit index of an internal daughter's branch (-1: final-state particle)
 };
 
 struct ShapeProg {
@@ -55,7 +57,8 @@ inline int shape_of(const ChannelDesc& cd, const NodeCmd* prog) {
       const ShapeNode& sn = sh.node[i];
       same = nc.friend_sign == 0 && nc.k[0].leaf == sn.leaf0 && nc.k[1].leaf == sn.leaf1 && nc.desc == sn.desc &&
              nc.push == sn.push && nc.x_phi == sn.xphi && (sn.leaf0 || nc.k[0].xi == sn.xi0) &&
-             (sn.leaf1 || nc.k[1].xi == sn.xi1);
+             (sn.leaf1 || nc.k[1].xi == sn.xi1) && nc.parent == sn.parent &&
+             (sn.leaf0 || nc.k[0].node == sn.node0) && (sn.leaf1 || nc.k[1].node == sn.node1);
     }
     if (same) return s;
   }

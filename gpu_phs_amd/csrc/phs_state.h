@@ -29,9 +29,7 @@ constexpr int kTileClasses = 2;
 
 // A run of consecutive tiles of one tile class whose channels share one tree shape (or -1:
 // channels without a shape).
-struct ShapeRun {
-  int sid, begin, end;
-};
+using ShapeRun = TileRun;
 
 struct PlanObj {
   std::vector<gpu_phs_segment> user;       // segments as given

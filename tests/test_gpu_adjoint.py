@@ -183,3 +183,36 @@ def test_autograd_function_and_dot_product():
     ref_par = _dual_param_grads(proc, x.detach().cpu().numpy(), ch, okn, bw, gp.cpu().numpy(), gf.cpu().numpy(), 0)
     got = np.array([[M.grad[c, b - 1].item(), Gm.grad[c, b - 1].item()] for c, b in bw])
     assert np.allclose(got, ref_par, rtol=1e-9, atol=1e-12 * np.abs(ref_par).max())
+
+
+@pytest.mark.parametrize("n_out", [2, 3, 4])
+@pytest.mark.parametrize("loss_mode", [1, 0])
+def test_shape_adjoint_identical_to_generic(n_out, loss_mode):
+    """Every ordered tree of 2-4 final-state particles: the adjoint specialised for its shape
+    gives bit for bit the generic adjoint's forward outputs and losses, and its gradients to
+    1e-12 (the derivative code lets the compiler fuse products, which it does differently in
+    the unrolled kernels)."""
+    import torch
+    proc = W.process_shapes(n_out)
+    res = []
+    for spec in (True, False):
+        rng, ch, x, ph, plan, bufs, counts = _setup(proc, 12001, 5 + n_out)
+        if not spec:
+            plan.set_specialize(False)
+        g_x = [torch.zeros_like(b.x) for b in bufs]
+        loss = [torch.zeros(b.x.shape[0], dtype=torch.float64, device="cuda") for b in bufs]
+        gp = [torch.from_numpy(rng.standard_normal((b.x.shape[0], proc.n_out, 4))).cuda() for b in bufs]
+        gf = [torch.from_numpy(rng.standard_normal(b.x.shape[0])).cuda() for b in bufs]
+        gparam = plan.adjoint(g_x, g_p=gp if loss_mode == 0 else None, g_f=gf if loss_mode == 0 else None,
+                              loss=loss, loss_mode=loss_mode)
+        torch.cuda.synchronize()
+        cat = lambda ts: torch.cat(ts).cpu().numpy()
+        res.append(dict(gx=cat(g_x), loss=cat(loss), gparam=gparam.cpu().numpy(), p=cat([b.p for b in bufs]),
+                        f=cat([b.factor for b in bufs]), ok=cat([b.ok for b in bufs])))
+    a, b = res
+    assert a["ok"].sum() > 100
+    for k in ("loss", "p", "f", "ok"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    for k in ("gx", "gparam"):
+        scale = np.abs(b[k]).max(axis=-1, keepdims=True) if k == "gx" else np.abs(b[k]).max()
+        assert np.all(np.abs(a[k] - b[k]) <= 1e-12 * scale), (k, np.abs(a[k] - b[k]).max())
