@@ -356,6 +356,11 @@ def main():
                     help="only run the adaptive-grid generation and update on c1 and print the figures")
     ap.add_argument("--beams-only", action="store_true",
                     help="only run c1 with per-event beam energies and print the figures")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="only run the batched forward step of the workload (direct launches, --warmup + "
+                         "--steps of them, with --tuned or the autotuner's tuning) and print its figures; "
+                         "for counter profiles, whose per-step figures are the totals over all its dispatches "
+                         "divided by warmup + steps")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -394,6 +399,21 @@ def main():
         r = {"c1": run_inverse("c1", max(3, args.steps // 2), 1, torch, G, W),
              "c2": run_inverse("c2", max(3, args.steps // 2), 1, torch, G, W, scale=0.1)}
         print(json.dumps(r))
+        return
+    if args.forward_only:
+        wl = W.workload(args.workload)
+        handles, segs = make_inputs(wl, torch, G)
+        plan = G.Plan(segs)
+        if args.tuned:
+            plan.apply_tuning(json.loads(args.tuned))
+        else:
+            plan.autotune(reps=3)
+        torch.cuda.synchronize()
+        wall, ev_s = time_steps(plan.run, args.steps, args.warmup, torch)
+        print(json.dumps({"workload": args.workload, "n_events": wl.n_events, "steps": args.steps,
+                          "warmup": args.warmup, "tuning": plan.tuning(),
+                          "kernel_ms_per_step": ev_s / args.steps * 1e3,
+                          "events_s": wl.n_events * args.steps / wall}))
         return
     fixed = None
     if args.config:
