@@ -1,7 +1,9 @@
 // C ABI of the inverse mapping (include/gpu_phs.h): all-channel factors and randoms of the
 // events of a plan (reference: the do_inverse_mapping branch of gen_phs_from_x_gpu,
 // phs.cu:1491-1554).
+#include <algorithm>
 #include <cstring>
+#include <tuple>
 
 #include "phs_state.h"
 
@@ -27,14 +29,43 @@ int inverse_prepare(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows) {
     is[s].pad = 0;
     is[s].row0 = r0;
   }
+  const bool shared = !cp.has_friends && pl.specialize;
   if (pl.inv_dev && pl.inv_process == proc && pl.inv.per_event_beams == pl.dp.per_event_beams &&
-      is.size() == pl.inv_segs.size() &&
+      pl.inv.shared_roots == (shared ? 1 : 0) && is.size() == pl.inv_segs.size() &&
       std::memcmp(is.data(), pl.inv_segs.data(), is.size() * sizeof(InvSeg)) == 0)
     return ok_status();
+  const int n_ch = (int)cp.channels.size();
+  // channel order: by the root's daughters (final-state particles and sum program of each), so
+  // that channels sharing their root's evaluation are adjacent; channel index within a group
+  auto root_key = [&](int c) {
+    const NodeCmd& r = cp.nodes[cp.channels[c].node0];
+    return std::make_tuple(r.k[0].leaves, r.k[0].sum_len, r.k[0].sum_code, r.k[1].leaves, r.k[1].sum_len,
+                           r.k[1].sum_code);
+  };
+  std::vector<int> order(n_ch);
+  for (int c = 0; c < n_ch; c++) order[c] = c;
+  if (shared) std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return root_key(a) < root_key(b); });
+  std::vector<InvStep> steps(n_ch);
+  for (int k = 0; k < n_ch; k++) {
+    steps[k].channel = order[k];
+    steps[k].new_group = k == 0 || root_key(order[k]) != root_key(order[k - 1]) ? 1 : 0;
+  }
+  // save-bank levels of the subtrees deferred below the root (the root's own deferred daughter
+  // is re-formed from the shared root)
+  int rest_banks = 0;
+  for (const ChannelDesc& c : cp.channels) {
+    int depth = 0;
+    for (int i = 1; i < c.n_nodes; i++) {
+      const NodeCmd& nd = cp.nodes[c.node0 + i];
+      if (nd.push >= 0) rest_banks = std::max(rest_banks, ++depth);
+      if (nd.desc < 0 && depth > 0) depth--;
+    }
+  }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t o_chans = al(is.size() * sizeof(InvSeg));
   const size_t o_nodes = o_chans + al(cp.channels.size() * sizeof(ChannelDesc));
-  const size_t total = o_nodes + al(cp.nodes.size() * sizeof(NodeCmd));
+  const size_t o_steps = o_nodes + al(cp.nodes.size() * sizeof(NodeCmd));
+  const size_t total = o_steps + al(steps.size() * sizeof(InvStep));
   if (total > pl.inv_bytes) {
     if (pl.inv_dev) (void)hipFree(pl.inv_dev);
     pl.inv_dev = nullptr;
@@ -47,13 +78,15 @@ int inverse_prepare(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows) {
   std::memcpy(host.data(), is.data(), is.size() * sizeof(InvSeg));
   std::memcpy(host.data() + o_chans, cp.channels.data(), cp.channels.size() * sizeof(ChannelDesc));
   std::memcpy(host.data() + o_nodes, cp.nodes.data(), cp.nodes.size() * sizeof(NodeCmd));
+  std::memcpy(host.data() + o_steps, steps.data(), steps.size() * sizeof(InvStep));
   hipError_t e = hipMemcpy(pl.inv_dev, host.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_error(e, "hipMemcpy(inverse tables)");
   char* base = (char*)pl.inv_dev;
   pl.inv.isegs = (const InvSeg*)base;
   pl.inv.chans = (const ChannelDesc*)(base + o_chans);
   pl.inv.nodes = (const NodeCmd*)(base + o_nodes);
-  pl.inv.n_ch = (int)cp.channels.size();
+  pl.inv.steps = (const InvStep*)(base + o_steps);
+  pl.inv.n_ch = n_ch;
   pl.inv.n_out = cp.spec.n_out;
   pl.inv.n_x = cp.spec.n_x();
   pl.inv.n_banks = cp.max_banks;
@@ -66,6 +99,11 @@ int inverse_prepare(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows) {
   pl.inv.root_bank = root_bank;
   pl.inv.has_friends = cp.has_friends ? 1 : 0;
   pl.inv.per_event_beams = pl.dp.per_event_beams;
+  pl.inv.shared_roots = shared ? 1 : 0;
+  pl.inv.rest_banks = rest_banks;
+  pl.inv.generic = 0;
+  for (const ChannelDesc& c : cp.channels)
+    if (c.shape < 0 || c.shape >= kInvShapes) pl.inv.generic = 1;
   pl.inv_segs = std::move(is);
   pl.inv_process = proc;
   return ok_status();
@@ -81,7 +119,7 @@ int gphs::plan_inverse(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows, dou
   if (block < 64 || block > 256 || block % 64) return set_error(GPU_PHS_E_ARG, "inverse: block must be 64, 128, 192 or 256");
   int rc = inverse_prepare(pl, seg_row0, n_rows);
   if (rc) return rc;
-  if (inverse_lds_bytes(block, pl.inv.n_out, pl.inv.n_banks, pl.inv.root_bank) > forward_lds_limit())
+  if (inverse_lds_bytes(pl.inv, block) > forward_lds_limit())
     return set_error(GPU_PHS_E_ARG, "inverse: the trees need more LDS than a workgroup of this size may use");
   hipError_t e = launch_inverse(pl.inv, pl.dp.tiles, pl.dp.n_tiles, pl.dp.segs, block, 0, factors, x_out, n_rows, stream);
   if (e != hipSuccess) return hip_error(e, "inverse launch");

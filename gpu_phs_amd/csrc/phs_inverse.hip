@@ -21,8 +21,11 @@
 // four angle values instead of G where only roots defer).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "phs_device.h"
 #include "phs_kernels.h"
+#include "phs_walk.h"
 
 namespace gphs {
 
@@ -152,6 +155,244 @@ __device__ __forceinline__ void rotate_back(double (&G)[16], double ct, double s
     G[8 + j] = -sp * w1 + cp * w2;
     G[12 + j] = st * t1 + ct * w3;
   }
+}
+
+// ---- shared roots --------------------------------------------------------------------------
+//
+// The root of every channel sits in the centre-of-mass frame (G = 1, p_decay = 0, so its boost
+// is the identity and v = the first daughter's momentum, bit for bit), and what it computes from
+// the event -- both daughters' msq, lambda, p_decay and the first daughter's angles -- depends
+// only on which final-state particles each daughter holds and in which association they are
+// added.  Channels whose roots agree in both (the same root split up to the orientation of the
+// subtrees below it) share one evaluation: the host orders the channels so that they follow one
+// another, and the kernel evaluates the root when a new group starts.  Each channel then only
+// applies its own maps to the shared numbers (x and factor of its internal daughters' msq, x
+// and factor of cos theta) and walks the rest of its tree from the root's frame, which is the
+// rotation R_y(-theta) R_z(-phi) of the identity, re-formed from four numbers.  That frame and
+// the group's msq, m and p_decay also stand in for a subtree the root defers, so the save bank
+// holds only subtrees deferred below the root.
+
+struct RootGeo {
+  double msq[2];          // msq of the daughters, from the sums of their final-state momenta
+  double m[2];            // sqrt(msq) of an internal daughter (unused for a final-state one)
+  double lam;             // sqrt(lambda) / s, 0 where lambda <= 0 (phs_cpu.cpp:275-279)
+  double pk;              // p_decay = sqrt(lambda) / (2 sqrt(s))
+  double xphi;            // phi / 2 pi of daughter 0
+  double ct, st, cp, sp;  // cos and sin of its polar and azimuthal angles
+};
+
+__device__ __forceinline__ RootGeo root_geometry(const NodeCmd& nc, const EvBeams& eb,
+                                                 const double* __restrict__ pl, int S, int n_out) {
+  RootGeo r;
+  double q0[4], q1[4];
+  branch_sum(pl, S, n_out, nc.k[0], q0);
+  branch_sum(pl, S, n_out, nc.k[1], q1);
+  r.msq[0] = q0[0] * q0[0] - q0[1] * q0[1] - q0[2] * q0[2] - q0[3] * q0[3];
+  r.msq[1] = q1[0] * q1[0] - q1[1] * q1[1] - q1[2] * q1[2] - q1[3] * q1[3];
+#pragma unroll
+  for (int d = 0; d < 2; d++) r.m[d] = nc.k[d].leaf ? 0.0 : math::sqrt_phys(r.msq[d]);
+  const double s = eb.s;
+  const double lda = (s - r.msq[0] - r.msq[1]) * (s - r.msq[0] - r.msq[1]) - 4 * r.msq[0] * r.msq[1];
+  const bool pos = lda > 0;
+  const double sl = pos ? math::sqrt_phys(lda) : 0.0;
+  r.pk = sl / (2 * eb.sqrts);
+  r.lam = pos ? sl / s : 0.0;
+  const double v0 = q0[1], v1 = q0[2], v2 = q0[3];
+  const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
+  const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
+  double phi = atan2(v1, v0);
+  phi = phi < 0 ? phi + kTwoPi : phi;
+  r.xphi = phi / kTwoPi;
+  r.cp = rho > 0 ? v0 / rho : 1.0;
+  r.sp = rho > 0 ? v1 / rho : 0.0;
+  r.ct = v2 / vv;
+  r.st = rho / vv;
+  return r;
+}
+
+// Momentum sum of the subtree below visit index V of shape SID, in the reference's association
+// (each branch the sum of its two daughters, phs.cu:1262-1275; the sum programs of branch_sum
+// add the same pairs, in an order that differs only by commutation).
+template <int SID, int V>
+__device__ __forceinline__ void subtree_sum(const NodeCmd* __restrict__ prog, const double* __restrict__ pl, int S,
+                                            double (&q)[4]) {
+  constexpr ShapeNode N = kShapes[SID].node[V];
+  double a[4], b[4];
+  if constexpr (N.leaf0) {
+    const double* r = pl + 4 * prog[V].k[0].out * S;
+    a[0] = r[0]; a[1] = r[S]; a[2] = r[2 * S]; a[3] = r[3 * S];
+  } else {
+    subtree_sum<SID, (N.node0 < 0 ? 0 : N.node0)>(prog, pl, S, a);
+  }
+  if constexpr (N.leaf1) {
+    const double* r = pl + 4 * prog[V].k[1].out * S;
+    b[0] = r[0]; b[1] = r[S]; b[2] = r[2 * S]; b[3] = r[3 * S];
+  } else {
+    subtree_sum<SID, (N.node1 < 0 ? 0 : N.node1)>(prog, pl, S, b);
+  }
+  q[0] = a[0] + b[0]; q[1] = a[1] + b[1]; q[2] = a[2] + b[2]; q[3] = a[3] + b[3];
+}
+
+// Daughter c of branch i: a final-state momentum, or its subtree's sum.
+template <int SID, int I, int C>
+__device__ __forceinline__ void daughter_sum(const NodeCmd* __restrict__ prog, const double* __restrict__ pl, int S,
+                                             double (&q)[4]) {
+  constexpr ShapeNode N = kShapes[SID].node[I];
+  if constexpr (C == 0 ? N.leaf0 : N.leaf1) {
+    const double* r = pl + 4 * prog[I].k[C].out * S;
+    q[0] = r[0]; q[1] = r[S]; q[2] = r[2 * S]; q[3] = r[3 * S];
+  } else {
+    constexpr int child = C == 0 ? N.node0 : N.node1;
+    subtree_sum<SID, (child < 0 ? 0 : child)>(prog, pl, S, q);
+  }
+}
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// The inverse of one event into channel `cd` (program `prog`) from its root group's geometry
+// `g`: the randoms go to xo[0..n_x), returns the factor.  SID >= 0: the channel's program has
+// shape SID of the compile-time table and every branch's structure is a constant; SID < 0: it
+// is read from the program.  No branch has a friend.
+template <int SID, bool PEV>
+__device__ __forceinline__ double inverse_channel(const ChannelDesc& cd, const EvBeams& eb,
+                                                  const NodeCmd* __restrict__ prog, const RootGeo& g,
+                                                  const double* __restrict__ pl, int S, double* bank, int B,
+                                                  gdouble* __restrict__ xo) {
+  using W = Walk<SID>;
+  const double s = eb.s;
+  double factor = 1;
+  // the root: this channel's maps of the shared numbers
+  {
+    const NodeCmd& nc = prog[0];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      if (!W::leaf(nc, 0, d)) {
+        double x, f;
+        if constexpr (PEV) x_from_msq(child_at(nc.k[d], eb.sqrts, s, cd.m_tot), g.msq[d], s, x, f);
+        else x_from_msq(nc.k[d], g.msq[d], s, x, f);
+        xo[W::xi(nc, 0, d)] = x;
+        factor *= f;
+      }
+    }
+    factor *= g.lam;
+    xo[W::xphi(nc, 0)] = g.xphi;
+    double xc, fct, m_max, cb0 = nc.b0, cb1 = nc.b1;
+    if constexpr (PEV) node_at(nc, true, eb, cd.m_tot, m_max, cb0, cb1);
+    x_from_ct(nc.ct_kind, g.ct, cb0, cb1, xc, fct);
+    xo[W::xphi(nc, 0) + 1] = xc;
+    factor *= fct;
+  }
+  if (W::n_nodes(cd) == 1) return factor;
+
+  // the root's frame, and the daughter the walk descends into
+  double G[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  rotate_back(G, g.ct, g.st, g.cp, g.sp);
+  const int rdesc = W::desc(prog[0], 0);
+  double cur_msq = g.msq[rdesc], cur_m = g.m[rdesc], cur_p = rdesc == 0 ? g.pk : -g.pk;
+  bool root_pending = W::push(prog[0], 0) >= 0;
+  int depth = 0;   // save-bank entries of subtrees deferred below the root
+
+  auto branch = [&](auto ic) {
+    const int i = ic;
+    const NodeCmd& nc = prog[i];
+    double q0[4], q1[4];
+    if constexpr (SID >= 0) {
+      daughter_sum<(SID < 0 ? 0 : SID), decltype(ic)::value, 0>(prog, pl, S, q0);
+      daughter_sum<(SID < 0 ? 0 : SID), decltype(ic)::value, 1>(prog, pl, S, q1);
+    } else {
+      branch_sum(pl, S, cd.n_out, nc.k[0], q0);
+      branch_sum(pl, S, cd.n_out, nc.k[1], q1);
+    }
+    const double msq0 = q0[0] * q0[0] - q0[1] * q0[1] - q0[2] * q0[2] - q0[3] * q0[3];
+    const double msq1 = q1[0] * q1[0] - q1[1] * q1[1] - q1[2] * q1[2] - q1[3] * q1[3];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      if (!W::leaf(nc, i, d)) {
+        double x, f;
+        if constexpr (PEV) x_from_msq(child_at(nc.k[d], eb.sqrts, s, cd.m_tot), d == 0 ? msq0 : msq1, s, x, f);
+        else x_from_msq(nc.k[d], d == 0 ? msq0 : msq1, s, x, f);
+        xo[W::xi(nc, i, d)] = x;
+        factor *= f;
+      }
+    }
+    const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
+    const bool pos = lda > 0;
+    const double sl = pos ? math::sqrt_phys(lda) : 0.0;
+    const double pk = sl / (2 * cur_m);
+    factor *= pos ? sl / cur_msq : 0.0;
+
+    const double bg = cur_m > 0 ? cur_p / cur_m : 0.0;
+    const double gm = math::sqrt_phys(1 + bg * bg);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double r0 = G[j], r3 = G[12 + j];
+      G[j] = gm * r0 - bg * r3;
+      G[12 + j] = -bg * r0 + gm * r3;
+    }
+    const double v0 = G[4] * q0[0] + G[5] * q0[1] + G[6] * q0[2] + G[7] * q0[3];
+    const double v1 = G[8] * q0[0] + G[9] * q0[1] + G[10] * q0[2] + G[11] * q0[3];
+    const double v2 = G[12] * q0[0] + G[13] * q0[1] + G[14] * q0[2] + G[15] * q0[3];
+    const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
+    const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
+    double phi = atan2(v1, v0);
+    phi = phi < 0 ? phi + kTwoPi : phi;
+    const double cp = rho > 0 ? v0 / rho : 1.0, sp = rho > 0 ? v1 / rho : 0.0;
+    const double ct = v2 / vv, st = rho / vv;
+    xo[W::xphi(nc, i)] = phi / kTwoPi;
+    double xc, fct, m_max, cb0 = nc.b0, cb1 = nc.b1;
+    if constexpr (PEV) node_at(nc, false, eb, cd.m_tot, m_max, cb0, cb1);
+    x_from_ct(nc.ct_kind, ct, cb0, cb1, xc, fct);
+    xo[W::xphi(nc, i) + 1] = xc;
+    factor *= fct;
+    rotate_back(G, ct, st, cp, sp);
+
+    const int push = W::push(nc, i), desc = W::desc(nc, i);
+    if (push >= 0) {
+      const double mp = push == 0 ? msq0 : msq1;
+      double* b = bank + kInvBankWords * depth * B;
+#pragma unroll
+      for (int j = 0; j < 16; j++) b[j * B] = G[j];
+      b[16 * B] = mp;
+      b[17 * B] = math::sqrt_phys(mp);
+      b[18 * B] = push == 0 ? pk : -pk;
+      depth++;
+    }
+    if (desc >= 0) {
+      cur_msq = desc == 0 ? msq0 : msq1;
+      cur_m = math::sqrt_phys(cur_msq);
+      cur_p = desc == 0 ? pk : -pk;
+    } else if (depth > 0) {
+      depth--;
+      const double* b = bank + kInvBankWords * depth * B;
+#pragma unroll
+      for (int j = 0; j < 16; j++) G[j] = b[j * B];
+      cur_msq = b[16 * B];
+      cur_m = b[17 * B];
+      cur_p = b[18 * B];
+    } else if (root_pending) {
+      // the subtree the root deferred: the root's frame again, re-formed bit for bit
+      root_pending = false;
+      const int rp = 1 - rdesc;
+#pragma unroll
+      for (int j = 0; j < 16; j++) G[j] = (j % 5 == 0) ? 1.0 : 0.0;
+      rotate_back(G, g.ct, g.st, g.cp, g.sp);
+      cur_msq = g.msq[rp];
+      cur_m = g.m[rp];
+      cur_p = rp == 0 ? g.pk : -g.pk;
+    }
+  };
+  if constexpr (SID >= 0) {
+    static_for<1, kShapes[SID < 0 ? 0 : SID].n_nodes>(branch);
+  } else {
+    for (int i = 1; i < cd.n_nodes; ++i) branch(i);
+  }
+  return factor;
 }
 
 // Inverse of one event into channel `cd` (program `prog`): the randoms go to xo[0..n_x),
@@ -340,20 +581,112 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
   }
 }
 
+// Shared roots (processes without friends): the channels in the order of `steps`, a root
+// evaluated where a group starts.  Channels of up to four final-state particles run the walk
+// specialised for their shape; the others the walk that reads its structure from the program,
+// which only the GEN instances contain (it needs some thirty more registers than the
+// specialised walks).  WPE: waves per SIMD the register allocation aims at (0: the compiler's
+// choice).
+// LDS per workgroup: the momenta column (4 n_out rows of blockDim.x + 1 doubles), then the
+// save banks of subtrees deferred below the root (19 rows of blockDim.x doubles per level).
+template <bool PEV, bool GEN, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void inverse_kernel_shared(const TileDesc* __restrict__ tiles, int n_tiles,
+                                                             const SegDesc* __restrict__ segs,
+                                                             const InvSeg* __restrict__ isegs,
+                                                             const InvStep* __restrict__ steps,
+                                                             const ChannelDesc* __restrict__ chans,
+                                                             const NodeCmd* __restrict__ nodes, int n_ch, int n_out,
+                                                             int n_x, int64_t n_rows, double* __restrict__ factors,
+                                                             double* __restrict__ x_out) {
+  const int B = (int)blockDim.x;
+  const int S = B + 1;
+  const int lane = (int)threadIdx.x;
+  double* ps = inv_lds;
+  double* bank = ps + 4 * n_out * S + lane;
+  double* pl = ps + lane;
+  for (int t = (int)blockIdx.x; t < n_tiles; t += (int)gridDim.x) {
+    const TileDesc td = tiles[t];
+    const SegDesc sd = segs[td.seg];
+    const InvSeg is = isegs[td.seg];
+    for (int base = 0; base < td.count; base += B) {
+      const int cnt = min(B, td.count - base);
+      const bool valid = lane < cnt;   // lanes past the end stage the last event and store nothing
+      const int64_t e = td.e0 + base + (valid ? lane : cnt - 1);
+      const int64_t row = is.row0 + e;
+      EvBeams eb;
+      if constexpr (PEV) eb = event_beams(chans[0], as_global(sd.e1)[e], as_global(sd.e2)[e]);
+      else eb = fixed_beams(chans[0]);
+      const gdouble* pe = as_global((const double*)sd.p) + e * 4 * n_out;
+      for (int j = 0; j < n_out; j++) {
+        double q0 = pe[4 * j], q1 = pe[4 * j + 1], q2 = pe[4 * j + 2], q3 = pe[4 * j + 3];
+        if (eb.bg_lab != 0) {
+          const double a = eb.g_lab * q0 - eb.bg_lab * q3;
+          const double b = -eb.bg_lab * q0 + eb.g_lab * q3;
+          q0 = a; q3 = b;
+        }
+        pl[(4 * j) * S] = q0; pl[(4 * j + 1) * S] = q1; pl[(4 * j + 2) * S] = q2; pl[(4 * j + 3) * S] = q3;
+      }
+      RootGeo geo;
+      for (int k = 0; k < n_ch; k++) {
+        const InvStep st = steps[k];
+        const int c = st.channel;
+        const ChannelDesc cd = chans[c];
+        const NodeCmd* prog = nodes + cd.node0;
+        if (st.new_group) geo = root_geometry(prog[0], eb, pl, S, n_out);
+        gdouble* xo = as_global(x_out) + ((int64_t)c * n_rows + row) * n_x;
+        if (c == is.own) {
+          if (valid) {
+            const gdouble* xi = as_global(sd.x) + e * n_x;
+            for (int q = 0; q < n_x; q++) xo[q] = xi[q];
+            as_global(factors)[row * n_ch + c] = as_global((const double*)sd.factor)[e];
+          }
+          continue;
+        }
+        if (!valid) continue;
+        double f;
+        switch (cd.shape) {
+          case 0: f = inverse_channel<0, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 1: f = inverse_channel<1, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 2: f = inverse_channel<2, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 3: f = inverse_channel<3, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 4: f = inverse_channel<4, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 5: f = inverse_channel<5, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 6: f = inverse_channel<6, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          case 7: f = inverse_channel<7, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
+          default:
+            if constexpr (GEN) f = inverse_channel<-1, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo);
+            else f = 0;   // not reached: the host runs channels of other programs with GEN
+            break;
+        }
+        as_global(factors)[row * n_ch + c] = f;
+      }
+    }
+  }
+}
+
 }  // namespace
 
-size_t inverse_lds_bytes(int block, int n_out, int n_banks, int root_bank) {
-  const size_t words = root_bank ? kInvRootBankWords : kInvBankWords;
-  return ((size_t)4 * n_out * (block + 1) + words * n_banks * block) * sizeof(double);
+size_t inverse_lds_bytes(const InverseTables& it, int block) {
+  if (it.shared_roots)
+    return ((size_t)4 * it.n_out * (block + 1) + (size_t)kInvBankWords * it.rest_banks * block) * sizeof(double);
+  const size_t words = it.root_bank ? kInvRootBankWords : kInvBankWords;
+  return ((size_t)4 * it.n_out * (block + 1) + words * it.n_banks * block) * sizeof(double);
 }
 
 hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
                           int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream) {
   if (n_tiles <= 0) return hipSuccess;
   if (block < 64 || block > 256 || block % 64) return hipErrorInvalidValue;
-  const size_t shmem = inverse_lds_bytes(block, it.n_out, it.n_banks, it.root_bank);
+  const size_t shmem = inverse_lds_bytes(it, block);
   if (shmem > forward_lds_limit()) return hipErrorInvalidValue;
   const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
+  if (it.shared_roots) {
+    auto k = it.per_event_beams ? (it.generic ? inverse_kernel_shared<true, true, 0> : inverse_kernel_shared<true, false, 3>)
+                                : (it.generic ? inverse_kernel_shared<false, true, 0> : inverse_kernel_shared<false, false, 3>);
+    hipLaunchKernelGGL(k, dim3(g), dim3(block), shmem, stream, tiles, n_tiles, segs, it.isegs, it.steps, it.chans,
+                       it.nodes, it.n_ch, it.n_out, it.n_x, n_rows, factors, x_out);
+    return hipGetLastError();
+  }
   using K = decltype(&inverse_kernel<false, false, false>);
   K k;
   if (it.root_bank)

@@ -75,15 +75,30 @@ struct InvSeg {
   int32_t pad;
   int64_t row0;
 };
+// One channel of the inverse's channel order: channels whose roots share their evaluation
+// (the same daughters' final-state particles, added in the same association) follow one
+// another, and the first of each group has new_group set.
+struct InvStep {
+  int32_t channel;
+  int32_t new_group;
+};
 // Device copies of one process's complete channel table (every channel is a target).
 struct InverseTables {
   const InvSeg* isegs = nullptr;
   const ChannelDesc* chans = nullptr;
   const NodeCmd* nodes = nullptr;
+  const InvStep* steps = nullptr;
   int n_ch = 0, n_out = 0, n_x = 0, n_banks = 0, has_friends = 0, per_event_beams = 0;
   int root_bank = 0;   // every deferred subtree is a daughter of the root and no root has a friend
+  // shared roots (no friends): the channels in the order of `steps`; save-bank levels of the
+  // subtrees deferred below the root
+  int shared_roots = 0, rest_banks = 0;
+  int generic = 0;   // some channel's program has none of the shapes of up to four final-state particles
 };
-size_t inverse_lds_bytes(int block, int n_out, int n_banks, int root_bank);
+// The inverse walks specialised per tree shape cover the shapes 0 .. kInvShapes - 1 of the
+// compile-time table (the trees of two to four final-state particles).
+constexpr int kInvShapes = 8;
+size_t inverse_lds_bytes(const InverseTables& it, int block);
 // factors [n_rows][n_ch] and x_out [n_ch][n_rows][n_x] for the events of tiles [0, n_tiles).
 hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_tiles, const SegDesc* segs,
                           int block, int grid, double* factors, double* x_out, int64_t n_rows, hipStream_t stream);

@@ -70,10 +70,12 @@ def test_per_event_beams_match_oracle(name):
     assert_parity(forward_check(proc, x, ch, out, E1=E1, E2=E2))
 
 
-def test_per_event_beams_inverse():
-    proc = W.process_friends(4)
+@pytest.mark.parametrize("name", ["c1", "c2", "2to6", "friends4"])
+def test_per_event_beams_inverse(name):
+    proc = PROCS[name]()
     rng = np.random.default_rng(43)
-    n = 3001
+    n = 3001 if proc.n_channels < 50 else 1201
+    print("per-event beams inverse", name, flush=True)
     ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
     x = rng.random((n, proc.n_x))
     E1, E2 = _beams(proc, n, rng)

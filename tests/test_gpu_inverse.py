@@ -93,6 +93,30 @@ def test_inverse_matches_oracle(name, block):
     assert_inverse_parity(df, dx, *sensitivity(proc, p_ok, f_ref, x_ref, own, ok))
 
 
+@pytest.mark.parametrize("name", ["c1", "c2", "2to6", "large8", "edge", "beams_c2"])
+def test_shared_roots_identical_to_per_channel(name):
+    """The inverse that evaluates each root once per group of channels sharing it (and walks
+    the trees of up to four final-state particles with kernels specialised per shape) against
+    the kernel that inverts every channel on its own: the same operations on the same numbers,
+    so the same bits."""
+    from test_gpu_beams import _beams, run_beams
+    proc = W.process_c2() if name == "beams_c2" else (W.process_large(8) if name == "large8" else PROCS[name]())
+    rng = np.random.default_rng(29)
+    n = 3001 if proc.n_channels < 50 else 1201
+    ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
+    x = rng.random((n, proc.n_x))
+    print("shared roots vs per channel", name, flush=True)
+    if name.startswith("beams"):
+        out, plan, _, _ = run_beams(proc, x, ch, *_beams(proc, n, rng))
+    else:
+        out, plan, _ = run_gpu(proc, x, ch)
+    f1, x1 = (t.cpu().numpy() for t in plan.inverse())
+    plan.set_specialize(False)
+    f2, x2 = (t.cpu().numpy() for t in plan.inverse())
+    assert np.array_equal(f1, f2, equal_nan=True)
+    assert np.array_equal(x1, x2, equal_nan=True)
+
+
 def test_inverse_unequal_beams():
     proc = W.process_friends(4)
     E1, E2 = 650.0, 1000.0 ** 2 / (4 * 650.0)
