@@ -31,7 +31,6 @@ namespace gphs {
 
 namespace {
 
-constexpr double kTwoPi = 6.28318530717958647693;   // mappings.h:5
 constexpr int kInvBankWords = 19;
 // With RB (every deferred subtree is a daughter of the root, and no root has a friend) a bank
 // entry is the root's cos theta, sin theta, cos phi, sin phi and the subtree's msq, m and
@@ -157,6 +156,43 @@ __device__ __forceinline__ void rotate_back(double (&G)[16], double ct, double s
   }
 }
 
+// One branch's decay from its own msq and mass and its daughters' msq (phs.cu:658-664):
+// p_decay, the factor sqrt(lambda) / msq (0 where lambda <= 0, phs_cpu.cpp:275-279), and the
+// boost bg = p_parent / m.  The quotients are IEEE divisions, as in the reference: they feed
+// the frames of the daughters and so the cosines that the peaked maps take near +-1, where an
+// ulp of cos theta is amplified by up to sqrt(s) / m^2 in the map's factor.
+struct Decay {
+  double pk, lam, bg;
+};
+__device__ __forceinline__ Decay decay(double cur_msq, double cur_m, double cur_p, double msq0, double msq1) {
+  const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
+  const bool pos = lda > 0;
+  const double sl = pos ? math::sqrt_phys(lda) : 0.0;
+  Decay d;
+  d.pk = sl / (2 * cur_m);
+  d.lam = pos ? sl / cur_msq : 0.0;
+  d.bg = cur_m > 0 ? cur_p / cur_m : 0.0;
+  return d;
+}
+
+// Polar and azimuthal angles of the direction v (mappings_gpu.hpp:196-219 inverted): the
+// cosines and sines of both angles (IEEE divisions, as above) and x_phi = atan2(v1, v0) / 2 pi
+// in [0, 1], formed in turns by math::atan2_turns.
+struct Dir {
+  double xphi, ct, st, cp, sp;
+};
+__device__ __forceinline__ Dir direction(double v0, double v1, double v2) {
+  const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
+  const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
+  Dir d;
+  d.xphi = math::atan2_turns(v1, v0);
+  d.cp = rho > 0 ? v0 / rho : 1.0;
+  d.sp = rho > 0 ? v1 / rho : 0.0;
+  d.ct = v2 / vv;
+  d.st = rho / vv;
+  return d;
+}
+
 // ---- shared roots --------------------------------------------------------------------------
 //
 // The root of every channel sits in the centre-of-mass frame (G = 1, p_decay = 0, so its boost
@@ -191,22 +227,13 @@ __device__ __forceinline__ RootGeo root_geometry(const NodeCmd& nc, const EvBeam
   r.msq[1] = q1[0] * q1[0] - q1[1] * q1[1] - q1[2] * q1[2] - q1[3] * q1[3];
 #pragma unroll
   for (int d = 0; d < 2; d++) r.m[d] = nc.k[d].leaf ? 0.0 : math::sqrt_phys(r.msq[d]);
-  const double s = eb.s;
-  const double lda = (s - r.msq[0] - r.msq[1]) * (s - r.msq[0] - r.msq[1]) - 4 * r.msq[0] * r.msq[1];
-  const bool pos = lda > 0;
-  const double sl = pos ? math::sqrt_phys(lda) : 0.0;
-  r.pk = sl / (2 * eb.sqrts);
-  r.lam = pos ? sl / s : 0.0;
-  const double v0 = q0[1], v1 = q0[2], v2 = q0[3];
-  const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
-  const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
-  double phi = atan2(v1, v0);
-  phi = phi < 0 ? phi + kTwoPi : phi;
-  r.xphi = phi / kTwoPi;
-  r.cp = rho > 0 ? v0 / rho : 1.0;
-  r.sp = rho > 0 ? v1 / rho : 0.0;
-  r.ct = v2 / vv;
-  r.st = rho / vv;
+  const Decay dk = decay(eb.s, eb.sqrts, 0.0, r.msq[0], r.msq[1]);
+  r.pk = dk.pk;
+  r.lam = dk.lam;
+  // G = 1 and bg = 0 at the root: v is daughter 0's momentum
+  const Dir dr = direction(q0[1], q0[2], q0[3]);
+  r.xphi = dr.xphi;
+  r.ct = dr.ct; r.st = dr.st; r.cp = dr.cp; r.sp = dr.sp;
   return r;
 }
 
@@ -321,13 +348,10 @@ __device__ __forceinline__ double inverse_channel(const ChannelDesc& cd, const E
         factor *= f;
       }
     }
-    const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
-    const bool pos = lda > 0;
-    const double sl = pos ? math::sqrt_phys(lda) : 0.0;
-    const double pk = sl / (2 * cur_m);
-    factor *= pos ? sl / cur_msq : 0.0;
+    const Decay dk = decay(cur_msq, cur_m, cur_p, msq0, msq1);
+    const double pk = dk.pk, bg = dk.bg;
+    factor *= dk.lam;
 
-    const double bg = cur_m > 0 ? cur_p / cur_m : 0.0;
     const double gm = math::sqrt_phys(1 + bg * bg);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -338,13 +362,9 @@ __device__ __forceinline__ double inverse_channel(const ChannelDesc& cd, const E
     const double v0 = G[4] * q0[0] + G[5] * q0[1] + G[6] * q0[2] + G[7] * q0[3];
     const double v1 = G[8] * q0[0] + G[9] * q0[1] + G[10] * q0[2] + G[11] * q0[3];
     const double v2 = G[12] * q0[0] + G[13] * q0[1] + G[14] * q0[2] + G[15] * q0[3];
-    const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
-    const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
-    double phi = atan2(v1, v0);
-    phi = phi < 0 ? phi + kTwoPi : phi;
-    const double cp = rho > 0 ? v0 / rho : 1.0, sp = rho > 0 ? v1 / rho : 0.0;
-    const double ct = v2 / vv, st = rho / vv;
-    xo[W::xphi(nc, i)] = phi / kTwoPi;
+    const Dir dr = direction(v0, v1, v2);
+    const double ct = dr.ct, st = dr.st, cp = dr.cp, sp = dr.sp;
+    xo[W::xphi(nc, i)] = dr.xphi;
     double xc, fct, m_max, cb0 = nc.b0, cb1 = nc.b1;
     if constexpr (PEV) node_at(nc, false, eb, cd.m_tot, m_max, cb0, cb1);
     x_from_ct(nc.ct_kind, ct, cb0, cb1, xc, fct);
@@ -423,15 +443,12 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvB
         factor *= f;
       }
     }
-    // this branch's decay (phs.cu:658-664), lambda <= 0 gives factor 0 (phs_cpu.cpp:275-279)
-    const double lda = (cur_msq - msq0 - msq1) * (cur_msq - msq0 - msq1) - 4 * msq0 * msq1;
-    const bool pos = lda > 0;
-    const double sl = pos ? math::sqrt_phys(lda) : 0.0;
-    const double pk = sl / (2 * cur_m);
-    factor *= pos ? sl / cur_msq : 0.0;
+    // this branch's decay
+    const Decay dk = decay(cur_msq, cur_m, cur_p, msq0, msq1);
+    const double pk = dk.pk, bg = dk.bg;
+    factor *= dk.lam;
 
     // angles of the first daughter in W = B_z(-bg) G
-    const double bg = cur_m > 0 ? cur_p / cur_m : 0.0;
     const double g = math::sqrt_phys(1 + bg * bg);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -465,13 +482,9 @@ __device__ __forceinline__ double inverse_event(const ChannelDesc& cd, const EvB
         G[12 + j] = a0 * w1 + a1 * w2 + c * w3;
       }
     }
-    const double rho = math::sqrt_phys(v0 * v0 + v1 * v1);
-    const double vv = math::sqrt_phys(v0 * v0 + v1 * v1 + v2 * v2);
-    double phi = atan2(v1, v0);
-    phi = phi < 0 ? phi + kTwoPi : phi;
-    const double cp = rho > 0 ? v0 / rho : 1.0, sp = rho > 0 ? v1 / rho : 0.0;
-    const double ct = v2 / vv, st = rho / vv;
-    xo[nc.x_phi] = phi / kTwoPi;
+    const Dir dr = direction(v0, v1, v2);
+    const double ct = dr.ct, st = dr.st, cp = dr.cp, sp = dr.sp;
+    xo[nc.x_phi] = dr.xphi;
     double xc, fct, m_max, cb0 = nc.b0, cb1 = nc.b1;
     if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max, cb0, cb1);
     x_from_ct(nc.ct_kind, ct, cb0, cb1, xc, fct);

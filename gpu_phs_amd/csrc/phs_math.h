@@ -1,4 +1,5 @@
-// Bounded-range elementary functions for the kernels: sin and cos of 2 pi x, tan, exp, log.
+// Bounded-range elementary functions for the kernels: sin and cos of 2 pi x, tan, exp, log,
+// and the azimuth atan2(y, x) in turns.
 //
 // The maps need sin and cos of 2*pi*x for a random x in [0, 1] (the azimuth, phs_cpu.cpp:301-307),
 // tan of an angle inside (-pi/2, pi/2) (the Breit-Wigner map, phs_cpu.cpp:25-28), and exp and
@@ -6,8 +7,9 @@
 // every double and cost 40-190 VALU instructions each; worse, inside the kernels' branch loop
 // the compiler hoists their polynomial coefficients out of the loop into vector registers, some
 // sixty of them for the whole set, which halves the number of resident waves.  Here every
-// coefficient goes through GPHS_PIN, which materialises it in a scalar register pair right
-// where it is used (two scalar moves, issued beside the vector work) instead.
+// coefficient is materialised in a scalar register pair right where it is used (two scalar
+// moves, issued beside the vector work): GPHS_FMA and GPHS_LIN for the polynomial steps,
+// GPHS_PIN for the other constants.
 //
 // Methods (all with explicit fused multiply-adds, so they do not depend on contraction):
 //   * sin/cos: reduction to |theta| <= pi/4 -- exact for turns (t = 4x, r = t - rint(t));
@@ -19,8 +21,9 @@
 //     e^r to r^13/13! (first dropped term 4.2e-18), then scaling by 2^k;
 //   * log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m - 1)/(m + 1),
 //     |s| <= 0.1716, series to s^21/21 (first dropped term 2.3e-17 relative), plus e ln 2.
+//   * atan2 in turns: see atan2_turns.
 // The tests check them against 300-bit arithmetic: within 2 units in the last place (sin,
-// cos, exp, log) and 3 (tan).  tan(-0) is +0.  The header has no HIP dependency, so the same
+// cos, exp, log) and 3 (tan, atan2 in turns).  tan(-0) is +0.  The header has no HIP dependency, so the same
 // code is compiled for the host in those tests.
 #pragma once
 
@@ -68,6 +71,11 @@ constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
 constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
 constexpr double kLog2e = 0x1.71547652b82fep+0;
 constexpr double kSqrtHalf = 0x1.6a09e667f3bcdp-1;
+// 1 / (2 pi) and atan(k / 4) / (2 pi), k = 1, 2, 3, correctly rounded.
+constexpr double kInv2Pi = 0x1.45f306dc9c883p-3;
+constexpr double kAtanQ1 = 0x1.3f670b6bdc73dp-5;
+constexpr double kAtanQ2 = 0x1.2e4051d9df308p-4;
+constexpr double kAtanQ3 = 0x1.a37f5c4c419efp-4;
 
 // A constant materialised in a scalar register pair at its point of use (see the file comment):
 // two scalar moves inside volatile asm, which the compiler neither hoists out of loops nor
@@ -86,18 +94,44 @@ GPHS_MATH_FN double pin_bits(double after) {
 }
 #define GPHS_PIN(c, after) ::gphs::math::pin_bits<__builtin_bit_cast(uint64_t, (double)(c))>(after)
 
-// a * b + c with c in a scalar register pair: one v_fma_f64.  (Left to itself the compiler
-// picks v_fmac_f64, whose addend must be the destination vector register, and copies the
-// scalar constant into vector registers first: two extra moves per Horner step.)
-GPHS_MATH_FN double fma_sc(double a, double b, double c) {
+// a * b + C and z * C1 + C0 (a product and a sum, rounded twice) for compile-time constants:
+// each in one asm statement that forms the constants in the scalar pair s[96:97] right before
+// the vector instruction that reads them, so that they are neither hoisted into vector
+// registers nor kept live in scalar ones.  One statement per step matters: the compiler pads
+// every inline asm statement with a wait state.  (Left to itself it also picks v_fmac_f64,
+// whose addend must be the destination vector register, and copies a scalar constant into
+// vector registers first.)
+template <uint64_t C>
+GPHS_MATH_FN double fma_k(double a, double b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double r;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  asm volatile("s_mov_b32 s96, %3\n\ts_mov_b32 s97, %4\n\tv_fma_f64 %0, %1, %2, s[96:97]"
+               : "=v"(r)
+               : "v"(a), "v"(b), "i"((int32_t)(uint32_t)C), "i"((int32_t)(uint32_t)(C >> 32))
+               : "s96", "s97");
   return r;
 #else
-  return __builtin_fma(a, b, c);
+  return __builtin_fma(a, b, __builtin_bit_cast(double, C));
 #endif
 }
+template <uint64_t C1, uint64_t C0>
+GPHS_MATH_FN double lin_k(double z) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r;
+  asm volatile("s_mov_b32 s96, %2\n\ts_mov_b32 s97, %3\n\tv_mul_f64 %0, %1, s[96:97]\n\t"
+               "s_mov_b32 s96, %4\n\ts_mov_b32 s97, %5\n\tv_add_f64 %0, %0, s[96:97]"
+               : "=&v"(r)
+               : "v"(z), "i"((int32_t)(uint32_t)C1), "i"((int32_t)(uint32_t)(C1 >> 32)),
+                 "i"((int32_t)(uint32_t)C0), "i"((int32_t)(uint32_t)(C0 >> 32))
+               : "s96", "s97");
+  return r;
+#else
+  return z * __builtin_bit_cast(double, C1) + __builtin_bit_cast(double, C0);
+#endif
+}
+#define GPHS_BITS(c) __builtin_bit_cast(uint64_t, (double)(c))
+#define GPHS_FMA(a, b, c) ::gphs::math::fma_k<GPHS_BITS(c)>((a), (b))
+#define GPHS_LIN(z, c1, c0) ::gphs::math::lin_k<GPHS_BITS(c1), GPHS_BITS(c0)>(z)
 
 // a / b where an ulp or so does not matter: the reciprocal estimate refined by two Newton
 // steps (r <- r + r (1 - b r)), times a; within a few ulps of the quotient for normal b, and
@@ -111,7 +145,7 @@ GPHS_MATH_FN double quot(double a, double b) {
   r = __builtin_fma(r, e, r);
   return a * r;
 #else
-  return a / b;
+  return a * (1.0 / b);   // the reciprocal, then the product: rounded twice, as on the device
 #endif
 }
 
@@ -154,21 +188,21 @@ GPHS_MATH_FN double sqrt_phys(double x) {
 GPHS_MATH_FN void sincos_reduced(double t, double& s, double& c) {
   const double t2 = t * t;
   // the first step as a product and a sum, so that both constants can stay scalar
-  double ps = t2 * GPHS_PIN(kS17, t2) + GPHS_PIN(kS15, t2);
-  ps = fma_sc(ps, t2, GPHS_PIN(kS13, ps));
-  ps = fma_sc(ps, t2, GPHS_PIN(kS11, ps));
-  ps = fma_sc(ps, t2, GPHS_PIN(kS9, ps));
-  ps = fma_sc(ps, t2, GPHS_PIN(kS7, ps));
-  ps = fma_sc(ps, t2, GPHS_PIN(kS5, ps));
-  ps = fma_sc(ps, t2, GPHS_PIN(kS3, ps));
+  double ps = GPHS_LIN(t2, kS17, kS15);
+  ps = GPHS_FMA(ps, t2, kS13);
+  ps = GPHS_FMA(ps, t2, kS11);
+  ps = GPHS_FMA(ps, t2, kS9);
+  ps = GPHS_FMA(ps, t2, kS7);
+  ps = GPHS_FMA(ps, t2, kS5);
+  ps = GPHS_FMA(ps, t2, kS3);
   s = __builtin_fma(t * t2, ps, t);
-  double pc = t2 * GPHS_PIN(kC18, t2) + GPHS_PIN(kC16, t2);
-  pc = fma_sc(pc, t2, GPHS_PIN(kC14, pc));
-  pc = fma_sc(pc, t2, GPHS_PIN(kC12, pc));
-  pc = fma_sc(pc, t2, GPHS_PIN(kC10, pc));
-  pc = fma_sc(pc, t2, GPHS_PIN(kC8, pc));
-  pc = fma_sc(pc, t2, GPHS_PIN(kC6, pc));
-  pc = fma_sc(pc, t2, GPHS_PIN(kC4, pc));
+  double pc = GPHS_LIN(t2, kC18, kC16);
+  pc = GPHS_FMA(pc, t2, kC14);
+  pc = GPHS_FMA(pc, t2, kC12);
+  pc = GPHS_FMA(pc, t2, kC10);
+  pc = GPHS_FMA(pc, t2, kC8);
+  pc = GPHS_FMA(pc, t2, kC6);
+  pc = GPHS_FMA(pc, t2, kC4);
   c = __builtin_fma(t2 * t2, pc, __builtin_fma(t2, -0.5, 1.0));
 }
 
@@ -201,16 +235,16 @@ GPHS_MATH_FN double exp_bounded(double x) {
   const double xc = x > 709.8 ? 709.8 : (x < -745.2 ? -745.2 : x);   // keeps k in int range
   const double k = __builtin_rint(xc * GPHS_PIN(kLog2e, xc));
   const double r = __builtin_fma(-k, GPHS_PIN(kLn2Lo, k), __builtin_fma(-k, GPHS_PIN(kLn2Hi, k), xc));
-  double p = r * GPHS_PIN(1.0 / 6227020800.0, r) + GPHS_PIN(1.0 / 479001600.0, r);
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 39916800.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 3628800.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 362880.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 40320.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 5040.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 720.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 120.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 24.0, p));
-  p = fma_sc(p, r, GPHS_PIN(1.0 / 6.0, p));
+  double p = GPHS_LIN(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);
+  p = GPHS_FMA(p, r, 1.0 / 39916800.0);
+  p = GPHS_FMA(p, r, 1.0 / 3628800.0);
+  p = GPHS_FMA(p, r, 1.0 / 362880.0);
+  p = GPHS_FMA(p, r, 1.0 / 40320.0);
+  p = GPHS_FMA(p, r, 1.0 / 5040.0);
+  p = GPHS_FMA(p, r, 1.0 / 720.0);
+  p = GPHS_FMA(p, r, 1.0 / 120.0);
+  p = GPHS_FMA(p, r, 1.0 / 24.0);
+  p = GPHS_FMA(p, r, 1.0 / 6.0);
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r * r, r) + 1.0;   // 1 + r + r^2 (1/2 + r/6 + ...)
   // 2^k in two steps, so that 2^k itself need not be representable near the ends of the range
@@ -227,20 +261,53 @@ GPHS_MATH_FN double log_bounded(double x) {
   const double f = m - 1.0;             // exact
   const double s = f / (2.0 + f);
   const double z = s * s;
-  double p = z * GPHS_PIN(1.0 / 21.0, z) + GPHS_PIN(1.0 / 19.0, z);
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 17.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 15.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 13.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 11.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 9.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 7.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 5.0, p));
-  p = fma_sc(p, z, GPHS_PIN(1.0 / 3.0, p));
+  double p = GPHS_LIN(z, 1.0 / 21.0, 1.0 / 19.0);
+  p = GPHS_FMA(p, z, 1.0 / 17.0);
+  p = GPHS_FMA(p, z, 1.0 / 15.0);
+  p = GPHS_FMA(p, z, 1.0 / 13.0);
+  p = GPHS_FMA(p, z, 1.0 / 11.0);
+  p = GPHS_FMA(p, z, 1.0 / 9.0);
+  p = GPHS_FMA(p, z, 1.0 / 7.0);
+  p = GPHS_FMA(p, z, 1.0 / 5.0);
+  p = GPHS_FMA(p, z, 1.0 / 3.0);
   const double lm = __builtin_fma(2 * s * z, p, 2 * s);     // 2 atanh(s)
   const double de = (double)e;
   const double y = __builtin_fma(de, GPHS_PIN(kLn2Hi, lm), __builtin_fma(de, GPHS_PIN(kLn2Lo, lm), lm));
   if (!(x > 0)) return x == 0 ? -__builtin_inf() : __builtin_nan("");
   return x == __builtin_inf() ? x : y;
+}
+
+// atan2(y, x) / (2 pi), plus 1 where it is negative: the azimuth of (x, y) in turns, in [0, 1]
+// (the inverse maps' x_phi = phi / 2 pi with phi = atan2(y, x) + 2 pi for phi < 0).  With
+// a = min(|x|, |y|) / max(|x|, |y|) in [0, 1], c = k / 4 for k = rint(4 a) and
+// t = (a - c) / (1 + a c), |t| <= 1/8, atan a = atan c + atan t (the addition theorem); atan c is
+// a table of five turns, atan t the Taylor series t - t^3/3 + ... cut after t^17/17 (first
+// dropped term below 1e-17 of t).  The octant, half-plane and sign then place the angle.
+// atan2(+-0, +-0) gives 0 (x = +0) and 1/2 (x = -0), as atan2 does; NaN in, NaN out.
+GPHS_MATH_FN double atan2_turns(double y, double x) {
+  const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+  const bool swap = ay > ax;
+  const double num = swap ? ax : ay, den = swap ? ay : ax;
+  const double a = den > 0 ? quot(num, den) : (den == den ? 0.0 : den);
+  const double k = __builtin_rint(4.0 * a);
+  const double c = 0.25 * k;
+  const double t = quot(a - c, __builtin_fma(a, c, 1.0));   // a - c is exact (Sterbenz)
+  const double z = t * t;
+  double p = GPHS_LIN(z, 1.0 / 17.0, -1.0 / 15.0);
+  p = GPHS_FMA(p, z, 1.0 / 13.0);
+  p = GPHS_FMA(p, z, -1.0 / 11.0);
+  p = GPHS_FMA(p, z, 1.0 / 9.0);
+  p = GPHS_FMA(p, z, -1.0 / 7.0);
+  p = GPHS_FMA(p, z, 1.0 / 5.0);
+  p = GPHS_FMA(p, z, -1.0 / 3.0);
+  const double tt = t * GPHS_PIN(kInv2Pi, t);            // atan t / 2 pi = tt (1 + z p)
+  // atan(k / 4) / 2 pi for k = 0 .. 4
+  const double tk = k < 2.0 ? (k < 1.0 ? 0.0 : GPHS_PIN(kAtanQ1, k)) :
+                    (k < 3.0 ? GPHS_PIN(kAtanQ2, k) : (k < 4.0 ? GPHS_PIN(kAtanQ3, k) : 0.125));
+  double r = tk + __builtin_fma(tt * z, p, tt);          // in [0, 1/8]
+  r = swap ? 0.25 - r : r;
+  r = __builtin_signbit(x) ? 0.5 - r : r;
+  return y < 0 ? 1.0 - r : r;
 }
 
 }  // namespace math

@@ -26,6 +26,7 @@ def lib(tmp_path_factory):
     L.host_tan_radians.argtypes = [P, ctypes.c_long, P]
     L.host_exp.argtypes = [P, ctypes.c_long, P]
     L.host_log.argtypes = [P, ctypes.c_long, P]
+    L.host_atan2_turns.argtypes = [P, P, ctypes.c_long, P]
     return L
 
 
@@ -127,3 +128,46 @@ def test_log_within_two_ulps(lib):
     ys = np.empty_like(sp)
     lib.host_log(_p(sp), len(sp), _p(ys))
     assert ys[0] == -np.inf and np.isnan(ys[1]) and ys[2] == np.inf and np.isnan(ys[3])
+
+
+def test_atan2_turns_within_three_ulps(lib):
+    """The azimuth in turns, atan2(y, x) / 2 pi (+ 1 where negative), against 300-bit
+    arithmetic, over the whole circle: random directions and lengths, the octant and half-plane
+    edges, the edges of the reduction's intervals (tan of (2k + 1) pi / 16... a = (2k + 1) / 8),
+    and tiny and huge components.  (The host build forms its quotients as a reciprocal and a
+    product, as the device does.)"""
+    mp.mp.prec = 300
+    rng = np.random.default_rng(5)
+    ang = rng.random(3000) * 2 * np.pi
+    r = 10.0 ** (rng.random(3000) * 8 - 4)
+    y, x = r * np.sin(ang), r * np.cos(ang)
+    edges = []
+    for a in [0.0, 1 / 8, 3 / 8, 5 / 8, 7 / 8, 1.0, 0.25, 0.5, 0.75]:
+        for aa in (a, np.nextafter(a, 0), np.nextafter(a, 2)):
+            for sx in (1, -1):
+                for sy in (1, -1):
+                    edges += [(sy * aa, sx * 1.0), (sy * 1.0, sx * aa)]
+    edges += [(1e-300, 1.0), (-1e-300, 1.0), (1e-300, -1.0), (-1e-300, -1.0), (3.0, 1e-310), (2.0, -5e-324),
+              (1e200, 3e199), (-7e-200, 2e-200)]
+    ey, ex = np.array(edges).T
+    y, x = np.concatenate([y, ey]), np.concatenate([x, ex])
+    t = np.empty_like(x)
+    lib.host_atan2_turns(_p(y), _p(x), len(x), _p(t))
+    exact = []
+    for yy, xx in zip(y, x):
+        v = mp.atan2(mp.mpf(yy), mp.mpf(xx)) / (2 * mp.pi)
+        exact.append(v + 1 if v < 0 else v)
+    u = _ulps(t, exact)
+    print("max ulps atan2_turns", u.max(), y[u.argmax()], x[u.argmax()])
+    assert u.max() <= 3.0
+    # signed zeros as atan2 places them, and NaN
+    sp = [(0.0, 1.0, 0.0), (0.0, -1.0, 0.5), (-0.0, -1.0, 0.5), (0.0, 0.0, 0.0), (0.0, -0.0, 0.5),
+          (1.0, 0.0, 0.25), (-1.0, 0.0, 0.75), (1.0, 1.0, 0.125), (-1.0, -1.0, 0.625)]
+    ys, xs, want = (np.array(v) for v in zip(*sp))
+    out = np.empty_like(xs)
+    lib.host_atan2_turns(_p(ys), _p(xs), len(xs), _p(out))
+    assert np.array_equal(out, want)
+    nan = np.array([np.nan, 1.0]), np.array([1.0, np.nan])
+    out = np.empty(2)
+    lib.host_atan2_turns(_p(nan[0]), _p(nan[1]), 2, _p(out))
+    assert np.isnan(out).all()
