@@ -74,6 +74,8 @@ _SIGS = {
     "gpu_phs_plan_get_class_config": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5),
     "gpu_phs_plan_set_schedule": (ctypes.c_int, [ctypes.c_int] * 3),
     "gpu_phs_plan_get_schedule": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 2),
+    "gpu_phs_plan_set_merge_below": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
+    "gpu_phs_plan_get_merge_below": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_set_specialize": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "gpu_phs_grid_create": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_grid_destroy": (ctypes.c_int, [ctypes.c_int]),
@@ -208,6 +210,11 @@ class Plan:
         if config:
             self.set_config(**config)
 
+    def set_merge_below(self, events):
+        """Tree shapes with fewer events than this in a tile class run in the class's generic
+        launch (0: every shape its own specialised launch)."""
+        _check(lib().gpu_phs_plan_set_merge_below(self.handle, int(events)), "plan_set_merge_below")
+
     def set_config(self, **config):
         """Launch configuration: block, ept, grid, order, waves_per_eu, lds; keys left out
         keep their current values (the library's defaults for a new plan)."""
@@ -227,7 +234,7 @@ class Plan:
 
     def tuning(self):
         """The launch shape of both tile classes and the schedule, as the autotuner left them:
-        dict(classes=[{block, ept, grid, waves_per_eu, lds}] * 2, order, serial)."""
+        dict(classes=[{block, ept, grid, waves_per_eu, lds}] * 2, order, serial, merge_below)."""
         classes = []
         for c in range(2):
             v = [ctypes.c_int() for _ in range(5)]
@@ -236,7 +243,9 @@ class Plan:
             classes.append(dict(zip(("block", "ept", "grid", "waves_per_eu", "lds"), (q.value for q in v))))
         o, se = ctypes.c_int(), ctypes.c_int()
         _check(lib().gpu_phs_plan_get_schedule(self.handle, ctypes.byref(o), ctypes.byref(se)), "plan_get_schedule")
-        return dict(classes=classes, order=o.value, serial=se.value)
+        mb = ctypes.c_int64()
+        _check(lib().gpu_phs_plan_get_merge_below(self.handle, ctypes.byref(mb)), "plan_get_merge_below")
+        return dict(classes=classes, order=o.value, serial=se.value, merge_below=mb.value)
 
     def apply_tuning(self, t):
         """Set what ``tuning()`` returned (e.g. from an earlier autotuner run)."""
@@ -244,6 +253,8 @@ class Plan:
             _check(lib().gpu_phs_plan_set_class_config(self.handle, c, lc["block"], lc["ept"], lc["grid"],
                                                        lc["waves_per_eu"], lc["lds"]), "plan_set_class_config")
         _check(lib().gpu_phs_plan_set_schedule(self.handle, t["order"], t["serial"]), "plan_set_schedule")
+        if "merge_below" in t:
+            _check(lib().gpu_phs_plan_set_merge_below(self.handle, t["merge_below"]), "plan_set_merge_below")
 
     def info(self):
         v = [ctypes.c_int() for _ in range(3)]

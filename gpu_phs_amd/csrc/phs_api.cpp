@@ -163,10 +163,17 @@ int plan_build(PlanObj& pl) {
   if (tiles.size() > (size_t)0x7fffffff) return set_error(GPU_PHS_E_ARG, "too many tiles");
   std::vector<int> idx(tiles.size());
   std::iota(idx.begin(), idx.end(), 0);
-  // each class grouped by tree shape (one specialised launch per shape); shapeless channels last
+  // each class grouped by tree shape (one specialised launch per shape); shapeless channels,
+  // and shapes with fewer than merge_below events in the class, last (one generic launch)
+  std::vector<int64_t> shape_events((size_t)kTileClasses * kNumShapes, 0);
+  for (size_t t = 0; t < tiles.size(); t++) {
+    const int sh = pl.chans[pl.segs[tiles[t].seg].channel].shape;
+    if (sh >= 0) shape_events[(size_t)tile_class[t] * kNumShapes + sh] += tiles[t].count;
+  }
   auto shape_key = [&](int t) {
     const int sh = pl.chans[pl.segs[tiles[t].seg].channel].shape;
-    return sh < 0 ? kNumShapes : sh;
+    if (sh < 0 || shape_events[(size_t)tile_class[t] * kNumShapes + sh] < pl.merge_below) return kNumShapes;
+    return sh;
   };
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
     if (tile_class[a] != tile_class[b]) return tile_class[a] < tile_class[b];
@@ -257,7 +264,8 @@ int plan_build(PlanObj& pl) {
     runs.clear();
     if (!pl.specialize || pl.dps[c].per_event_beams) continue;
     for (int t = pl.class_begin[c]; t < pl.class_end[c]; t++) {
-      const int sh = pl.chans[pl.segs[pl.tiles[t].seg].channel].shape;
+      int sh = pl.chans[pl.segs[pl.tiles[t].seg].channel].shape;
+      if (sh >= 0 && shape_events[(size_t)c * kNumShapes + sh] < pl.merge_below) sh = -1;
       if (runs.empty() || runs.back().sid != sh) runs.push_back(ShapeRun{sh, t, t + 1});
       else runs.back().end = t + 1;
     }
@@ -541,6 +549,25 @@ int gpu_phs_plan_set_schedule(int plan, int order, int serial) {
   return ok_status();
 }
 
+int gpu_phs_plan_set_merge_below(int plan, int64_t events) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (events < 0) return set_error(GPU_PHS_E_ARG, "merge_below must not be negative");
+  if (events == pl->merge_below) return ok_status();
+  pl->merge_below = events;
+  drop_graph(*pl);
+  return plan_build(*pl);
+}
+
+int gpu_phs_plan_get_merge_below(int plan, int64_t* events) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (events) *events = pl->merge_below;
+  return ok_status();
+}
+
 int gpu_phs_plan_get_schedule(int plan, int* order, int* serial) {
   std::lock_guard<std::mutex> lk(api_mutex());
   PlanObj* pl = find_plan(plan);
@@ -746,11 +773,25 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
       if (!rc) rc = time_now(-1, &ms);
       if (!rc && ms < 0.98f * best_order_ms) { best_order_ms = ms; best_order = order; best_serial = serial; }
     }
+  // then the size below which a tree shape joins its class's generic launch
+  pl->order = best_order;
+  pl->serial_classes = best_serial;
+  int64_t best_merge = pl->merge_below;
+  float best_merge_ms = 1e30f;
+  if (pl->specialize)
+    for (int64_t mb : {kMergeBelowDefault, (int64_t)0, (int64_t)1 << 16, (int64_t)1 << 20}) {
+      if (rc) break;
+      pl->merge_below = mb;
+      drop_graph(*pl);
+      rc = plan_build(*pl);
+      float ms = 0;
+      if (!rc) rc = time_now(-1, &ms);
+      if (!rc && ms < 0.98f * best_merge_ms) { best_merge_ms = ms; best_merge = mb; }
+    }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;
-  pl->order = best_order;
-  pl->serial_classes = best_serial;
+  pl->merge_below = best_merge;
   drop_graph(*pl);
   rc = plan_build(*pl);
   if (rc) return rc;
@@ -764,7 +805,8 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
                lc.waves_per_eu, lc.lds);
       s += buf;
     }
-    snprintf(buf, sizeof buf, "order=%d serial=%d %.4f ms;", best_order, best_serial, best_order_ms);
+    snprintf(buf, sizeof buf, "order=%d serial=%d %.4f ms; merge_below=%lld %.4f ms;", best_order, best_serial,
+             best_order_ms, (long long)best_merge, best_merge_ms);
     s += buf;
     for (const auto& c : tried) {
       snprintf(buf, sizeof buf, " c%d:%d/%d/%d/%d/%d:%.4f", c.cls, c.lc.block, c.lc.ept, c.lc.grid, c.lc.waves_per_eu,

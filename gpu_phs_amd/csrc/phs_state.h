@@ -14,6 +14,8 @@
 
 namespace gphs {
 
+constexpr int64_t kMergeBelowDefault = (int64_t)1 << 18;
+
 // ---- errors: every entry point records its status for gpu_phs_last_error()
 int set_error(int code, const std::string& msg);
 inline int ok_status() { return set_error(GPU_PHS_OK, ""); }
@@ -36,6 +38,10 @@ struct PlanObj {
   std::vector<const double*> beam_e1, beam_e2;   // per-event beam energies of each segment, or null
   std::vector<double*> seg_msq;            // per-branch msq output of each segment, or null
   int specialize = 1;                      // 1: trees with a shape run the shape-specialised kernels
+  // a shape whose tiles in a class hold fewer events than this runs in the class's one launch
+  // of the generic kernel instead of a launch of its own (a launch of a few tiles costs about
+  // one wave's lifetime however small it is)
+  int64_t merge_below = kMergeBelowDefault;
   std::vector<ShapeRun> shape_runs[2];     // each tile class split by shape (empty: one generic launch)
   std::vector<ChannelDesc> chans;
   std::vector<NodeCmd> nodes;

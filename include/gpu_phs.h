@@ -163,6 +163,13 @@ int gpu_phs_plan_get_class_config(int plan, int cls, int* block, int* ept, int* 
  * other on the caller's stream (serial = 1) or share the GPU (0: the second class on a side
  * stream, joined back by events). */
 int gpu_phs_plan_set_schedule(int plan, int order, int serial);
+/* Tree shapes with fewer than `events` events in a tile class run in that class's one launch of
+ * the generic kernel instead of a launch of the kernel specialised for them (default 2^18; 0:
+ * every shape its own launch).  A launch however small lasts about one wave's lifetime, so
+ * small shapes are cheaper folded into a launch that runs anyway.  No counterpart in the
+ * reference. */
+int gpu_phs_plan_set_merge_below(int plan, int64_t events);
+int gpu_phs_plan_get_merge_below(int plan, int64_t* events);
 int gpu_phs_plan_get_schedule(int plan, int* order, int* serial);
 /* Per-event beam energies of segment `segment` (device arrays [n_events], or both NULL for the
  * process's fixed beams): each event gets sqrt(s) = 2 sqrt(E1 E2), its lab boost and the map

@@ -27,6 +27,7 @@ def _setup(proc, n, seed, lo=0.02):
         segs.append((ph, c, b))
         start += k
     plan = G.Plan(segs)
+    plan.set_merge_below(0)   # every shape on its specialised adjoint, however few its events
     return rng, ch, x, ph, plan, bufs, counts
 
 

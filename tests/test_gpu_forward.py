@@ -52,6 +52,10 @@ def run_gpu(proc, x, channels, E1=None, E2=None, msq=False, specialize=True, **c
     plan = G.Plan(segs, **cfg)
     if not specialize:
         plan.set_specialize(False)
+    # every shape its own specialised launch, however few its events (the default folds shapes
+    # below 2^18 events into the generic launch, which would leave the specialised kernels
+    # untested at these sizes)
+    plan.set_merge_below(0)
     msqs = []
     if msq:
         for i, b in enumerate(bufs):
@@ -137,7 +141,11 @@ def test_plain_batched_graph_identical():
         bufs.append(b)
     plan = G.Plan(segs)
     results = []
-    for mode in ("run", "run_plain", "run_graph", "run_graph"):
+    # every shape its own specialised launch (merge_below 0), shapes below 2^12 events folded
+    # into the class's generic launch, and every tree on the generic kernel
+    for merge, mode in ((0, "run"), (0, "run_plain"), (0, "run_graph"), (0, "run_graph"), (1 << 12, "run"),
+                        (1 << 12, "run_graph"), (1 << 62, "run")):
+        plan.apply_tuning(dict(plan.tuning(), merge_below=merge))
         for b in bufs:
             b.p.fill_(0)
             b.factor.fill_(0)
@@ -175,6 +183,7 @@ def test_ragged_and_empty_segments():
         bufs.append(b)
     for cfg in (dict(block=64, ept=1), dict(block=256, ept=4), dict(block=192, ept=2, grid=3)):
         plan = G.Plan(segs, **cfg)
+        plan.set_merge_below(0)
         plan.run()
         torch.cuda.synchronize()
         for i, (n, b) in enumerate(zip(sizes, bufs)):
