@@ -56,74 +56,102 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 constexpr int kAdjXReg = 8;             // n_x <= 8 <=> n_out <= 4
+constexpr int kAdjMultiShapes = 8;      // shapes 0-7 of the table: the trees of 2-4 final-state particles
 constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_out = 4 tree
 
+// One 64-event unit of the adjoint: sub-tile `sub` of tile t, events [64 sub, 64 sub + 64).
 // XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
-// SID >= 0: every channel has that compile-time tree shape (XN is then its n_x).  The launch
-// covers tiles [t_begin, t_begin + n_units / n_sub); partials go to the tile's global unit.
-// The register budget keeps two waves per SIMD (at most 256 VGPRs).
+// SID >= 0: the tile's channel has that compile-time tree shape (XN is then its n_x).
+// Partials go to the tile's global unit.
 template <int XN, bool COMPACT, int SID, int LM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(
-    const TileDesc* __restrict__ tiles, int t_begin, int n_units, int n_sub, const SegDesc* __restrict__ segs,
-    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, double* __restrict__ gpart,
-    int part_stride, AdjLayout lay) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const TileDesc& td, const SegDesc& sd,
+                                             const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
+                                             double* __restrict__ gpart, int part_stride, const AdjLayout& lay,
+                                             double* lds) {
   const int lane = threadIdx.x;
   LdsStore<COMPACT> S{lds + lane, lay.off_in, lay.off_st};
   // the (M, Gamma) accumulators: in registers for the small programs (XN > 0: n_out <= 4, at
   // most kSmallParams), else in LDS after the tape
   double accv[XN > 0 ? kSmallParams : 1];
   LdsAcc acc_lds{lds + lay.off_acc * 64 + lane};
-  // unit u = sub-tile (u % n_sub) of tile t_begin + u / n_sub: events [64 sub, 64 sub + 64) of
-  // the tile
+  const int np = 2 * cd.n_nodes;
+  if constexpr (XN > 0) {
+#pragma unroll
+    for (int v = 0; v < kSmallParams; v++) accv[v] = 0;
+  } else {
+    for (int v = 0; v < np; v++) acc_lds[v] = 0;
+  }
+  const int end = min(td.count, 64 * (sub + 1));
+  for (int local = 64 * sub + lane; local < end; local += 64) {
+    const int64_t e = td.e0 + local;
+    const double* xg = sd.x + e * cd.n_x;
+    const double* gp = sd.g_p ? sd.g_p + e * (int64_t)cd.n_out * 4 : nullptr;
+    const double gf = sd.g_f ? sd.g_f[e] : 0.0;
+    double* p_out = sd.p ? sd.p + e * (int64_t)cd.n_out * 4 : nullptr;
+    double F = 0, L = 0;
+    bool ok;
+    if constexpr (XN > 0) {
+      double xv[XN];
+#pragma unroll
+      for (int k = 0; k < XN; k++) xv[k] = xg[SID >= 0 || k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
+      ok = event_adjoint<SID, LM>(cd, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
+    } else {
+      ok = event_adjoint<SID, LM>(cd, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
+    }
+    if (sd.factor) sd.factor[e] = F;
+    if (sd.volume) sd.volume[e] = cd.volume;
+    if (sd.ok) sd.ok[e] = ok ? 1 : 0;
+    if (sd.loss) sd.loss[e] = L;
+  }
+  if constexpr (XN > 0) {
+#pragma unroll
+    for (int v = 0; v < kSmallParams; v++) {
+      if (v < np) {   // uniform
+        const double tot = wave_sum(accv[v]);
+        if (lane == 0) gpart[((int64_t)t * n_sub + sub) * part_stride + v] = tot;
+      }
+    }
+  } else {
+    for (int v = 0; v < np; v++) {
+      const double tot = wave_sum(acc_lds[v]);
+      if (lane == 0) gpart[((int64_t)t * n_sub + sub) * part_stride + v] = tot;
+    }
+  }
+}
+
+// The launch covers tiles [t_begin, t_begin + n_units / n_sub): unit u is sub-tile (u % n_sub)
+// of tile t_begin + u / n_sub.  SID >= 0: every tile's channel has that shape; SID = kAdjMulti:
+// every tile's channel has one of the shapes of up to four final-state particles, and each
+// workgroup takes the specialised code of its tile's shape (one launch for several shapes:
+// a launch lasts at least one wave's lifetime, so a launch per shape would add that much per
+// shape).  The register budget keeps two waves per SIMD (at most 256 VGPRs).
+constexpr int kAdjMulti = 1000;
+template <int XN, bool COMPACT, int SID, int LM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(
+    const TileDesc* __restrict__ tiles, int t_begin, int n_units, int n_sub, const SegDesc* __restrict__ segs,
+    const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, double* __restrict__ gpart,
+    int part_stride, AdjLayout lay) {
+  extern __shared__ double lds[];
   for (int u = (int)blockIdx.x; u < n_units; u += (int)gridDim.x) {
     const int tl = u / n_sub, sub = u - tl * n_sub, t = t_begin + tl;
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
     const ChannelDesc cd = chans[sd.channel];
     const NodeCmd* prog = nodes + cd.node0;
-    const int np = 2 * cd.n_nodes;
-    if constexpr (XN > 0) {
-#pragma unroll
-      for (int v = 0; v < kSmallParams; v++) accv[v] = 0;
-    } else {
-      for (int v = 0; v < np; v++) acc_lds[v] = 0;
-    }
-    const int end = min(td.count, 64 * (sub + 1));
-    for (int local = 64 * sub + lane; local < end; local += 64) {
-      const int64_t e = td.e0 + local;
-      const double* xg = sd.x + e * cd.n_x;
-      const double* gp = sd.g_p ? sd.g_p + e * (int64_t)cd.n_out * 4 : nullptr;
-      const double gf = sd.g_f ? sd.g_f[e] : 0.0;
-      double* p_out = sd.p ? sd.p + e * (int64_t)cd.n_out * 4 : nullptr;
-      double F = 0, L = 0;
-      bool ok;
-      if constexpr (XN > 0) {
-        double xv[XN];
-#pragma unroll
-        for (int k = 0; k < XN; k++) xv[k] = xg[SID >= 0 || k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
-        ok = event_adjoint<SID, LM>(cd, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
-      } else {
-        ok = event_adjoint<SID, LM>(cd, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
-      }
-      if (sd.factor) sd.factor[e] = F;
-      if (sd.volume) sd.volume[e] = cd.volume;
-      if (sd.ok) sd.ok[e] = ok ? 1 : 0;
-      if (sd.loss) sd.loss[e] = L;
-    }
-    if constexpr (XN > 0) {
-#pragma unroll
-      for (int v = 0; v < kSmallParams; v++) {
-        if (v < np) {   // uniform
-          const double tot = wave_sum(accv[v]);
-          if (lane == 0) gpart[((int64_t)t * n_sub + sub) * part_stride + v] = tot;
-        }
+    if constexpr (SID == kAdjMulti) {
+      switch (cd.shape) {
+        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        default: break;   // not reached: the host launches only tiles of shapes 0-7 here
       }
     } else {
-      for (int v = 0; v < np; v++) {
-        const double tot = wave_sum(acc_lds[v]);
-        if (lane == 0) gpart[((int64_t)t * n_sub + sub) * part_stride + v] = tot;
-      }
+      adjoint_unit<XN, COMPACT, SID, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds);
     }
   }
 }
@@ -224,17 +252,29 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
   if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
   if ((int64_t)dp.n_tiles * n_sub > 0x7fffffff) return hipErrorInvalidValue;
   AdjKernel generic = loss_mode ? adj_generic<1>(dp, lay) : adj_generic<0>(dp, lay);
-  for (int r = 0; r < n_runs; r++) {
-    const int units = (runs[r].end - runs[r].begin) * n_sub;
-    if (units <= 0) continue;
+  AdjKernel multi = loss_mode ? (lay.compact ? adjoint_kernel<kAdjXReg, true, kAdjMulti, 1>
+                                             : adjoint_kernel<kAdjXReg, false, kAdjMulti, 1>)
+                              : (lay.compact ? adjoint_kernel<kAdjXReg, true, kAdjMulti, 0>
+                                             : adjoint_kernel<kAdjXReg, false, kAdjMulti, 0>);
+  auto small = [](int sid) { return sid >= 0 && sid < kAdjMultiShapes; };
+  for (int r = 0; r < n_runs;) {
+    // consecutive runs of the shapes of up to four final-state particles: one launch
+    int r1 = r + 1, end = runs[r].end;
+    if (small(runs[r].sid))
+      while (r1 < n_runs && small(runs[r1].sid) && runs[r1].begin == end) end = runs[r1++].end;
+    const int units = (end - runs[r].begin) * n_sub;
     AdjKernel k = nullptr;
-    if (runs[r].sid >= 0)
+    if (r1 - r > 1) k = multi;
+    else if (runs[r].sid >= 0)
       k = adj_pick_seq(runs[r].sid, lay.compact != 0, loss_mode, std::make_integer_sequence<int, kNumShapes>{});
     if (!k) k = generic;
-    hipLaunchKernelGGL(k, dim3(units), dim3(64), shmem, stream, dp.tiles, runs[r].begin, units, n_sub, dp.segs,
-                       dp.chans, dp.nodes, gpart, part_stride, lay);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (units > 0) {
+      hipLaunchKernelGGL(k, dim3(units), dim3(64), shmem, stream, dp.tiles, runs[r].begin, units, n_sub, dp.segs,
+                         dp.chans, dp.nodes, gpart, part_stride, lay);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    r = r1;
   }
   if (n_chans > 0) {
     hipLaunchKernelGGL(param_reduce, dim3(n_chans * kSplit), dim3(kReduceThreads), 0, stream, gpart, part_stride,
