@@ -361,6 +361,10 @@ def main():
                          "--steps of them, with --tuned or the autotuner's tuning) and print its figures; "
                          "for counter profiles, whose per-step figures are the totals over all its dispatches "
                          "divided by warmup + steps")
+    ap.add_argument("--sweep-schedule", action="store_true",
+                    help="with --forward-only: time the step for merge_below in {0, 2^14, 2^16, 2^18, 2^20} x "
+                         "classes shared / back to back, 3 interleaved repeats of --steps steps each, on top of "
+                         "--tuned (or the autotuner's tuning), and print the times")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -409,6 +413,18 @@ def main():
         else:
             plan.autotune(reps=3)
         torch.cuda.synchronize()
+        if args.sweep_schedule:
+            base, times = plan.tuning(), {}
+            for _ in range(3):
+                for mb in (0, 1 << 14, 1 << 16, 1 << 18, 1 << 20):
+                    for serial in (0, 1):
+                        plan.apply_tuning(dict(base, merge_below=mb, serial=serial))
+                        torch.cuda.synchronize()
+                        _, ev = time_steps(plan.run, args.steps, args.warmup, torch)
+                        times.setdefault(f"merge_below={mb} serial={serial}", []).append(ev / args.steps * 1e3)
+            print(json.dumps({"workload": args.workload, "base_tuning": base,
+                              "ms_per_step": {k: sorted(v) for k, v in times.items()}}))
+            return
         wall, ev_s = time_steps(plan.run, args.steps, args.warmup, torch)
         print(json.dumps({"workload": args.workload, "n_events": wl.n_events, "steps": args.steps,
                           "warmup": args.warmup, "tuning": plan.tuning(),
