@@ -13,8 +13,7 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   const int n_seg = (int)pl.segs.size(), n_ch = (int)pl.chans.size(), n_t = pl.dp.n_tiles;
   if (pl.dp.has_friends)
     return set_error(GPU_PHS_E_TREE, "adjoint: branches with a friend (beam polar axis) have no adjoint");
-  if (pl.dp.per_event_beams)
-    return set_error(GPU_PHS_E_ARG, "adjoint: segments with per-event beam energies have no adjoint");
+
   int max_nodes = 0, max_banks = 0;
   for (const auto& c : pl.chans) {
     max_nodes = std::max(max_nodes, (int)c.n_nodes);
@@ -143,6 +142,9 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
   for (const auto& c : pl->chans)
     if (c.bg_lab != 0 && loss_mode == 1)
       return set_error(GPU_PHS_E_ARG, "adjoint: the built-in loss assumes symmetric beams");
+  if (pl->dp.per_event_beams && loss_mode == 1)
+    return set_error(GPU_PHS_E_ARG, "adjoint: the built-in loss assumes symmetric beams; with per-event beam energies "
+                                    "give the cotangents (loss_mode 0)");
   int rc = adjoint_prepare(*pl, adj, write_forward);
   if (rc) return rc;
   // the tile table by class and, within a class, by tree shape (the forward launches' runs):

@@ -14,7 +14,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "phs_device.h"
 #include "phs_kernels.h"
+#include "phs_mapconst.h"
 #include "phs_math.h"
 #include "phs_types.h"
 #include "phs_walk.h"
@@ -293,17 +295,32 @@ GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16])
 //   unrolled and every link of the walk is a constant.  LM is the loss mode; with LM = 1 the
 //   beams are symmetric (the entry point checks it), so no lab boost, and the cotangents of
 //   every leaf's energy and longitudinal momentum are zero, which the sweep leaves out.
-template <int SID, int LM, class Store, class GPar, class XR>
-GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR& xr, const double* gp,
-                           double gf_in, double* p_out, double* f_out, double* loss_out, double* gx, GPar& gpar,
-                           Store& S) {
+template <int SID, int LM, bool PEV, class Store, class GPar, class XR>
+GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeCmd* prog, const XR& xr,
+                           const double* gp, double gf_in, double* p_out, double* f_out, double* loss_out, double* gx,
+                           GPar& gpar, Store& S) {
   constexpr int loss_mode = LM;
   using Wk = Walk<SID>;
-  const double s = cd.s;
+  const double s = eb.s;
+  // a map of daughter c of branch i at this event's beams: the plan's constants, or (PEV) the
+  // ones of the event's sqrt(s) with their derivatives with respect to the map's mass and width
+  auto child = [&](const NodeCmd& nc, int i, int c) -> ChildCmd {
+    if constexpr (!PEV) {
+      return nc.k[c];
+    } else {
+      ChildCmd r = child_at(nc.k[c], eb.sqrts, s, cd.m_tot);
+      mc::DualMW a[3], b[3];
+      mc::map_constants_d(prog[Wk::child(nc, i, c)].map_id, nc.k[c].m, nc.k[c].w, nc.k[c].m_min, cd.m_tot, eb.sqrts,
+                          a, b);
+#pragma unroll
+      for (int k = 0; k < 3; k++) { r.da_dm[k] = a[k].dm; r.da_dw[k] = a[k].dw; }
+      return r;
+    }
+  };
   const int N = Wk::n_nodes(cd);
   // ---------------- forward (phs_forward.hip, plus the tape)
   double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  double cur_msq = s, cur_m = cd.sqrts, cur_p = 0, F = 1, S2 = 0;
+  double cur_msq = s, cur_m = eb.sqrts, cur_p = 0, F = 1, S2 = 0;
   bool ok = true;
   int depth = 0;
   auto fwd_branch = [&](const int i) {
@@ -315,11 +332,16 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
-      else { MapEval r = msq_eval(nc.k[c], xr[Wk::xi(nc, i, c)], s); msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq); }
+      else {
+        MapEval r = msq_eval(PEV ? child_at(nc.k[c], eb.sqrts, s, cd.m_tot) : nc.k[c], xr[Wk::xi(nc, i, c)], s);
+        msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq);
+      }
     }
+    double m_max = nc.m_max, cb0 = nc.b0, cb1 = nc.b1;
+    if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max, cb0, cb1);
     const double m = cur_m;
     const double lda = (cur_msq - msq[0] - msq[1]) * (cur_msq - msq[0] - msq[1]) - 4 * msq[0] * msq[1];
-    ok = ok && (lda > 0 && m > mk[0] + mk[1] && m <= nc.m_max);
+    ok = ok && (lda > 0 && m > mk[0] + mk[1] && m <= m_max);
     // quotients to a few ulps (math::quot): the outputs agree with the forward kernel's to
     // ~1e-15 relative, far inside the parity bound
     const double sl = math::sqrt_phys(lda), pk = math::quot(sl, 2 * m);
@@ -327,7 +349,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     const double bg = m > 0 ? math::quot(cur_p, m) : 0.0, g = math::sqrt_phys(1 + bg * bg);
     double sp, cp;
     math::sincos_turns(xr[Wk::xphi(nc, i)], sp, cp);
-    const CtEval ce = ct_eval(nc.ct_kind, xr[Wk::xphi(nc, i) + 1], nc.b0, nc.b1);
+    const CtEval ce = ct_eval(nc.ct_kind, xr[Wk::xphi(nc, i) + 1], cb0, cb1);
     F *= ce.f;
     compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
     S.T(i, 0) = g; S.T(i, 1) = bg; S.T(i, 2) = ce.ct; S.T(i, 3) = ce.st; S.T(i, 4) = cp; S.T(i, 5) = sp;
@@ -338,8 +360,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       const double E = nc.k[c].msq_leaf == 0 ? fabs(pc) : math::sqrt_phys(nc.k[c].msq_leaf + pc * pc);
       double q0 = __builtin_fma(M[3], pc, M[0] * E), q1 = __builtin_fma(M[7], pc, M[4] * E), q2 = __builtin_fma(M[11], pc, M[8] * E),
              q3 = __builtin_fma(M[15], pc, M[12] * E);
-      if (LM == 0 && cd.bg_lab != 0) {
-        const double b0 = cd.g_lab * q0 + cd.bg_lab * q3, b3 = cd.bg_lab * q0 + cd.g_lab * q3;
+      if (LM == 0 && eb.bg_lab != 0) {
+        const double b0 = eb.g_lab * q0 + eb.bg_lab * q3, b3 = eb.bg_lab * q0 + eb.g_lab * q3;
         q0 = b0; q3 = b3;
       }
       S2 += q1 * q1 + q2 * q2;
@@ -385,6 +407,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
   double gc_msq = 0, gc_m = 0, gc_p = 0;
   auto rev_branch = [&](const int i) {
     const NodeCmd& nc = prog[i];
+    double m_max_ev, cb0 = nc.b0, cb1 = nc.b1;
+    if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max_ev, cb0, cb1);
     const bool leaf[2] = {Wk::leaf(nc, i, 0), Wk::leaf(nc, i, 1)};
     const int push = Wk::push(nc, i), desc = Wk::desc(nc, i);
     double G[16] = {0};
@@ -403,7 +427,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     }
     // the branch's local quantities: boost parameters from the tape, maps recomputed
     const double a = S.in(i, 0), p0 = S.in(i, 1);
-    const double m = i == 0 ? cd.sqrts : math::sqrt_phys(a);
+    const double m = i == 0 ? eb.sqrts : math::sqrt_phys(a);
     double P[16];
     parent_boost<SID>(prog, i, S, P);
     MapEval r[2];
@@ -411,7 +435,11 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
 #pragma unroll
     for (int c = 0; c < 2; c++) {
       if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
-      else { xk[c] = xr[Wk::xi(nc, i, c)]; r[c] = msq_eval(nc.k[c], xk[c], s); msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]); }
+      else {
+        xk[c] = xr[Wk::xi(nc, i, c)];
+        r[c] = msq_eval(PEV ? child_at(nc.k[c], eb.sqrts, s, cd.m_tot) : nc.k[c], xk[c], s);
+        msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]);
+      }
     }
     const double d = a - msq[0] - msq[1];
     const double lda = d * d - 4 * msq[0] * msq[1];
@@ -428,7 +456,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       ce.in = true;
       ce.lo = false;
     } else {
-      ce = ct_eval(nc.ct_kind, xct, nc.b0, nc.b1);
+      ce = ct_eval(nc.ct_kind, xct, cb0, cb1);
     }
     double Ma[16];
     if (i == 0) {   // the root's parent boost is the identity: Ma = L1
@@ -457,10 +485,10 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       } else {
         gq[0] = gq[1] = gq[2] = gq[3] = 0;
       }
-      if (LM == 0 && cd.bg_lab != 0) {
+      if (LM == 0 && eb.bg_lab != 0) {
         const double a0 = gq[0], a3 = gq[3];
-        gq[0] = cd.g_lab * a0 + cd.bg_lab * a3;
-        gq[3] = cd.bg_lab * a0 + cd.g_lab * a3;
+        gq[0] = eb.g_lab * a0 + eb.bg_lab * a3;
+        gq[3] = eb.bg_lab * a0 + eb.g_lab * a3;
       }
       double gE = 0, gpc = 0;
 #pragma unroll
@@ -511,9 +539,16 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
     // cos theta map of this branch (its own M through b0, b1)
     const double gfct = ce.f != 0 ? math::quot(W, ce.f) : 0.0;
     double gxc, gb0, gb1;
-    ct_eval_adj(nc.ct_kind, xct, nc.b0, nc.b1, ce, gct, gst, gfct, gxc, gb0, gb1);
+    ct_eval_adj(nc.ct_kind, xct, cb0, cb1, ce, gct, gst, gfct, gxc, gb0, gb1);
     gx[Wk::xphi(nc, i) + 1] = gxc;
-    gpar[2 * i] += gb0 * nc.db_dm[0] + gb1 * nc.db_dm[1];
+    if constexpr (PEV) {   // the cos-theta constants at this event's s: b0 = M^2 / s, b1 = log((b0 + 1) / b0)
+      mc::DualMW a[3], b[3];
+      mc::map_constants_d(nc.map_id, nc.m, 0.0, nc.m_min, cd.m_tot, eb.sqrts, a, b,
+                          nc.friend_sign != 0 ? eb.s_friend : 0.0);
+      gpar[2 * i] += gb0 * b[0].dm + gb1 * b[1].dm;
+    } else {
+      gpar[2 * i] += gb0 * nc.db_dm[0] + gb1 * nc.db_dm[1];
+    }
     // gamma = sqrt(1 + bg^2), bg = p / m
     gbg += math::quot(gg * bg, g);
     double g_a = 0, g_mnode = 0, g_p0 = 0;
@@ -543,7 +578,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const NodeCmd* prog, const XR&
       if (leaf[c]) continue;
       if (mk[c] > 0) g_msq[c] += math::quot(g_m[c], 2 * mk[c]);
       double gxk = 0, gM = 0, gW = 0;
-      msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      msq_eval_adj(child(nc, i, c), xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
       gx[Wk::xi(nc, i, c)] = gxk;
       gpar[2 * Wk::child(nc, i, c)] += gM;
       gpar[2 * Wk::child(nc, i, c) + 1] += gW;

@@ -63,7 +63,7 @@ constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_
 // XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
 // SID >= 0: the tile's channel has that compile-time tree shape (XN is then its n_x).
 // Partials go to the tile's global unit.
-template <int XN, bool COMPACT, int SID, int LM>
+template <int XN, bool COMPACT, int SID, int LM, bool PEV>
 __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const TileDesc& td, const SegDesc& sd,
                                              const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
                                              double* __restrict__ gpart, int part_stride, const AdjLayout& lay,
@@ -90,16 +90,19 @@ __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const Ti
     double* p_out = sd.p ? sd.p + e * (int64_t)cd.n_out * 4 : nullptr;
     double F = 0, L = 0;
     bool ok;
+    EvBeams eb;
+    if constexpr (PEV) eb = event_beams(cd, sd.e1[e], sd.e2[e]);
+    else eb = fixed_beams(cd);
     if constexpr (XN > 0) {
       double xv[XN];
 #pragma unroll
       for (int k = 0; k < XN; k++) xv[k] = xg[SID >= 0 || k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
-      ok = event_adjoint<SID, LM>(cd, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
+      ok = event_adjoint<SID, LM, PEV>(cd, eb, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
     } else {
-      ok = event_adjoint<SID, LM>(cd, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
+      ok = event_adjoint<SID, LM, PEV>(cd, eb, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
     }
     if (sd.factor) sd.factor[e] = F;
-    if (sd.volume) sd.volume[e] = cd.volume;
+    if (sd.volume) sd.volume[e] = eb.volume;
     if (sd.ok) sd.ok[e] = ok ? 1 : 0;
     if (sd.loss) sd.loss[e] = L;
   }
@@ -126,7 +129,7 @@ __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const Ti
 // a launch lasts at least one wave's lifetime, so a launch per shape would add that much per
 // shape).  The register budget keeps two waves per SIMD (at most 256 VGPRs).
 constexpr int kAdjMulti = 1000;
-template <int XN, bool COMPACT, int SID, int LM>
+template <int XN, bool COMPACT, int SID, int LM, bool PEV = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(
     const TileDesc* __restrict__ tiles, int t_begin, int n_units, int n_sub, const SegDesc* __restrict__ segs,
     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, double* __restrict__ gpart,
@@ -140,18 +143,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ad
     const NodeCmd* prog = nodes + cd.node0;
     if constexpr (SID == kAdjMulti) {
       switch (cd.shape) {
-        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
         default: break;   // not reached: the host launches only tiles of shapes 0-7 here
       }
     } else {
-      adjoint_unit<XN, COMPACT, SID, LM>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds);
+      adjoint_unit<XN, COMPACT, SID, LM, PEV>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds);
     }
   }
 }
@@ -238,6 +241,11 @@ AdjKernel adj_pick_seq(int sid, bool compact, int lm, std::integer_sequence<int,
 }
 template <int LM>
 AdjKernel adj_generic(const DevicePlan& dp, const AdjLayout& lay) {
+  if constexpr (LM == 0) {   // per-event beams: the external cotangents only (the built-in loss assumes equal beams)
+    if (dp.per_event_beams)
+      return dp.nx_max > kAdjXReg ? adjoint_kernel<0, false, -1, 0, true>
+             : lay.compact ? adjoint_kernel<kAdjXReg, true, -1, 0, true> : adjoint_kernel<kAdjXReg, false, -1, 0, true>;
+  }
   return dp.nx_max > kAdjXReg ? adjoint_kernel<0, false, -1, LM>
          : lay.compact ? adjoint_kernel<kAdjXReg, true, -1, LM> : adjoint_kernel<kAdjXReg, false, -1, LM>;
 }
@@ -251,6 +259,7 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
   const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
   if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
   if ((int64_t)dp.n_tiles * n_sub > 0x7fffffff) return hipErrorInvalidValue;
+  if (dp.per_event_beams && loss_mode != 0) return hipErrorInvalidValue;
   AdjKernel generic = loss_mode ? adj_generic<1>(dp, lay) : adj_generic<0>(dp, lay);
   AdjKernel multi = loss_mode ? (lay.compact ? adjoint_kernel<kAdjXReg, true, kAdjMulti, 1>
                                              : adjoint_kernel<kAdjXReg, false, kAdjMulti, 1>)
@@ -264,7 +273,8 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
       while (r1 < n_runs && small(runs[r1].sid) && runs[r1].begin == end) end = runs[r1++].end;
     const int units = (end - runs[r].begin) * n_sub;
     AdjKernel k = nullptr;
-    if (r1 - r > 1) k = multi;
+    if (dp.per_event_beams) k = generic;
+    else if (r1 - r > 1) k = multi;
     else if (runs[r].sid >= 0)
       k = adj_pick_seq(runs[r].sid, lay.compact != 0, loss_mode, std::make_integer_sequence<int, kNumShapes>{});
     if (!k) k = generic;

@@ -1,6 +1,7 @@
 // Host work-list compiler (see tree_compiler.h).
 #include "tree_compiler.h"
 
+#include "phs_mapconst.h"
 #include "phs_shapes.h"
 
 #include <algorithm>
@@ -11,6 +12,9 @@
 namespace gphs {
 
 namespace {
+
+using mc::DualMW;
+using mc::map_constants_d;
 
 const double kTwoPi2 = 39.47841760435743447534;   // mappings.h:6
 const double kTwoPi5 = 9792.62991312900650440772; // mappings.h:7
@@ -37,78 +41,6 @@ bool ct_kind(int id, int32_t* kind) {
     case MAP_COLLINEAR: case MAP_TCHANNEL: case MAP_UCHANNEL:
       *kind = CK_PEAKED; return true;
     default: return false;
-  }
-}
-
-// Value and derivatives with respect to (M, Gamma) of a map constant: lets the same code
-// give the constants and their Jacobians, which the adjoint kernel chains through.
-struct DualMW {
-  double v = 0, dm = 0, dw = 0;
-  DualMW() = default;
-  DualMW(double x) : v(x) {}
-  DualMW(double x, double a, double b) : v(x), dm(a), dw(b) {}
-};
-inline DualMW operator+(DualMW a, DualMW b) { return {a.v + b.v, a.dm + b.dm, a.dw + b.dw}; }
-inline DualMW operator-(DualMW a, DualMW b) { return {a.v - b.v, a.dm - b.dm, a.dw - b.dw}; }
-inline DualMW operator-(DualMW a) { return {-a.v, -a.dm, -a.dw}; }
-inline DualMW operator*(DualMW a, DualMW b) { return {a.v * b.v, a.dm * b.v + a.v * b.dm, a.dw * b.v + a.v * b.dw}; }
-inline DualMW operator/(DualMW a, DualMW b) {
-  return {a.v / b.v, (a.dm * b.v - a.v * b.dm) / (b.v * b.v), (a.dw * b.v - a.v * b.dw) / (b.v * b.v)};
-}
-inline DualMW d2chain(DualMW a, double f, double df) { return {f, df * a.dm, df * a.dw}; }
-inline DualMW atan(DualMW a) { return d2chain(a, std::atan(a.v), 1 / (1 + a.v * a.v)); }
-inline DualMW log(DualMW a) { return d2chain(a, std::log(a.v), 1 / a.v); }
-inline DualMW exp(DualMW a) { double e = std::exp(a.v); return d2chain(a, e, e); }
-inline DualMW sqrt(DualMW a) { double r = std::sqrt(a.v); return d2chain(a, r, 0.5 / r); }
-inline DualMW pow2(DualMW a) { return d2chain(a, std::pow(a.v, 2), 2 * a.v); }   // pow(a, 2) of the reference
-inline DualMW dmax(DualMW a, double b) { return a.v < b ? DualMW(b) : a; }           // std::max(a, b)
-inline double val(const DualMW& a) { return a.v; }
-
-// phs_cpu.cpp:452-531 (init_mapping_constants_cpu) for one branch, same expressions and
-// evaluation order; the values are those of the reference, bit for bit, and dm / dw carry
-// their derivatives with respect to the map mass and width.  s_ct, when nonzero, is the s the
-// cos-theta constants are taken at (a branch with a friend: 4 E1 E2, phs.cu:879).
-void map_constants_d(int id, double m_, double w_, double m_min, double m_tot, double sqrts, DualMW a[3], DualMW b[3],
-                     double s_ct = 0) {
-  const DualMW m(m_, 1, 0), w(w_, 0, 1);
-  double m_max = sqrts - m_tot + m_min;
-  double msq_min = m_min * m_min, msq_max = m_max * m_max, s = sqrts * sqrts;
-  const double sb = s_ct != 0 ? s_ct : s;
-  DualMW msq0;
-  a[0] = a[1] = a[2] = DualMW(0);
-  switch (id) {
-    case MAP_NO:
-      a[0] = 0; a[1] = msq_max - msq_min; a[2] = a[1] / s; break;
-    case MAP_SCHANNEL:
-      msq0 = m * m;
-      a[0] = atan((msq_min - msq0) / (m * w));
-      a[1] = atan((msq_max - msq0) / (m * w));
-      a[2] = (a[1] - a[0]) * (m * w) / s; break;
-    case MAP_RADIATION: case MAP_COLLINEAR: case MAP_INFRARED:
-      msq0 = id == MAP_RADIATION ? w * w : m * m;
-      a[0] = msq0;
-      a[1] = log((msq_max - msq_min) / msq0 + 1);
-      a[2] = a[1] / s; break;
-    case MAP_TCHANNEL: case MAP_UCHANNEL:
-      msq0 = m * m;
-      a[0] = msq0;
-      a[1] = 2 * log((msq_max - msq_min) / (2 * msq0) + 1);
-      a[2] = a[1] / s; break;
-    case MAP_STEP_E:
-      a[2] = dmax(2 * m * w / (msq_max - msq_min), 0.01);
-      a[1] = exp(-(m * m - msq_min) / (msq_max - msq_min) / a[2]);
-      a[0] = 1 - a[2] * log((1 + a[1] * exp(1 / a[2])) / (1 + a[1])); break;
-    case MAP_STEP_H:
-      a[2] = (m * m - msq_min) / (msq_max - msq_min);
-      a[1] = dmax(pow2(2 * m * w / (msq_max - msq_min)) / a[2], 0.000001);
-      a[0] = (1 + sqrt(1 + 4 * a[1] / (1 - a[2]))) / 2; break;
-    default: break;
-  }
-  switch (id) {
-    case MAP_TCHANNEL: case MAP_UCHANNEL: case MAP_COLLINEAR:
-      b[0] = m * m / sb; b[1] = log((b[0] + 1) / b[0]); b[2] = 0; break;
-    default:
-      b[0] = b[1] = b[2] = DualMW(0); break;
   }
 }
 

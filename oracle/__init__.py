@@ -174,11 +174,9 @@ def inverse(proc, p, E1=None, E2=None):
     return f, xo
 
 
-def forward_dual(proc, x, channels, x_tan=None, m_tan=None, w_tan=None, K=1):
-    """Dual-number forward: tangents of p and factor along K seeded directions (trees
-    without friends)."""
-    if proc.friends is not None and np.any(proc.friends):
-        raise ValueError("forward_dual: the friend axis has no dual-number path")
+def forward_dual(proc, x, channels, x_tan=None, m_tan=None, w_tan=None, K=1, E1=None, E2=None):
+    """Dual-number forward: tangents of p and factor along K seeded directions (friend trees
+    and per-event beam energies E1, E2 included)."""
     x = np.ascontiguousarray(x, np.float64)
     ch = np.ascontiguousarray(channels, np.int32)
     n = ch.shape[0]
@@ -190,8 +188,12 @@ def forward_dual(proc, x, channels, x_tan=None, m_tan=None, w_tan=None, K=1):
     cvt = (lambda a: None if a is None else np.ascontiguousarray(a, np.float64))
     xt, mt, wt = cvt(x_tan), cvt(m_tan), cvt(w_tan)
     t = _tree_args(proc)
+    fr = None if proc.friends is None else np.ascontiguousarray(proc.friends, np.int32)
+    e1 = None if E1 is None else np.ascontiguousarray(E1, np.float64)
+    e2 = None if E2 is None else np.ascontiguousarray(E2, np.float64)
     rc = lib().oracle_forward_dual(_I(proc.n_in), _I(proc.n_out), _I(proc.n_channels),
-                                   *[_ptr(a) for a in t], _D(proc.sqrts), _L(n), _ptr(ch),
+                                   *[_ptr(a) for a in t[:3]], _ptr(fr), *[_ptr(a) for a in t[3:]],
+                                   _D(proc.sqrts), _ptr(e1), _ptr(e2), _L(n), _ptr(ch),
                                    _I(proc.n_x), _ptr(x), _I(K), _ptr(xt), _ptr(mt), _ptr(wt),
                                    _ptr(p), _ptr(f), _ptr(ok), _ptr(pt), _ptr(ft))
     if rc:

@@ -908,15 +908,19 @@ void oracle_set_perturbation(uint64_t seed) { g_pert_seed = seed; }
 // directions: x_tan [n_events][n_x][K], m_tan / w_tan [n_channels][n_prt_out][K] seed the
 // directions of the randoms and of the map masses / widths.  Outputs p_tan
 // [n_events][n_out][4][K] and factor_tan [n_events][K] (zero for non-ok events), plus the
-// primal values as oracle_forward (fixed sqrts only).
+// primal values as oracle_forward.  friends (as oracle_forward_friends) and per-event beam
+// energies E1, E2 may be NULL; with them the map constants are formed per event from the dual
+// masses and widths, so their tangents follow each event's sqrt(s).
 int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, const int* d2, const int* hc,
-                        const int* map_ids, const double* map_m, const double* map_w, const double* flv,
-                        double sqrts, long n_events, const int* channels, int n_x, const double* x,
-                        int K, const double* x_tan, const double* m_tan, const double* w_tan,
-                        double* p, double* factors, unsigned char* oks, double* p_tan, double* factor_tan) {
+                        const int* friends, const int* map_ids, const double* map_m, const double* map_w,
+                        const double* flv, double sqrts, const double* E1, const double* E2, long n_events,
+                        const int* channels, int n_x, const double* x, int K, const double* x_tan,
+                        const double* m_tan, const double* w_tan, double* p, double* factors, unsigned char* oks,
+                        double* p_tan, double* factor_tan) {
   if (K < 1 || K > KD) return 4;
   Proc P;
   if (!build_proc(P, n_in, n_out, n_channels, d1, d2, hc, map_ids, flv)) return 1;
+  if (friends) set_friends(P, friends);
   MapPar<Dual> M;
   size_t nb = (size_t)n_channels * P.n_prt_out;
   M.m.resize(nb); M.w.resize(nb); M.a.resize(3 * nb); M.b.resize(3 * nb);
@@ -927,19 +931,30 @@ int oracle_forward_dual(int n_in, int n_out, int n_channels, const int* d1, cons
       M.w[i].d[k] = w_tan ? w_tan[i * K + k] : 0.0;
     }
   }
-  for (int c = 0; c < n_channels; c++) init_constants(P, M, c, sqrts);
+  double cur_sqrts = sqrts;
+  for (int c = 0; c < n_channels; c++) init_constants(P, M, c, cur_sqrts);
   Ev<Dual> E;
   std::vector<Dual> xd(n_x), pd(4 * n_out);
   for (long i = 0; i < n_events; i++) {
     int c = channels[i];
     if (c < 0 || c >= n_channels) return 3;
+    double s_ev = sqrts, bg_lab = 0, s_friend = sqrts * sqrts;
+    if (E1 && E2) {   // as oracle_forward_friends
+      s_friend = 4 * E1[i] * E2[i];
+      s_ev = 2 * std::sqrt(E1[i] * E2[i]);
+      bg_lab = (E1[i] - E2[i]) / (2 * std::sqrt(E1[i] * E2[i]));
+      if (s_ev != cur_sqrts) {
+        cur_sqrts = s_ev;
+        for (int cc = 0; cc < n_channels; cc++) init_constants(P, M, cc, cur_sqrts);
+      }
+    }
     for (int k = 0; k < n_x; k++) {
       xd[k] = Dual(x[(size_t)n_x * i + k]);
       for (int t = 0; t < K; t++) xd[k].d[t] = x_tan ? x_tan[((size_t)n_x * i + k) * K + t] : 0.0;
     }
     for (auto& q : pd) q = Dual(0.0);
     Dual f, v;
-    bool ok = one_event<Dual>(P, M, c, sqrts, 0.0, sqrts * sqrts, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr, nullptr);
+    bool ok = one_event<Dual>(P, M, c, s_ev, bg_lab, s_friend, xd.data(), E, &f, &v, pd.data(), nullptr, nullptr, nullptr);
     oks[i] = ok ? 1 : 0;
     factors[i] = ok ? f.v : 0;
     for (int j = 0; j < 4 * n_out; j++) {

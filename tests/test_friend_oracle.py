@@ -173,8 +173,18 @@ def test_friend_sign_from_the_friends_array():
     assert not np.allclose(minus["p"][ok], plain["p"][ok])
 
 
-def test_forward_dual_rejects_friends():
+def test_forward_dual_follows_the_friend_axis():
+    """The dual-number forward takes the friend axis: its primal values are the friend
+    forward's, bit for bit, and differ from those of the same trees without friends."""
     proc = W.process_friends(4)
-    x, ch = _events(proc, 4, 1)
-    with pytest.raises(ValueError):
-        oracle.forward_dual(proc, x, ch)
+    x, ch = _events(proc, 400, 1)
+    d = oracle.forward_dual(proc, x, ch)
+    f = oracle.forward(proc, x, ch)
+    ok = f["ok"]
+    assert np.array_equal(d["ok"], ok)
+    assert np.array_equal(d["p"][ok], f["p"][ok]) and np.array_equal(d["factor"][ok], f["factor"][ok])
+    q = proc.subset(range(proc.n_channels))
+    q.friends = None
+    plain = oracle.forward_dual(q, x, ch)
+    both = ok & plain["ok"]
+    assert not np.allclose(plain["p"][both], d["p"][both])

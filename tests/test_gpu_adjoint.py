@@ -31,15 +31,15 @@ def _setup(proc, n, seed, lo=0.02):
     return rng, ch, x, ph, plan, bufs, counts
 
 
-def _dual_grads(proc, x, ch, gp=None, gf=None, loss_mode=1):
+def _dual_grads(proc, x, ch, gp=None, gf=None, loss_mode=1, E1=None, E2=None):
     """d loss / d x from the oracle's dual numbers (K = n_x tangents)."""
     n = len(ch)
     K = proc.n_x
     xt = np.zeros((n, K, K))
     for k in range(K):
         xt[:, k, k] = 1
-    d = oracle.forward_dual(proc, x, ch, xt, None, None, K)
-    ref = oracle.forward(proc, x, ch)
+    d = oracle.forward_dual(proc, x, ch, xt, None, None, K, E1=E1, E2=E2)
+    ref = oracle.forward(proc, x, ch, E1=E1, E2=E2)
     s = proc.sqrts ** 2
     if loss_mode == 1:
         S2 = (ref["p"][..., 1] ** 2 + ref["p"][..., 2] ** 2).sum(1)
@@ -51,11 +51,11 @@ def _dual_grads(proc, x, ch, gp=None, gf=None, loss_mode=1):
     return g, ref
 
 
-def _dual_param_grads(proc, x, ch, ok, slots, gp=None, gf=None, loss_mode=1):
+def _dual_param_grads(proc, x, ch, ok, slots, gp=None, gf=None, loss_mode=1, E1=None, E2=None):
     """d loss / d (M, Gamma) of every (channel, bincode) slot from dual numbers."""
     s = proc.sqrts ** 2
     out = np.zeros((len(slots), 2))
-    ref = oracle.forward(proc, x, ch)
+    ref = oracle.forward(proc, x, ch, E1=E1, E2=E2)
     for i, (c, bc) in enumerate(slots):
         sel = (ch == c) & ok
         if not sel.any():
@@ -64,7 +64,8 @@ def _dual_param_grads(proc, x, ch, ok, slots, gp=None, gf=None, loss_mode=1):
             mt = np.zeros((proc.n_channels, proc.n_prt_out, 1))
             wt = np.zeros_like(mt)
             (mt if w == 0 else wt)[c, bc - 1, 0] = 1
-            dd = oracle.forward_dual(proc, x[sel], ch[sel], None, mt, wt, 1)
+            dd = oracle.forward_dual(proc, x[sel], ch[sel], None, mt, wt, 1,
+                                     E1=None if E1 is None else E1[sel], E2=None if E2 is None else E2[sel])
             if loss_mode == 1:
                 p = ref["p"][sel]
                 S2 = (p[..., 1] ** 2 + p[..., 2] ** 2).sum(1)
@@ -120,6 +121,51 @@ def test_adjoint_matches_dual_numbers(name, loss_mode):
     gpar = gparam.cpu().numpy().reshape(-1, 2)[: len(slots)]
     sc = np.abs(gpar_ref).max() + 1e-300
     print(name, loss_mode, "param max rel err", np.abs(gpar - gpar_ref).max() / sc)
+    assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "2to6"])
+def test_adjoint_per_event_beams_matches_dual_numbers(name):
+    """Per-event beam energies: each event's sqrt(s), lab boost and map constants (and their
+    derivatives with respect to M and Gamma) at its own beams, against the oracle's dual numbers
+    with the same beams."""
+    import torch
+    proc = {"c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6}[name]()
+    n = 3000 if name != "c2" else 6000
+    rng, ch, x, ph, plan, bufs, counts = _setup(proc, n, 23)
+    h2 = proc.sqrts / 2
+    E1, E2 = h2 * rng.uniform(0.8, 1.3, n), h2 * rng.uniform(0.8, 1.3, n)
+    e1t, e2t = torch.from_numpy(E1).cuda(), torch.from_numpy(E2).cuda()
+    e1s, e2s = torch.split(e1t, list(counts)), torch.split(e2t, list(counts))
+    for i in range(len(bufs)):
+        plan.set_event_beams(i, e1s[i], e2s[i])
+    gp = rng.standard_normal((n, proc.n_out, 4))
+    gf = rng.standard_normal(n)
+    gpt, gft = torch.from_numpy(gp).cuda(), torch.from_numpy(gf).cuda()
+    kw = dict(g_p=list(torch.split(gpt, list(counts))), g_f=list(torch.split(gft, list(counts))))
+    gx = [torch.empty_like(b.x) for b in bufs]
+    print("per-event beams adjoint", name, flush=True)
+    gparam = plan.adjoint(gx, loss_mode=0, **kw)
+    torch.cuda.synchronize()
+    g_gpu = torch.cat(gx).cpu().numpy()
+    g_ref, ref = _dual_grads(proc, x, ch, gp, gf, 0, E1=E1, E2=E2)
+    ok = ref["ok"]
+    okg = torch.cat([b.ok for b in bufs]).cpu().numpy().astype(bool)
+    assert np.array_equal(okg, ok) and ok.sum() > 200
+    scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(g_gpu - g_ref) / scale
+    print(name, "max rel err", err[ok].max())
+    assert err[ok].max() < 1e-9
+    assert (g_gpu[~ok] == 0).all()
+    # the forward outputs the adjoint kernel wrote: each event at its own beams
+    p_gpu = torch.cat([b.p for b in bufs]).cpu().numpy()
+    assert np.allclose(p_gpu[ok], ref["p"][ok], rtol=0, atol=1e-10 * proc.sqrts)
+    pr, pch, pbc = plan.param_map()
+    slots = list(zip(pch.tolist(), pbc.tolist()))
+    gpar_ref = _dual_param_grads(proc, x, ch, ok, slots, gp, gf, 0, E1=E1, E2=E2)
+    gpar = gparam.cpu().numpy().reshape(-1, 2)[: len(slots)]
+    sc = np.abs(gpar_ref).max() + 1e-300
+    print(name, "param max rel err", np.abs(gpar - gpar_ref).max() / sc)
     assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
 
 

@@ -90,7 +90,9 @@ def test_per_event_beams_inverse(name):
     assert_inverse_parity(df, dx, *sensitivity(proc, p_ok, f_ref, x_ref, ch[order], ok, E1=e1, E2=e2))
 
 
-def test_per_event_beams_have_no_adjoint():
+def test_per_event_beams_built_in_loss_is_rejected():
+    """The built-in loss of the adjoint (loss_mode 1) assumes equal beams; with per-event beam
+    energies the adjoint takes the caller's cotangents (loss_mode 0, test_gpu_adjoint.py)."""
     import torch
     import gpu_phs_amd as G
     proc = W.process_c1()
@@ -100,4 +102,4 @@ def test_per_event_beams_have_no_adjoint():
     E1, E2 = _beams(proc, n, rng)
     _, plan, bufs, _ = run_beams(proc, rng.random((n, proc.n_x)), ch, E1, E2)
     with pytest.raises(G.GpuPhsError, match="per-event beam"):
-        plan.adjoint([torch.zeros_like(b.x) for b in bufs])
+        plan.adjoint([torch.zeros_like(b.x) for b in bufs], loss_mode=1)

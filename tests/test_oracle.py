@@ -204,46 +204,69 @@ def test_ct_map_endpoints_and_jacobian(map_id):
         assert np.isclose(fc, d, rtol=1e-6), (map_id, x)
 
 
-def test_dual_numbers_match_finite_differences():
-    """The forward-mode reference of the adjoint: d p / d x and d factor / d(x, M, Gamma)."""
-    proc = W.process_c1()
-    rng = np.random.default_rng(4)
-    n = 200
-    ch = rng.integers(0, 3, n).astype(np.int32)
+def _dual_against_differences(proc, n, seed, beams=False):
+    """The oracle's dual numbers against central differences of the oracle itself: d p / d x,
+    d factor / d x and d (p, factor) / d (M, Gamma) of every mapped branch (Breit-Wigner masses
+    and widths, and the masses of the t-channel maps, which enter through the cos-theta
+    constants b0 = M^2 / s), with fixed or per-event beam energies."""
+    rng = np.random.default_rng(seed)
+    ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
     x = 0.05 + 0.9 * rng.random((n, proc.n_x))
+    E1 = E2 = None
+    if beams:
+        h2 = proc.sqrts / 2
+        E1, E2 = h2 * rng.uniform(0.8, 1.3, n), h2 * rng.uniform(0.8, 1.3, n)
     K = proc.n_x + 2
     xt = np.zeros((n, proc.n_x, K))
     for k in range(proc.n_x):
         xt[:, k, k] = 1
     mt = np.zeros((proc.n_channels, proc.n_prt_out, K))
     wt = np.zeros_like(mt)
-    bw = proc.map_ids == P.MAP_SCHANNEL
-    mt[bw, proc.n_x] = 1
-    wt[bw, proc.n_x + 1] = 1
-    d = oracle.forward_dual(proc, x, ch, xt, mt, wt, K)
+    mapped = (proc.map_ids == P.MAP_SCHANNEL) | (proc.map_ids == P.MAP_TCHANNEL)
+    mt[mapped, proc.n_x] = 1
+    wt[proc.map_ids == P.MAP_SCHANNEL, proc.n_x + 1] = 1
+    d = oracle.forward_dual(proc, x, ch, xt, mt, wt, K, E1=E1, E2=E2)
     ok = d["ok"]
+    assert ok.sum() > n // 4
+    fwd = lambda q, xx: oracle.forward(q, xx, ch, E1=E1, E2=E2)
+    base = fwd(proc, x)
+    assert np.array_equal(base["ok"], ok)
+    assert np.array_equal(base["p"][ok], d["p"][ok]) and np.array_equal(base["factor"][ok], d["factor"][ok])
     h = 1e-6
     for k in range(proc.n_x):
         xp, xm = x.copy(), x.copy()
         xp[:, k] += h
         xm[:, k] -= h
-        rp, rm = oracle.forward(proc, xp, ch), oracle.forward(proc, xm, ch)
+        rp, rm = fwd(proc, xp), fwd(proc, xm)
         good = ok & rp["ok"] & rm["ok"]
         fd = (rp["p"] - rm["p"]) / (2 * h)
         sc = np.abs(fd[good]).max() + 1
         assert np.allclose(d["p_tan"][good][..., k], fd[good], atol=1e-5 * sc, rtol=1e-5), k
         fdf = (rp["factor"] - rm["factor"]) / (2 * h)
         assert np.allclose(d["factor_tan"][good, k], fdf[good], rtol=1e-5, atol=1e-9), k
-    for j, arr in ((proc.n_x, "map_masses"), (proc.n_x + 1, "map_widths")):
-        pp, pm = proc.subset(range(3)), proc.subset(range(3))
-        getattr(pp, arr)[bw] += h
-        getattr(pm, arr)[bw] -= h
-        rp, rm = oracle.forward(pp, x, ch), oracle.forward(pm, x, ch)
+    for j, arr, sel in ((proc.n_x, "map_masses", mapped), (proc.n_x + 1, "map_widths", proc.map_ids == P.MAP_SCHANNEL)):
+        pp, pm = proc.subset(range(proc.n_channels)), proc.subset(range(proc.n_channels))
+        getattr(pp, arr)[sel] += h
+        getattr(pm, arr)[sel] -= h
+        rp, rm = fwd(pp, x), fwd(pm, x)
         good = ok & rp["ok"] & rm["ok"]
         fdf = (rp["factor"] - rm["factor"]) / (2 * h)
         assert np.allclose(d["factor_tan"][good, j], fdf[good], rtol=1e-5, atol=1e-9), arr
         fd = (rp["p"] - rm["p"]) / (2 * h)
         assert np.allclose(d["p_tan"][good][..., j], fd[good], rtol=1e-5, atol=1e-5), arr
+
+
+def test_dual_numbers_match_finite_differences():
+    """The forward-mode reference of the adjoint: d p / d x and d factor / d(x, M, Gamma)."""
+    _dual_against_differences(W.process_c1(), 200, 4)
+
+
+@pytest.mark.parametrize("name,beams", [("c1", True), ("friends4", False), ("friends4", True)])
+def test_dual_numbers_friends_and_event_beams(name, beams):
+    """The dual numbers through the friend axis and with per-event beam energies (each event's
+    map constants formed from the dual masses at its own sqrt(s))."""
+    proc = W.process_c1() if name == "c1" else W.process_friends(4)
+    _dual_against_differences(proc, 200, 9, beams=beams)
 
 
 def test_per_event_beams_and_lab_boost():
