@@ -32,7 +32,7 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   int max_nx = 0;
   for (const auto& c : pl.chans) max_nx = std::max(max_nx, (int)c.n_x);
   lay.words = lay.off_acc + (max_nx <= 8 ? 0 : 2 * max_nodes);
-  if ((size_t)lay.words * 64 * sizeof(double) > 65536)
+  if ((size_t)lay.words * 64 * sizeof(double) > forward_lds_limit())
     return set_error(GPU_PHS_E_TREE, "adjoint: trees too large for the LDS tape");
   std::vector<SegDesc> segs = pl.segs;
   for (int s = 0; s < n_seg; s++) {

@@ -257,7 +257,7 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
                           const int* chan_tiles, const int* chan_tile_off, int n_chans, double* gparam,
                           hipStream_t stream) {
   const size_t shmem = (size_t)lay.words * 64 * sizeof(double);
-  if (shmem > 65536 || n_sub < 1) return hipErrorInvalidValue;
+  if (shmem > forward_lds_limit() || n_sub < 1) return hipErrorInvalidValue;
   if ((int64_t)dp.n_tiles * n_sub > 0x7fffffff) return hipErrorInvalidValue;
   if (dp.per_event_beams && loss_mode != 0) return hipErrorInvalidValue;
   AdjKernel generic = loss_mode ? adj_generic<1>(dp, lay) : adj_generic<0>(dp, lay);

@@ -169,6 +169,48 @@ def test_adjoint_per_event_beams_matches_dual_numbers(name):
     assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
 
 
+@pytest.mark.parametrize("n_out,n", [(8, 60000), (10, 600000)])
+def test_adjoint_large_trees(n_out, n):
+    """Trees of 8 and 10 final-state particles (save banks up to three levels deep, randoms and
+    (M, Gamma) accumulators in LDS; the tape of a 2->10 tree takes 75 KB of a workgroup's LDS)
+    against the oracle's dual numbers: d loss / d x over the first (up to 24) randoms, and
+    d loss / d (M, Gamma)."""
+    import torch
+    proc = W.process_large(n_out)
+    rng, ch, x, ph, plan, bufs, counts = _setup(proc, n, 31)
+    gp = rng.standard_normal((n, proc.n_out, 4))
+    gf = rng.standard_normal(n)
+    gpt, gft = torch.from_numpy(gp).cuda(), torch.from_numpy(gf).cuda()
+    kw = dict(g_p=list(torch.split(gpt, list(counts))), g_f=list(torch.split(gft, list(counts))))
+    gx = [torch.empty_like(b.x) for b in bufs]
+    print("adjoint 2 ->", n_out, flush=True)
+    gparam = plan.adjoint(gx, loss_mode=0, **kw)
+    torch.cuda.synchronize()
+    g_gpu = torch.cat(gx).cpu().numpy()
+    ref = oracle.forward(proc, x, ch)
+    ok = ref["ok"]
+    assert ok.sum() >= 100
+    sel = np.nonzero(ok)[0]
+    K = min(proc.n_x, 24)
+    xt = np.zeros((sel.size, proc.n_x, K))
+    for k in range(K):
+        xt[:, k, k] = 1
+    d = oracle.forward_dual(proc, x[sel], ch[sel], xt, None, None, K)
+    g_ref = (gp[sel][..., None] * d["p_tan"]).sum((1, 2)) + gf[sel][:, None] * d["factor_tan"]
+    scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(g_gpu[sel][:, :K] - g_ref) / scale
+    print(n_out, "max rel err", err.max())
+    assert err.max() < 1e-9
+    assert (g_gpu[~ok] == 0).all()
+    pr, pch, pbc = plan.param_map()
+    slots = list(zip(pch.tolist(), pbc.tolist()))
+    gpar_ref = _dual_param_grads(proc, x, ch, ok, slots, gp, gf, 0)
+    gpar = gparam.cpu().numpy().reshape(-1, 2)[: len(slots)]
+    sc = np.abs(gpar_ref).max() + 1e-300
+    print(n_out, "param max rel err", np.abs(gpar - gpar_ref).max() / sc)
+    assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
+
+
 def test_adjoint_central_differences_c4():
     """The C4 gradient on the c1 trees against central differences (h = 1e-6) of the loss."""
     import torch
