@@ -11,8 +11,6 @@ namespace {
 // table carrying the gradient buffers, the per-tile partials and the tiles of each channel.
 int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_forward) {
   const int n_seg = (int)pl.segs.size(), n_ch = (int)pl.chans.size(), n_t = pl.dp.n_tiles;
-  if (pl.dp.has_friends)
-    return set_error(GPU_PHS_E_TREE, "adjoint: branches with a friend (beam polar axis) have no adjoint");
 
   int max_nodes = 0, max_banks = 0;
   for (const auto& c : pl.chans) {
@@ -24,14 +22,15 @@ int adjoint_prepare(PlanObj& pl, const gpu_phs_adjoint_segment* adj, int write_f
   // eight one-wavefront workgroups of the 2->4 trees within a CU's LDS
   int max_nx_all = 0;
   for (const auto& c : pl.chans) max_nx_all = std::max(max_nx_all, (int)c.n_x);
-  lay.compact = pl.dp.root_banks_only && max_nx_all <= 8 ? 1 : 0;
+  lay.compact = pl.dp.root_banks_only && max_nx_all <= 8 && !pl.dp.has_friends ? 1 : 0;
   lay.off_in = 6 * max_nodes;
   lay.off_st = lay.off_in + 2 * max_nodes;
   lay.off_acc = lay.off_st + (lay.compact ? kAdjCompactStashWords : kAdjStashWords) * max_banks;
-  // programs of up to 4 final-state particles keep their accumulators in registers
+  // programs of up to 4 final-state particles keep their accumulators in registers (not those
+  // of plans with friends, which run the generic kernel)
   int max_nx = 0;
   for (const auto& c : pl.chans) max_nx = std::max(max_nx, (int)c.n_x);
-  lay.words = lay.off_acc + (max_nx <= 8 ? 0 : 2 * max_nodes);
+  lay.words = lay.off_acc + (max_nx <= 8 && !pl.dp.has_friends ? 0 : 2 * max_nodes);
   if ((size_t)lay.words * 64 * sizeof(double) > forward_lds_limit())
     return set_error(GPU_PHS_E_TREE, "adjoint: trees too large for the LDS tape");
   std::vector<SegDesc> segs = pl.segs;

@@ -257,14 +257,27 @@ GPHS_HD constexpr int compact_stash_slot(int k) {
 // P = product of the L1 of the ancestors of branch j (root first): the boost its own L1 is
 // composed onto.  The ancestors are found by walking the parent links (a few scalar loads, or
 // constants in a kernel specialised for the tree's shape).
-template <int SID, class Store>
+//   FR: some branches take their polar axis from a beam; their L1 depends on the product of
+//   their own ancestors' boosts, which the walk from the root has at hand (identity first).
+template <int SID, bool FR, class Store>
 GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16]) {
   using W = Walk<SID>;
   int d = 0;
   for (int a = W::parent(prog, j); a >= 0; a = W::parent(prog, a)) d++;
-  if (d == 0) {
+  if (d == 0 || FR) {
 #pragma unroll
     for (int q = 0; q < 16; q++) P[q] = (q % 5 == 0) ? 1.0 : 0.0;
+  }
+  if (d == 0) return;
+  if constexpr (FR) {
+    for (int level = 0; level < d; level++) {
+      int a = W::parent(prog, j);
+      for (int u = 0; u < d - 1 - level; u++) a = W::parent(prog, a);
+      if (prog[a].friend_sign != 0)
+        compose_friend(P, S.T(a, 0), S.T(a, 1), S.T(a, 2), S.T(a, 3), S.T(a, 4), S.T(a, 5), (double)prog[a].friend_sign);
+      else
+        compose_l1(P, S.T(a, 0), S.T(a, 1), S.T(a, 2), S.T(a, 3), S.T(a, 4), S.T(a, 5));
+    }
     return;
   }
   // the root's L1 itself (the identity times L1: the same values), then the others
@@ -284,6 +297,91 @@ GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16])
   }
 }
 
+// Adjoint of Ma = P L1 for a branch with a friend (L1 = B_z(bg) (1 (+) Q), Q = R A,
+// A = R_z(phi) R_y(theta), R the rotation to the beam's direction n / |n|, n formed from rows 0
+// and 3 of P; phs_device.h friend_axis, friend_q).  G: the adjoint of Ma.  Writes the adjoint
+// of P into Gc (both terms: G L1^T and the one through n; zero without a parent) and the
+// adjoints of the branch's gamma, bg, cos/sin theta and cos/sin phi.
+GPHS_HD void friend_adj(const double (&P)[16], const double (&G)[16], bool has_parent, double g, double bg, double ct,
+                        double st, double cp, double sp, double f, const FriendAxis& ax, const double (&R)[9],
+                        const double (&Q)[9], const double (&L)[16], double (&Gc)[16], double& gg, double& gbg,
+                        double& gct, double& gst, double& gcp, double& gsp) {
+#pragma clang fp contract(fast)
+  double gL[16];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      gL[4 * k + j] = P[k] * G[j] + P[4 + k] * G[4 + j] + P[8 + k] * G[8 + j] + P[12 + k] * G[12 + j];
+  // L1 = [[g, bg q20, bg q21, bg q22], [0, q00 ..], [0, q10 ..], [bg, g q20, g q21, g q22]]
+  double gQ[9];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    gQ[j] = gL[5 + j];
+    gQ[3 + j] = gL[9 + j];
+    gQ[6 + j] = bg * gL[1 + j] + g * gL[13 + j];
+  }
+  gg = gL[0] + Q[6] * gL[13] + Q[7] * gL[14] + Q[8] * gL[15];
+  gbg = gL[12] + Q[6] * gL[1] + Q[7] * gL[2] + Q[8] * gL[3];
+  // Q = R A: gA = R^T gQ, gR = gQ A^T; A = [[cp ct, -sp, cp st], [sp ct, cp, sp st], [-st, 0, ct]]
+  const double A[9] = {cp * ct, -sp, cp * st, sp * ct, cp, sp * st, -st, 0.0, ct};
+  double gA[9], gR[9];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      gA[3 * k + j] = R[k] * gQ[j] + R[3 + k] * gQ[3 + j] + R[6 + k] * gQ[6 + j];
+      gR[3 * k + j] = gQ[3 * k] * A[3 * j] + gQ[3 * k + 1] * A[3 * j + 1] + gQ[3 * k + 2] * A[3 * j + 2];
+    }
+  gct = gA[0] * cp + gA[3] * sp + gA[8];
+  gst = gA[2] * cp + gA[5] * sp - gA[6];
+  gcp = gA[0] * ct + gA[4] + gA[2] * st;
+  gsp = -gA[1] + gA[3] * ct + gA[5] * st;
+  // hand G L1^T to the parent, then the term through the beam's direction
+  if (!has_parent) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) Gc[q] = 0;
+  } else {
+#pragma unroll
+    for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        Gc[4 * ii + k] = G[4 * ii] * L[4 * k] + G[4 * ii + 1] * L[4 * k + 1] + G[4 * ii + 2] * L[4 * k + 2] +
+                         G[4 * ii + 3] * L[4 * k + 3];
+  }
+  // R: r00 = c + h kx^2, r01 = r10 = h kx ky, r11 = c + h ky^2, r02 = -r20 = a0, r12 = -r21 = a1,
+  // r22 = c; h = 1 - c, (kx, ky) = (-a1, a0) / sa, sa = |(a0, a1)| (for sa = 0: (1, 0))
+  const double a0 = ax.a0, a1 = ax.a1, c = ax.c;
+  const double sa = math::sqrt_phys(a0 * a0 + a1 * a1);
+  const double kx = sa > 0 ? -a1 / sa : 1.0, ky = sa > 0 ? a0 / sa : 0.0, h = 1 - c;
+  const double g01 = gR[1] + gR[3];
+  const double gh = gR[0] * kx * kx + g01 * kx * ky + gR[4] * ky * ky;
+  const double gkx = 2 * h * kx * gR[0] + h * ky * g01;
+  const double gky = 2 * h * ky * gR[4] + h * kx * g01;
+  const double gc = gR[0] + gR[4] + gR[8] - gh;
+  double ga0 = gR[2] - gR[6], ga1 = gR[5] - gR[7];
+  if (sa > 0) {
+    ga1 -= gkx / sa;
+    ga0 += gky / sa;
+    const double gsa = (gkx * a1 - gky * a0) / (sa * sa);
+    ga0 += gsa * a0 / sa;
+    ga1 += gsa * a1 / sa;
+  }
+  // (a0, a1, c) = n / |n|
+  const double dot = ga0 * a0 + ga1 * a1 + gc * c;
+  const double gn0 = (ga0 - a0 * dot) / ax.nn, gn1 = (ga1 - a1 * dot) / ax.nn, gn2 = (gc - c * dot) / ax.nn;
+  // n2 = -bg u0 + g u3; u0 = P00 - f P30, n0 = -P01 + f P31, n1 = -P02 + f P32, u3 = -P03 + f P33
+  gbg -= ax.u0 * gn2;
+  gg += ax.u3 * gn2;
+  if (has_parent) {
+    const double gu0 = -bg * gn2, gu3 = g * gn2;
+    Gc[0] += gu0;  Gc[12] -= f * gu0;
+    Gc[1] -= gn0;  Gc[13] += f * gn0;
+    Gc[2] -= gn1;  Gc[14] += f * gn1;
+    Gc[3] -= gu3;  Gc[15] += f * gu3;
+  }
+}
+
 // Forward pass of one event with tape recording, then the reverse sweep.
 //   xr: the event's n_x randoms (anything indexable); loss_mode 0: cotangents gp [n_out*4] (may be null) and gf
 //   are given; loss_mode 1: the loss is factor * sum_j (p_jx^2 + p_jy^2) / s and its
@@ -295,7 +393,10 @@ GPHS_HD void parent_boost(const NodeCmd* prog, int j, Store& S, double (&P)[16])
 //   unrolled and every link of the walk is a constant.  LM is the loss mode; with LM = 1 the
 //   beams are symmetric (the entry point checks it), so no lab boost, and the cotangents of
 //   every leaf's energy and longitudinal momentum are zero, which the sweep leaves out.
-template <int SID, int LM, bool PEV, class Store, class GPar, class XR>
+//   FR: branches with a friend (nc.friend_sign != 0) take their polar axis from a beam
+//   (phs_device.h compose_friend); their L1 depends on their parent boost, whose adjoint then
+//   has a second term, through the beam's direction.
+template <int SID, int LM, bool PEV, bool FR, class Store, class GPar, class XR>
 GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeCmd* prog, const XR& xr,
                            const double* gp, double gf_in, double* p_out, double* f_out, double* loss_out, double* gx,
                            GPar& gpar, Store& S) {
@@ -351,7 +452,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
     math::sincos_turns(xr[Wk::xphi(nc, i)], sp, cp);
     const CtEval ce = ct_eval(nc.ct_kind, xr[Wk::xphi(nc, i) + 1], cb0, cb1);
     F *= ce.f;
-    compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
+    if (FR && nc.friend_sign != 0) compose_friend(M, g, bg, ce.ct, ce.st, cp, sp, (double)nc.friend_sign);
+    else compose_l1(M, g, bg, ce.ct, ce.st, cp, sp);
     S.T(i, 0) = g; S.T(i, 1) = bg; S.T(i, 2) = ce.ct; S.T(i, 3) = ce.st; S.T(i, 4) = cp; S.T(i, 5) = sp;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
@@ -380,7 +482,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       cur_msq = msq[desc]; cur_m = mk[desc]; cur_p = desc == 0 ? pk : -pk;
     } else if (depth > 0 && i + 1 < N) {   // the next branch in visit order is the deferred one
       depth--;
-      parent_boost<SID>(prog, i + 1, S, M);
+      parent_boost<SID, FR>(prog, i + 1, S, M);
       cur_msq = S.in(i + 1, 0); cur_p = S.in(i + 1, 1); cur_m = math::sqrt_phys(cur_msq);
     }
   };
@@ -429,7 +531,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
     const double a = S.in(i, 0), p0 = S.in(i, 1);
     const double m = i == 0 ? eb.sqrts : math::sqrt_phys(a);
     double P[16];
-    parent_boost<SID>(prog, i, S, P);
+    parent_boost<SID, FR>(prog, i, S, P);
     MapEval r[2];
     double msq[2], f[2], mk[2], xk[2];
 #pragma unroll
@@ -459,7 +561,24 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       ce = ct_eval(nc.ct_kind, xct, cb0, cb1);
     }
     double Ma[16];
-    if (i == 0) {   // the root's parent boost is the identity: Ma = L1
+    const bool fr = FR && nc.friend_sign != 0;
+    // a friend branch: its full L1 = B_z(bg) (1 (+) Q), Q = R(axis) R_z(phi) R_y(theta)
+    FriendAxis ax{};
+    double fR[9], fQ[9], fL[16];
+    if (fr) {
+      ax = friend_axis(P, g, bg, (double)nc.friend_sign);
+      friend_q(ax, ce.ct, ce.st, cp, sp, fR, fQ);
+      fL[0] = g;  fL[1] = bg * fQ[6]; fL[2] = bg * fQ[7]; fL[3] = bg * fQ[8];
+      fL[4] = 0;  fL[5] = fQ[0];      fL[6] = fQ[1];      fL[7] = fQ[2];
+      fL[8] = 0;  fL[9] = fQ[3];      fL[10] = fQ[4];     fL[11] = fQ[5];
+      fL[12] = bg; fL[13] = g * fQ[6]; fL[14] = g * fQ[7]; fL[15] = g * fQ[8];
+#pragma unroll
+      for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          Ma[4 * ii + k] = P[4 * ii] * fL[k] + P[4 * ii + 1] * fL[4 + k] + P[4 * ii + 2] * fL[8 + k] +
+                           P[4 * ii + 3] * fL[12 + k];
+    } else if (i == 0) {   // the root's parent boost is the identity: Ma = L1
       Ma[0] = g;   Ma[1] = -bg * ce.st; Ma[2] = 0;   Ma[3] = bg * ce.ct;
       Ma[4] = 0;   Ma[5] = ce.ct * cp;  Ma[6] = -sp; Ma[7] = ce.st * cp;
       Ma[8] = 0;   Ma[9] = ce.ct * sp;  Ma[10] = cp; Ma[11] = ce.st * sp;
@@ -502,38 +621,44 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       if (E > 0) gpc += math::quot(gE * pc, E);
       g_pk += c == 0 ? gpc : -gpc;
     }
-    // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros of L1 only)
-    const double l01 = -bg * ce.st, l03 = bg * ce.ct, l11 = ce.ct * cp, l12 = -sp, l13 = ce.st * cp;
-    const double l21 = ce.ct * sp, l22 = cp, l23 = ce.st * sp, l31 = -g * ce.st, l33 = g * ce.ct;
-    // gP goes straight to Gc, the adjoint handed to the parent (Gc was consumed above)
-    if (i > 0) {   // the root has no parent to hand gP to
-#pragma unroll
-      for (int ii = 0; ii < 4; ii++) {
-        const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
-        Gc[4 * ii + 0] = __builtin_fma(G3, l03, __builtin_fma(G1, l01, G0 * g));
-        Gc[4 * ii + 1] = __builtin_fma(G3, l13, __builtin_fma(G2, l12, G1 * l11));
-        Gc[4 * ii + 2] = __builtin_fma(G3, l23, __builtin_fma(G2, l22, G1 * l21));
-        Gc[4 * ii + 3] = __builtin_fma(G3, l33, __builtin_fma(G1, l31, G0 * bg));
-      }
+    double gg, gbg, gct, gst, gcp, gsp;
+    if (fr) {
+      friend_adj(P, G, i > 0, g, bg, ce.ct, ce.st, cp, sp, (double)nc.friend_sign, ax, fR, fQ, fL, Gc, gg, gbg, gct,
+                 gst, gcp, gsp);
     } else {
+      // Ma = P L1: gP = G L1^T, gL1 = P^T G (structural non-zeros of L1 only)
+      const double l01 = -bg * ce.st, l03 = bg * ce.ct, l11 = ce.ct * cp, l12 = -sp, l13 = ce.st * cp;
+      const double l21 = ce.ct * sp, l22 = cp, l23 = ce.st * sp, l31 = -g * ce.st, l33 = g * ce.ct;
+      // gP goes straight to Gc, the adjoint handed to the parent (Gc was consumed above)
+      if (i > 0) {   // the root has no parent to hand gP to
 #pragma unroll
-      for (int q = 0; q < 16; q++) Gc[q] = 0;
+        for (int ii = 0; ii < 4; ii++) {
+          const double G0 = G[4 * ii], G1 = G[4 * ii + 1], G2 = G[4 * ii + 2], G3 = G[4 * ii + 3];
+          Gc[4 * ii + 0] = __builtin_fma(G3, l03, __builtin_fma(G1, l01, G0 * g));
+          Gc[4 * ii + 1] = __builtin_fma(G3, l13, __builtin_fma(G2, l12, G1 * l11));
+          Gc[4 * ii + 2] = __builtin_fma(G3, l23, __builtin_fma(G2, l22, G1 * l21));
+          Gc[4 * ii + 3] = __builtin_fma(G3, l33, __builtin_fma(G1, l31, G0 * bg));
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) Gc[q] = 0;
+      }
+      auto gl = [&](int k, int j) {
+        if (i == 0) return G[4 * k + j];   // P = identity
+        double acc = P[k] * G[j];
+#pragma unroll
+        for (int ii = 1; ii < 4; ii++) acc = __builtin_fma(P[4 * ii + k], G[4 * ii + j], acc);
+        return acc;
+      };
+      const double g00 = gl(0, 0), g01 = gl(0, 1), g03 = gl(0, 3), g11 = gl(1, 1), g12 = gl(1, 2), g13 = gl(1, 3);
+      const double g21 = gl(2, 1), g22 = gl(2, 2), g23 = gl(2, 3), g30 = gl(3, 0), g31 = gl(3, 1), g33 = gl(3, 3);
+      gg = g00 - g31 * ce.st + g33 * ce.ct;
+      gbg = -g01 * ce.st + g03 * ce.ct + g30;
+      gct = g03 * bg + g11 * cp + g21 * sp + g33 * g;
+      gst = -g01 * bg + g13 * cp + g23 * sp - g31 * g;
+      gcp = g11 * ce.ct + g13 * ce.st + g22;
+      gsp = -g12 + g21 * ce.ct + g23 * ce.st;
     }
-    auto gl = [&](int k, int j) {
-      if (i == 0) return G[4 * k + j];   // P = identity
-      double acc = P[k] * G[j];
-#pragma unroll
-      for (int ii = 1; ii < 4; ii++) acc = __builtin_fma(P[4 * ii + k], G[4 * ii + j], acc);
-      return acc;
-    };
-    const double g00 = gl(0, 0), g01 = gl(0, 1), g03 = gl(0, 3), g11 = gl(1, 1), g12 = gl(1, 2), g13 = gl(1, 3);
-    const double g21 = gl(2, 1), g22 = gl(2, 2), g23 = gl(2, 3), g30 = gl(3, 0), g31 = gl(3, 1), g33 = gl(3, 3);
-    double gg = g00 - g31 * ce.st + g33 * ce.ct;
-    double gbg = -g01 * ce.st + g03 * ce.ct + g30;
-    const double gct = g03 * bg + g11 * cp + g21 * sp + g33 * g;
-    const double gst = -g01 * bg + g13 * cp + g23 * sp - g31 * g;
-    const double gcp = g11 * ce.ct + g13 * ce.st + g22;
-    const double gsp = -g12 + g21 * ce.ct + g23 * ce.st;
     // phi = 2 pi x
     gx[Wk::xphi(nc, i)] = (gsp * cp - gcp * sp) * kTwoPi;
     // cos theta map of this branch (its own M through b0, b1)

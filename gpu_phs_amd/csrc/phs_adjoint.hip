@@ -63,7 +63,7 @@ constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_
 // XN > 0: every channel has at most XN randoms, and each event's row is held in registers.
 // SID >= 0: the tile's channel has that compile-time tree shape (XN is then its n_x).
 // Partials go to the tile's global unit.
-template <int XN, bool COMPACT, int SID, int LM, bool PEV>
+template <int XN, bool COMPACT, int SID, int LM, bool PEV, bool FR>
 __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const TileDesc& td, const SegDesc& sd,
                                              const ChannelDesc& cd, const NodeCmd* __restrict__ prog,
                                              double* __restrict__ gpart, int part_stride, const AdjLayout& lay,
@@ -97,9 +97,9 @@ __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const Ti
       double xv[XN];
 #pragma unroll
       for (int k = 0; k < XN; k++) xv[k] = xg[SID >= 0 || k < cd.n_x ? k : cd.n_x - 1];   // entries past n_x are never read
-      ok = event_adjoint<SID, LM, PEV>(cd, eb, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
+      ok = event_adjoint<SID, LM, PEV, FR>(cd, eb, prog, xv, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, accv, S);
     } else {
-      ok = event_adjoint<SID, LM, PEV>(cd, eb, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
+      ok = event_adjoint<SID, LM, PEV, FR>(cd, eb, prog, xg, gp, gf, p_out, &F, &L, sd.g_x + e * cd.n_x, acc_lds, S);
     }
     if (sd.factor) sd.factor[e] = F;
     if (sd.volume) sd.volume[e] = eb.volume;
@@ -127,9 +127,10 @@ __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const Ti
 // every tile's channel has one of the shapes of up to four final-state particles, and each
 // workgroup takes the specialised code of its tile's shape (one launch for several shapes:
 // a launch lasts at least one wave's lifetime, so a launch per shape would add that much per
-// shape).  The register budget keeps two waves per SIMD (at most 256 VGPRs).
+// shape).  The register budget keeps two waves per SIMD (at most 256 VGPRs).  FR: plans with
+// friend branches (the generic walk, randoms read from memory, accumulators in LDS).
 constexpr int kAdjMulti = 1000;
-template <int XN, bool COMPACT, int SID, int LM, bool PEV = false>
+template <int XN, bool COMPACT, int SID, int LM, bool PEV = false, bool FR = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(
     const TileDesc* __restrict__ tiles, int t_begin, int n_units, int n_sub, const SegDesc* __restrict__ segs,
     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, double* __restrict__ gpart,
@@ -143,18 +144,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ad
     const NodeCmd* prog = nodes + cd.node0;
     if constexpr (SID == kAdjMulti) {
       switch (cd.shape) {
-        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
         default: break;   // not reached: the host launches only tiles of shapes 0-7 here
       }
     } else {
-      adjoint_unit<XN, COMPACT, SID, LM, PEV>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds);
+      adjoint_unit<XN, COMPACT, SID, LM, PEV, FR>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds);
     }
   }
 }
@@ -241,6 +242,12 @@ AdjKernel adj_pick_seq(int sid, bool compact, int lm, std::integer_sequence<int,
 }
 template <int LM>
 AdjKernel adj_generic(const DevicePlan& dp, const AdjLayout& lay) {
+  if (dp.has_friends) {   // the host sets no compact stash and keeps the accumulators in LDS
+    if constexpr (LM == 0) {
+      if (dp.per_event_beams) return adjoint_kernel<0, false, -1, 0, true, true>;
+    }
+    return adjoint_kernel<0, false, -1, LM, false, true>;
+  }
   if constexpr (LM == 0) {   // per-event beams: the external cotangents only (the built-in loss assumes equal beams)
     if (dp.per_event_beams)
       return dp.nx_max > kAdjXReg ? adjoint_kernel<0, false, -1, 0, true>
@@ -273,7 +280,7 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
       while (r1 < n_runs && small(runs[r1].sid) && runs[r1].begin == end) end = runs[r1++].end;
     const int units = (end - runs[r].begin) * n_sub;
     AdjKernel k = nullptr;
-    if (dp.per_event_beams) k = generic;
+    if (dp.per_event_beams || dp.has_friends) k = generic;
     else if (r1 - r > 1) k = multi;
     else if (runs[r].sid >= 0)
       k = adj_pick_seq(runs[r].sid, lay.compact != 0, loss_mode, std::make_integer_sequence<int, kNumShapes>{});

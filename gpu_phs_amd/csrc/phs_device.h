@@ -209,10 +209,50 @@ __device__ __forceinline__ void compose_cols03(double (&M)[16], double g, double
   }
 }
 
+// The beam's direction in the branch's rest frame, n / |n| = (a0, a1, c), with the
+// intermediates the adjoint needs (u0 and u3: the beam's energy and z component seen from the
+// parent frame, n2 = -bg u0 + g u3).
+struct FriendAxis {
+  double u0, u3, n0, n1, n2, nn, a0, a1, c;
+};
+__host__ __device__ __forceinline__ FriendAxis friend_axis(const double (&M)[16], double g, double bg, double f) {
+  FriendAxis x;
+  x.u0 = M[0] - f * M[12];
+  x.n0 = -M[1] + f * M[13];
+  x.n1 = -M[2] + f * M[14];
+  x.u3 = -M[3] + f * M[15];
+  x.n2 = -bg * x.u0 + g * x.u3;
+  x.nn = math::sqrt_phys(x.n0 * x.n0 + x.n1 * x.n1 + x.n2 * x.n2);
+  x.a0 = x.n0 / x.nn;
+  x.a1 = x.n1 / x.nn;
+  x.c = x.n2 / x.nn;
+  return x;
+}
+
+// R (Rodrigues, about e_z x n, row-major r[3 a + b]) and Q = R R_z(phi) R_y(theta).
+__host__ __device__ __forceinline__ void friend_q(const FriendAxis& ax, double ct, double st, double cp, double sp,
+                                                  double (&r)[9], double (&q)[9]) {
+  const double a0 = ax.a0, a1 = ax.a1, c = ax.c;
+  const double sa = math::sqrt_phys(a0 * a0 + a1 * a1);
+  const double kx = sa > 0 ? -a1 / sa : 1.0;
+  const double ky = sa > 0 ? a0 / sa : 0.0;
+  const double h = 1 - c;
+  r[0] = c + h * kx * kx; r[1] = h * kx * ky; r[2] = a0;
+  r[3] = r[1];            r[4] = c + h * ky * ky; r[5] = a1;
+  r[6] = -a0;             r[7] = -a1;             r[8] = c;
+  const double rc0 = cp * ct, rc1 = sp * ct, rs0 = cp * st, rs1 = sp * st;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q[3 * a + 0] = r[3 * a] * rc0 + r[3 * a + 1] * rc1 + r[3 * a + 2] * -st;
+    q[3 * a + 1] = r[3 * a] * -sp + r[3 * a + 1] * cp;
+    q[3 * a + 2] = r[3 * a] * rs0 + r[3 * a + 1] * rs1 + r[3 * a + 2] * ct;
+  }
+}
+
 // M <- M * L1 for a branch whose polar axis is an incoming beam, its "friend"
 // (phs.cu:853-910).  The beam k = (1, 0, 0, f) (f = +1 along +z, -1 along -z; the reference
-// scales k by the beam energy, which the normalisation below removes) is seen from the
-// branch's rest frame as n = spatial part of B_z(bg)^-1 M^-1 k, with M^-1 = eta M^T eta.
+// scales k by the beam energy, which the normalisation removes) is seen from the branch's
+// rest frame as n = spatial part of B_z(bg)^-1 M^-1 k, with M^-1 = eta M^T eta (friend_axis).
 // R is the rotation about e_z x n that takes e_z to n / |n| (Rodrigues' formula; for n along
 // -e_z, the half turn about e_x), and L1 = B_z(bg) (1 (+) R R_z(phi) R_y(theta)), so the
 // first daughter's polar angle theta is measured from the beam.  The reference's matrix for
@@ -221,34 +261,17 @@ __device__ __forceinline__ void compose_cols03(double (&M)[16], double g, double
 // the order of the CPU restatement, which forms the same matrices.
 __device__ __forceinline__ void compose_friend(double (&M)[16], double g, double bg, double ct, double st, double cp,
                                                double sp, double f) {
-  const double u0 = M[0] - f * M[12];
-  const double n0 = -M[1] + f * M[13];
-  const double n1 = -M[2] + f * M[14];
-  const double u3 = -M[3] + f * M[15];
-  const double n2 = -bg * u0 + g * u3;
-  const double nn = math::sqrt_phys(n0 * n0 + n1 * n1 + n2 * n2);
-  const double a0 = n0 / nn, a1 = n1 / nn, c = n2 / nn;
-  const double sa = math::sqrt_phys(a0 * a0 + a1 * a1);
-  const double kx = sa > 0 ? -a1 / sa : 1.0;
-  const double ky = sa > 0 ? a0 / sa : 0.0;
-  const double h = 1 - c;
-  const double r00 = c + h * kx * kx, r01 = h * kx * ky, r02 = a0;
-  const double r10 = r01, r11 = c + h * ky * ky, r12 = a1;
-  const double r20 = -a0, r21 = -a1, r22 = c;
-  // Q = R R_z(phi) R_y(theta)
-  const double rc0 = cp * ct, rc1 = sp * ct, rs0 = cp * st, rs1 = sp * st;
-  const double q00 = r00 * rc0 + r01 * rc1 + r02 * -st, q01 = r00 * -sp + r01 * cp, q02 = r00 * rs0 + r01 * rs1 + r02 * ct;
-  const double q10 = r10 * rc0 + r11 * rc1 + r12 * -st, q11 = r10 * -sp + r11 * cp, q12 = r10 * rs0 + r11 * rs1 + r12 * ct;
-  const double q20 = r20 * rc0 + r21 * rc1 + r22 * -st, q21 = r20 * -sp + r21 * cp, q22 = r20 * rs0 + r21 * rs1 + r22 * ct;
-  const double l01 = bg * q20, l02 = bg * q21, l03 = bg * q22;
-  const double l31 = g * q20, l32 = g * q21, l33 = g * q22;
+  double r[9], q[9];
+  friend_q(friend_axis(M, g, bg, f), ct, st, cp, sp, r, q);
+  const double l01 = bg * q[6], l02 = bg * q[7], l03 = bg * q[8];
+  const double l31 = g * q[6], l32 = g * q[7], l33 = g * q[8];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const double x0 = M[4 * i], x1 = M[4 * i + 1], x2 = M[4 * i + 2], x3 = M[4 * i + 3];
     M[4 * i + 0] = x0 * g + x3 * bg;
-    M[4 * i + 1] = x0 * l01 + x1 * q00 + x2 * q10 + x3 * l31;
-    M[4 * i + 2] = x0 * l02 + x1 * q01 + x2 * q11 + x3 * l32;
-    M[4 * i + 3] = x0 * l03 + x1 * q02 + x2 * q12 + x3 * l33;
+    M[4 * i + 1] = x0 * l01 + x1 * q[0] + x2 * q[3] + x3 * l31;
+    M[4 * i + 2] = x0 * l02 + x1 * q[1] + x2 * q[4] + x3 * l32;
+    M[4 * i + 3] = x0 * l03 + x1 * q[2] + x2 * q[5] + x3 * l33;
   }
 }
 

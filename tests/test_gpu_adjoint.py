@@ -9,7 +9,7 @@ from gpu_phs_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _setup(proc, n, seed, lo=0.02):
+def _setup(proc, n, seed, lo=0.02, E1=None, E2=None):
     import torch
     import gpu_phs_amd as G
     rng = np.random.default_rng(seed)
@@ -17,7 +17,7 @@ def _setup(proc, n, seed, lo=0.02):
     counts[: n - counts.sum()] += 1
     ch = np.repeat(np.arange(proc.n_channels), counts).astype(np.int32)
     x = lo + (1 - 2 * lo) * rng.random((n, proc.n_x))
-    ph = G.ProcessHandle(proc)
+    ph = G.ProcessHandle(proc, E1=E1, E2=E2)
     xt = torch.from_numpy(x).cuda()
     bufs, segs = [], []
     start = 0
@@ -166,6 +166,64 @@ def test_adjoint_per_event_beams_matches_dual_numbers(name):
     gpar = gparam.cpu().numpy().reshape(-1, 2)[: len(slots)]
     sc = np.abs(gpar_ref).max() + 1e-300
     print(name, "param max rel err", np.abs(gpar - gpar_ref).max() / sc)
+    assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
+
+
+@pytest.mark.parametrize("case", ["friends4", "friends4-loss", "friends6-unequal", "friends4-event-beams",
+                                  "friends6-event-beams"])
+def test_adjoint_friends_matches_dual_numbers(case):
+    """Branches whose polar axis is an incoming beam (friend signs -1 and +1, on the root and on
+    inner branches): their boost depends on their parent boost through the beam's direction,
+    so the adjoint of the parent boost has a second term.  Against the oracle's dual numbers
+    through the same friend construction, with external cotangents and with the built-in loss,
+    at equal, unequal and per-event beams; d loss / d x and d loss / d (M, Gamma)."""
+    import torch
+    n_out = 6 if "friends6" in case else 4
+    proc = W.process_friends(n_out)
+    n = 4000 if n_out == 4 else 12000
+    lm = 1 if case.endswith("loss") else 0
+    E1 = E2 = None
+    if "unequal" in case:
+        E1, E2 = 0.6 * proc.sqrts, proc.sqrts ** 2 / (4 * 0.6 * proc.sqrts)
+    rng, ch, x, ph, plan, bufs, counts = _setup(proc, n, 41 + n_out, E1=E1, E2=E2)
+    e1 = e2 = None
+    if E1 is not None:
+        e1, e2 = np.full(n, E1), np.full(n, E2)
+    if "event-beams" in case:
+        h2 = proc.sqrts / 2
+        e1, e2 = h2 * rng.uniform(0.8, 1.3, n), h2 * rng.uniform(0.8, 1.3, n)
+        e1s = torch.split(torch.from_numpy(e1).cuda(), list(counts))
+        e2s = torch.split(torch.from_numpy(e2).cuda(), list(counts))
+        for i in range(len(bufs)):
+            plan.set_event_beams(i, e1s[i], e2s[i])
+    gp = rng.standard_normal((n, proc.n_out, 4))
+    gf = rng.standard_normal(n)
+    kw = {}
+    if lm == 0:
+        gpt, gft = torch.from_numpy(gp).cuda(), torch.from_numpy(gf).cuda()
+        kw = dict(g_p=list(torch.split(gpt, list(counts))), g_f=list(torch.split(gft, list(counts))))
+    gx = [torch.empty_like(b.x) for b in bufs]
+    print("friend adjoint", case, flush=True)
+    gparam = plan.adjoint(gx, loss_mode=lm, **kw)
+    torch.cuda.synchronize()
+    g_gpu = torch.cat(gx).cpu().numpy()
+    g_ref, ref = _dual_grads(proc, x, ch, gp, gf, lm, E1=e1, E2=e2)
+    ok = ref["ok"]
+    okg = torch.cat([b.ok for b in bufs]).cpu().numpy().astype(bool)
+    assert np.array_equal(okg, ok) and ok.sum() > 200
+    scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(g_gpu - g_ref) / scale
+    print(case, "max rel err", err[ok].max())
+    assert err[ok].max() < 1e-9
+    assert (g_gpu[~ok] == 0).all()
+    p_gpu = torch.cat([b.p for b in bufs]).cpu().numpy()
+    assert np.allclose(p_gpu[ok], ref["p"][ok], rtol=0, atol=1e-10 * proc.sqrts)
+    pr, pch, pbc = plan.param_map()
+    slots = list(zip(pch.tolist(), pbc.tolist()))
+    gpar_ref = _dual_param_grads(proc, x, ch, ok, slots, gp, gf, lm, E1=e1, E2=e2)
+    gpar = gparam.cpu().numpy().reshape(-1, 2)[: len(slots)]
+    sc = np.abs(gpar_ref).max() + 1e-300
+    print(case, "param max rel err", np.abs(gpar - gpar_ref).max() / sc)
     assert np.abs(gpar - gpar_ref).max() <= 1e-9 * sc
 
 

@@ -46,17 +46,6 @@ def test_root_friend_along_plus_z_equals_plain():
         assert np.array_equal(plain[k], fr[k], equal_nan=k != "ok"), k
 
 
-def test_adjoint_rejects_friends():
-    import torch
-    import gpu_phs_amd as G
-    proc = W.process_friends(4)
-    x, ch = _events(proc, 256, 4)
-    _, plan, bufs = run_gpu(proc, x, ch)
-    g_x = [torch.zeros_like(b.x) for b in bufs]
-    with pytest.raises(G.GpuPhsError, match="friend"):
-        plan.adjoint(g_x)
-
-
 def test_friend_trees_reproduce_whizard_momenta():
     """The 149 events of samples/bwbw.ref whose trees have friends, with the friend assignment
     that Whizard's own factors single out (test_whizard_friends.py), through the kernel."""
