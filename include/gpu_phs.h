@@ -64,7 +64,7 @@ void c_whizard_init_tree_structures(void);
  * read as phs.cu:276-284 reads it: 2^n_out -> the beam along -z, 2^(n_out+1) -> the beam
  * along +z, any other value -> the branch's own z axis).  The rotation onto the beam is a
  * proper rotation (Rodrigues), where the reference's L_rot_to_2nd (phs.cu:753-781) is not
- * orthogonal; trees with friends have no adjoint. */
+ * orthogonal. */
 void c_whizard_fill_tree_structure(int* channel, int* daughters1, int* daughters2, int* has_children,
                                    int* friends);
 /* whizard_gpu_phs.cpp:89-98: channel of every event of the next batch from cumulative
@@ -174,8 +174,8 @@ int gpu_phs_plan_get_schedule(int plan, int* order, int* serial);
 /* Per-event beam energies of segment `segment` (device arrays [n_events], or both NULL for the
  * process's fixed beams): each event gets sqrt(s) = 2 sqrt(E1 E2), its lab boost and the map
  * constants that depend on sqrt(s), as the reference does per event (phs.cu:1313-1317,
- * 1286-1301; mappings_gpu.hpp).  The arrays must outlive the plan's use.  Plans with per-event
- * beams have no adjoint. */
+ * 1286-1301; mappings_gpu.hpp).  The arrays must outlive the plan's use.  The adjoint of a
+ * plan with per-event beams takes external cotangents only (loss_mode 0). */
 int gpu_phs_plan_set_event_beams(int plan, int segment, const double* E1, const double* E2);
 /* Per-branch msq output of segment `segment`: a device array [n_events][2^n_out - 1] that
  * receives, at index bincode - 1, the invariant mass squared of every branch of the event's
@@ -214,7 +214,11 @@ typedef struct {
  * the loss is sum over ok events of factor * sum_j (p_jx^2 + p_jy^2) / s, and g_p, g_f are
  * ignored.  Writes g_x, loss, the forward outputs when write_forward != 0, and gparam
  * [n_params]: d loss / d M at 2 slot, d loss / d Gamma at 2 slot + 1, per (channel, branch)
- * slot of gpu_phs_plan_param_map.  Trees up to 2 -> 6.  The plan owns the partial sums and
+ * slot of gpu_phs_plan_param_map.  Covers friend trees and per-event beams (loss_mode 0
+ * there, since the built-in loss assumes symmetric beams); trees whose tape (8 doubles per
+ * branch, 19 per deferred level, 2 more per branch above four final-state particles, for each
+ * of a workgroup's 64 events) exceeds the device's LDS per workgroup are rejected with
+ * GPU_PHS_E_TREE: on MI355X, 2 -> 10 takes 75 KB of 160 KB.  The plan owns the partial sums and
  * per-channel completion counters of the reduction: adjoint calls on one plan must not overlap
  * (queue them on one stream). */
 int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adjoint, int loss_mode, int write_forward,
