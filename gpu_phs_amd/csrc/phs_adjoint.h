@@ -434,7 +434,9 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
     for (int c = 0; c < 2; c++) {
       if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
       else {
-        MapEval r = msq_eval(PEV ? child_at(nc.k[c], eb.sqrts, s, cd.m_tot) : nc.k[c], xr[Wk::xi(nc, i, c)], s);
+        MapEval r;
+        if constexpr (PEV) r = msq_eval(child_at(nc.k[c], eb.sqrts, s, cd.m_tot), xr[Wk::xi(nc, i, c)], s);
+        else r = msq_eval(nc.k[c], xr[Wk::xi(nc, i, c)], s);   // no copy of the record
         msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq);
       }
     }
@@ -539,7 +541,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
       else {
         xk[c] = xr[Wk::xi(nc, i, c)];
-        r[c] = msq_eval(PEV ? child_at(nc.k[c], eb.sqrts, s, cd.m_tot) : nc.k[c], xk[c], s);
+        if constexpr (PEV) r[c] = msq_eval(child_at(nc.k[c], eb.sqrts, s, cd.m_tot), xk[c], s);
+        else r[c] = msq_eval(nc.k[c], xk[c], s);
         msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]);
       }
     }
@@ -703,7 +706,8 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       if (leaf[c]) continue;
       if (mk[c] > 0) g_msq[c] += math::quot(g_m[c], 2 * mk[c]);
       double gxk = 0, gM = 0, gW = 0;
-      msq_eval_adj(child(nc, i, c), xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      if constexpr (PEV) msq_eval_adj(child(nc, i, c), xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      else msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
       gx[Wk::xi(nc, i, c)] = gxk;
       gpar[2 * Wk::child(nc, i, c)] += gM;
       gpar[2 * Wk::child(nc, i, c) + 1] += gW;
