@@ -261,7 +261,9 @@ __device__ __forceinline__ void forward_event(const ChannelDesc& cd, const EvBea
 #undef GPHS_X
 }
 
-// Momenta of `cnt` events from ps[(4 j + mu) * S + event] to dst = [cnt][n_out][4].  With a
+// Momenta of `cnt` events from ps[(4 j + mu) * S + event] to dst = [cnt][n_out][4], as
+// streaming (non-temporal) stores: nothing here reads them again, and keeping them out of the
+// caches measured 2-3% faster on the batched mix.  With a
 // compile-time workgroup size BLK and n_out NOUT of 2 or 4 (16-byte words per event W = 2 NOUT
 // dividing BLK), the word thread t stores in round k is t + BLK k: event t / W + k BLK / W,
 // component pair t % W -- one address per thread, and every round's LDS and global offsets are
@@ -282,7 +284,7 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
           dvec2 v;
           v.x = src[k * (BLK / W)];
           v.y = src[SS + k * (BLK / W)];
-          d2[k * BLK] = v;
+          __builtin_nontemporal_store(v, d2 + k * BLK);
         }
       }
       return;
@@ -300,13 +302,13 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
       dvec2 v;
       v.x = ps[c * S + ev];
       v.y = ps[(c + 1) * S + ev];
-      d2[w] = v;
+      __builtin_nontemporal_store(v, d2 + w);
     }
   } else {
     gdouble* d1 = as_global(dst);
     for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) {
       const int ev = (int)__umulhi((unsigned)i, cd.div_p);
-      d1[i] = ps[(i - ev * w4) * S + ev];
+      __builtin_nontemporal_store(ps[(i - ev * w4) * S + ev], d1 + i);
     }
   }
 }
@@ -364,8 +366,8 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
       forward_event<BANK, XN, FR, PEV, SID>(cd, eb, prog, xv, ps + lane, xs + lane, S, bank_base, want_msq, msq_row,
                                             factor, ok);
       if (valid) {
-        as_global(sd.factor)[e0 + lane] = factor;
-        as_global(sd.volume)[e0 + lane] = eb.volume;
+        __builtin_nontemporal_store(factor, as_global(sd.factor) + e0 + lane);
+        __builtin_nontemporal_store(eb.volume, as_global(sd.volume) + e0 + lane);
         as_global(sd.ok)[e0 + lane] = ok ? 1 : 0;
       }
       __syncthreads();
