@@ -76,6 +76,8 @@ _SIGS = {
     "gpu_phs_plan_get_schedule": (ctypes.c_int, [ctypes.c_int] + [ctypes.c_void_p] * 2),
     "gpu_phs_plan_set_merge_below": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
     "gpu_phs_plan_get_merge_below": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
+    "gpu_phs_plan_set_streams": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "gpu_phs_plan_get_streams": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_set_specialize": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "gpu_phs_grid_create": (ctypes.c_int, [ctypes.c_int]),
     "gpu_phs_grid_destroy": (ctypes.c_int, [ctypes.c_int]),
@@ -215,6 +217,11 @@ class Plan:
         launch (0: every shape its own specialised launch)."""
         _check(lib().gpu_phs_plan_set_merge_below(self.handle, int(events)), "plan_set_merge_below")
 
+    def set_streams(self, n_streams):
+        """Spread the plan's launches over 1-4 streams (1: the tile classes as the schedule
+        says)."""
+        _check(lib().gpu_phs_plan_set_streams(self.handle, int(n_streams)), "plan_set_streams")
+
     def set_config(self, **config):
         """Launch configuration: block, ept, grid, order, waves_per_eu, lds; keys left out
         keep their current values (the library's defaults for a new plan)."""
@@ -234,7 +241,7 @@ class Plan:
 
     def tuning(self):
         """The launch shape of both tile classes and the schedule, as the autotuner left them:
-        dict(classes=[{block, ept, grid, waves_per_eu, lds}] * 2, order, serial, merge_below)."""
+        dict(classes=[{block, ept, grid, waves_per_eu, lds}] * 2, order, serial, merge_below, streams)."""
         classes = []
         for c in range(2):
             v = [ctypes.c_int() for _ in range(5)]
@@ -245,7 +252,9 @@ class Plan:
         _check(lib().gpu_phs_plan_get_schedule(self.handle, ctypes.byref(o), ctypes.byref(se)), "plan_get_schedule")
         mb = ctypes.c_int64()
         _check(lib().gpu_phs_plan_get_merge_below(self.handle, ctypes.byref(mb)), "plan_get_merge_below")
-        return dict(classes=classes, order=o.value, serial=se.value, merge_below=mb.value)
+        ns = ctypes.c_int()
+        _check(lib().gpu_phs_plan_get_streams(self.handle, ctypes.byref(ns)), "plan_get_streams")
+        return dict(classes=classes, order=o.value, serial=se.value, merge_below=mb.value, streams=ns.value)
 
     def apply_tuning(self, t):
         """Set what ``tuning()`` returned (e.g. from an earlier autotuner run)."""
@@ -255,6 +264,8 @@ class Plan:
         _check(lib().gpu_phs_plan_set_schedule(self.handle, t["order"], t["serial"]), "plan_set_schedule")
         if "merge_below" in t:
             _check(lib().gpu_phs_plan_set_merge_below(self.handle, t["merge_below"]), "plan_set_merge_below")
+        if "streams" in t:
+            _check(lib().gpu_phs_plan_set_streams(self.handle, t["streams"]), "plan_set_streams")
 
     def info(self):
         v = [ctypes.c_int() for _ in range(3)]

@@ -71,6 +71,10 @@ PlanObj::~PlanObj() {
   if (side_stream) (void)hipStreamDestroy(side_stream);
   if (fork_event) (void)hipEventDestroy(fork_event);
   if (join_event) (void)hipEventDestroy(join_event);
+  for (int k = 0; k < kMaxStreams - 1; k++) {
+    if (lane_streams[k]) (void)hipStreamDestroy(lane_streams[k]);
+    if (lane_join[k]) (void)hipEventDestroy(lane_join[k]);
+  }
   if (dev) (void)hipFree(dev);
 }
 
@@ -270,6 +274,24 @@ int plan_build(PlanObj& pl) {
       else runs.back().end = t + 1;
     }
   }
+  // the launches, heaviest first, for spreading them over streams
+  pl.jobs.clear();
+  auto tile_bytes = [&](int t) {
+    const ChannelDesc& cd = pl.chans[pl.segs[pl.tiles[t].seg].channel];
+    return (double)pl.tiles[t].count * (8.0 * cd.n_x + 32.0 * cd.n_out + 17.0);
+  };
+  for (int c = 0; c < kTileClasses; c++) {
+    if (pl.class_end[c] <= pl.class_begin[c]) continue;
+    std::vector<ShapeRun> runs = pl.shape_runs[c];
+    if (runs.empty()) runs.push_back(ShapeRun{-1, pl.class_begin[c], pl.class_end[c]});
+    for (const ShapeRun& r : runs) {
+      double b = 0;
+      for (int t = r.begin; t < r.end; t++) b += tile_bytes(t);
+      pl.jobs.push_back(PlanObj::Job{c, r.begin, r.end, r.sid, b});
+    }
+  }
+  std::stable_sort(pl.jobs.begin(), pl.jobs.end(),
+                   [](const PlanObj::Job& a, const PlanObj::Job& b) { return a.bytes > b.bytes; });
   return ok_status();
 }
 
@@ -292,7 +314,40 @@ hipError_t launch_class(PlanObj& pl, int c, hipStream_t stream) {
 
 // Both classes on `stream`'s timeline: class 0 on `stream`, class 1 (when both have tiles) on
 // the side stream between a fork and a join event, so the two kernels can share the GPU.
+// Every launch of the plan over n_streams streams: heaviest first onto the least loaded one.
+hipError_t launch_jobs(PlanObj& pl, hipStream_t stream) {
+  const int ns = std::min<int>(pl.n_streams, (int)pl.jobs.size());
+  hipError_t e;
+  if (!pl.fork_event && (e = hipEventCreateWithFlags(&pl.fork_event, hipEventDisableTiming)) != hipSuccess) return e;
+  for (int k = 0; k < ns - 1; k++) {
+    if (!pl.lane_streams[k] && (e = hipStreamCreateWithFlags(&pl.lane_streams[k], hipStreamNonBlocking)) != hipSuccess)
+      return e;
+    if (!pl.lane_join[k] && (e = hipEventCreateWithFlags(&pl.lane_join[k], hipEventDisableTiming)) != hipSuccess)
+      return e;
+  }
+  if ((e = hipEventRecord(pl.fork_event, stream)) != hipSuccess) return e;
+  for (int k = 0; k < ns - 1; k++)
+    if ((e = hipStreamWaitEvent(pl.lane_streams[k], pl.fork_event, 0)) != hipSuccess) return e;
+  double load[kMaxStreams] = {0, 0, 0, 0};
+  for (const PlanObj::Job& j : pl.jobs) {
+    int k = 0;
+    for (int q = 1; q < ns; q++)
+      if (load[q] < load[k]) k = q;
+    load[k] += j.bytes;
+    hipStream_t st = k == 0 ? stream : pl.lane_streams[k - 1];
+    e = j.sid >= 0 ? launch_forward_shape(pl.dps[j.cls], j.sid, j.begin, j.end, pl.lcs[j.cls], st)
+                   : launch_forward(pl.dps[j.cls], j.begin, j.end, pl.lcs[j.cls], st);
+    if (e != hipSuccess) return e;
+  }
+  for (int k = 0; k < ns - 1; k++) {
+    if ((e = hipEventRecord(pl.lane_join[k], pl.lane_streams[k])) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(stream, pl.lane_join[k], 0)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_classes(PlanObj& pl, hipStream_t stream) {
+  if (pl.n_streams > 1 && pl.jobs.size() > 1) return launch_jobs(pl, stream);
   const bool two = pl.class_end[0] > pl.class_begin[0] && pl.class_end[1] > pl.class_begin[1];
   if (!two || pl.serial_classes) {
     for (int c = 0; c < kTileClasses; c++) {
@@ -560,6 +615,24 @@ int gpu_phs_plan_set_merge_below(int plan, int64_t events) {
   return plan_build(*pl);
 }
 
+int gpu_phs_plan_set_streams(int plan, int n_streams) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (n_streams < 1 || n_streams > kMaxStreams) return set_error(GPU_PHS_E_ARG, "n_streams in 1..4");
+  pl->n_streams = n_streams;
+  drop_graph(*pl);
+  return ok_status();
+}
+
+int gpu_phs_plan_get_streams(int plan, int* n_streams) {
+  std::lock_guard<std::mutex> lk(api_mutex());
+  PlanObj* pl = find_plan(plan);
+  if (!pl) return set_error(GPU_PHS_E_HANDLE, "unknown plan");
+  if (n_streams) *n_streams = pl->n_streams;
+  return ok_status();
+}
+
 int gpu_phs_plan_get_merge_below(int plan, int64_t* events) {
   std::lock_guard<std::mutex> lk(api_mutex());
   PlanObj* pl = find_plan(plan);
@@ -759,6 +832,7 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
   }
   // tile order, and whether two classes share the GPU or run one after the other, for the
   // whole plan
+  pl->n_streams = 1;
   float best_order_ms = 1e30f;
   int best_order = pl->order, best_serial = pl->serial_classes;
   const bool two = pl->class_end[0] > pl->class_begin[0] && pl->class_end[1] > pl->class_begin[1];
@@ -788,13 +862,28 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
       if (!rc) rc = time_now(-1, &ms);
       if (!rc && ms < 0.98f * best_merge_ms) { best_merge_ms = ms; best_merge = mb; }
     }
+  if (!rc) {
+    pl->merge_below = best_merge;
+    drop_graph(*pl);
+    rc = plan_build(*pl);
+  }
+  // then over how many streams the launches spread (1: the classes as chosen above)
+  int best_streams = pl->n_streams;
+  float best_streams_ms = 1e30f;
+  if (!rc && pl->jobs.size() > 1)
+    for (int ns : {1, 2, 3, 4}) {
+      if (rc) break;
+      pl->n_streams = ns;
+      drop_graph(*pl);
+      float ms = 0;
+      rc = time_now(-1, &ms);
+      if (!rc && ms < 0.98f * best_streams_ms) { best_streams_ms = ms; best_streams = ns; }
+    }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;
-  pl->merge_below = best_merge;
+  pl->n_streams = best_streams;
   drop_graph(*pl);
-  rc = plan_build(*pl);
-  if (rc) return rc;
   if (report && report_len > 0) {
     std::string s;
     char buf[128];
@@ -805,8 +894,8 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
                lc.waves_per_eu, lc.lds);
       s += buf;
     }
-    snprintf(buf, sizeof buf, "order=%d serial=%d %.4f ms; merge_below=%lld %.4f ms;", best_order, best_serial,
-             best_order_ms, (long long)best_merge, best_merge_ms);
+    snprintf(buf, sizeof buf, "order=%d serial=%d %.4f ms; merge_below=%lld %.4f ms; streams=%d %.4f ms;", best_order,
+             best_serial, best_order_ms, (long long)best_merge, best_merge_ms, best_streams, best_streams_ms);
     s += buf;
     for (const auto& c : tried) {
       snprintf(buf, sizeof buf, " c%d:%d/%d/%d/%d/%d:%.4f", c.cls, c.lc.block, c.lc.ept, c.lc.grid, c.lc.waves_per_eu,

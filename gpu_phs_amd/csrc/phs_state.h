@@ -28,6 +28,7 @@ struct ProcessObj {
 };
 
 constexpr int kTileClasses = 2;
+constexpr int kMaxStreams = 4;
 
 // A run of consecutive tiles of one tile class whose channels share one tree shape (or -1:
 // channels without a shape).
@@ -69,6 +70,18 @@ struct PlanObj {
   // the second tile class runs beside the first: a side stream joined by events
   hipStream_t side_stream = nullptr;
   hipEvent_t fork_event = nullptr, join_event = nullptr;
+  // n_streams > 1: every launch of the plan (a shape run, or a class's generic launch) goes to
+  // one of n_streams streams -- the caller's and n_streams - 1 side streams joined back by
+  // events -- heaviest first onto the least loaded one, so that a launch's last workgroups
+  // share the GPU with other launches instead of leaving it partly idle
+  int n_streams = 1;
+  hipStream_t lane_streams[kMaxStreams - 1] = {};
+  hipEvent_t lane_join[kMaxStreams - 1] = {};
+  struct Job {
+    int cls, begin, end, sid;
+    double bytes;   // algorithmic bytes of its events
+  };
+  std::vector<Job> jobs;   // every launch of a run of the plan, heaviest first
   // adjoint: a copy of the tables with the gradient buffers, per-tile partials, tiles by channel
   void* adj_dev = nullptr;
   size_t adj_bytes = 0;

@@ -169,6 +169,14 @@ int gpu_phs_plan_set_schedule(int plan, int order, int serial);
  * small shapes are cheaper folded into a launch that runs anyway.  No counterpart in the
  * reference. */
 int gpu_phs_plan_set_merge_below(int plan, int64_t events);
+/* Number of streams (1..4) the plan's launches spread over: with n_streams > 1 every launch of
+ * the plan (one per tree shape, plus the generic one) goes to one of the caller's stream and
+ * n_streams - 1 side streams, heaviest first onto the least loaded, joined back into the
+ * caller's stream by events; the classes' serial flag then does not apply.  A launch ends with
+ * its slowest workgroups while the others' slots idle; launches on several streams fill those
+ * slots.  Default 1.  No counterpart in the reference. */
+int gpu_phs_plan_set_streams(int plan, int n_streams);
+int gpu_phs_plan_get_streams(int plan, int* n_streams);
 int gpu_phs_plan_get_merge_below(int plan, int64_t* events);
 int gpu_phs_plan_get_schedule(int plan, int* order, int* serial);
 /* Per-event beam energies of segment `segment` (device arrays [n_events], or both NULL for the

@@ -142,10 +142,12 @@ def test_plain_batched_graph_identical():
     plan = G.Plan(segs)
     results = []
     # every shape its own specialised launch (merge_below 0), shapes below 2^12 events folded
-    # into the class's generic launch, and every tree on the generic kernel
-    for merge, mode in ((0, "run"), (0, "run_plain"), (0, "run_graph"), (0, "run_graph"), (1 << 12, "run"),
-                        (1 << 12, "run_graph"), (1 << 62, "run")):
-        plan.apply_tuning(dict(plan.tuning(), merge_below=merge))
+    # into the class's generic launch, and every tree on the generic kernel; the launches on one
+    # stream per class or spread over three or four streams (captured in a graph as well)
+    for merge, mode, streams in ((0, "run", 1), (0, "run_plain", 1), (0, "run_graph", 1), (0, "run_graph", 1),
+                                 (1 << 12, "run", 1), (1 << 12, "run_graph", 1), (1 << 62, "run", 1),
+                                 (0, "run", 3), (1 << 12, "run_graph", 4), (1 << 12, "run_graph", 4)):
+        plan.apply_tuning(dict(plan.tuning(), merge_below=merge, streams=streams))
         for b in bufs:
             b.p.fill_(0)
             b.factor.fill_(0)
@@ -192,3 +194,42 @@ def test_ragged_and_empty_segments():
             out = dict(p=b.p.cpu().numpy(), factor=b.factor.cpu().numpy(), volume=b.volume.cpu().numpy(),
                        ok=b.ok.cpu().numpy().astype(bool))
             assert_parity(forward_check(proc, xs[i], np.full(n, i % 3, np.int32), out))
+
+
+def test_autotune_tuning_round_trip():
+    """The autotuner's choice, read back with tuning() and applied to a new plan over the same
+    buffers, reproduces the tuned plan's results bit for bit; every stage shows in its report."""
+    torch = _torch()
+    import gpu_phs_amd as G
+    wl = W.workload("c3", scale=0.002)
+    handles = [G.ProcessHandle(p) for p in wl.processes]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    segs = []
+    for s in wl.segments:
+        ph = handles[s.process]
+        x = torch.rand((s.n_events, ph.n_x), dtype=torch.float64, device="cuda", generator=gen)
+        segs.append((ph, s.channel, G.EventBuffers.allocate(x, ph.n_out)))
+    plan = G.Plan(segs)
+    print("autotune", flush=True)
+    report = plan.autotune(reps=2)
+    for key in ("class 0:", "order=", "merge_below=", "streams="):
+        assert key in report, report
+    t = plan.tuning()
+    assert set(t) == {"classes", "order", "serial", "merge_below", "streams"}
+    assert 1 <= t["streams"] <= 4
+    plan.run()
+    torch.cuda.synchronize()
+    ref = [(b.p.clone(), b.factor.clone(), b.ok.clone()) for _, _, b in segs]
+    other = G.Plan(segs)
+    other.apply_tuning(t)
+    assert other.tuning() == t
+    for _, _, b in segs:
+        b.p.fill_(0)
+        b.factor.fill_(0)
+    other.run_graph()
+    torch.cuda.synchronize()
+    for (p0, f0, o0), (_, _, b) in zip(ref, segs):
+        assert torch.equal(o0, b.ok)
+        okm = o0.bool()
+        assert torch.equal(p0[okm], b.p[okm]) and torch.equal(f0[okm], b.factor[okm])
