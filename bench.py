@@ -444,6 +444,9 @@ def main():
         f"{main_res['plain_ev_s']:.4g} ev/s plain")
     kernel_s = main_res["batched_kernel_ms"] / 1e3
     achieved = main_res["algorithmic_bytes"] / kernel_s / 1e9
+    # the device's own streaming copy (2 GiB, read + written bytes): the practical ceiling
+    copy_gb_s = G.copy_bandwidth(2 << 30, 5)
+    log(f"streaming copy: {copy_gb_s:.0f} GB/s; the step moves {achieved:.0f} GB/s")
     extra = {}
     if not args.no_extra:
         del plan, segs
@@ -530,7 +533,9 @@ def main():
                      "traffic": counters["traffic"] if counters else None,
                      "traffic_counted_with": counters["tuning"] if counters else None,
                      "kernel": "forward_kernel (every launch of one step)", "kernel_ms": main_res["batched_kernel_ms"],
-                     "bytes_per_launch": main_res["algorithmic_bytes"], "fp64": fp64},
+                     "bytes_per_launch": main_res["algorithmic_bytes"], "fp64": fp64,
+                     "copy_ceiling": {"gb_s": copy_gb_s, "frac": achieved / copy_gb_s,
+                                      "what": "the device's streaming copy of 2 GiB, one 16-byte word per thread"}},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -77,6 +77,7 @@ _SIGS = {
     "gpu_phs_plan_set_merge_below": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64]),
     "gpu_phs_plan_get_merge_below": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_set_streams": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "gpu_phs_copy_bandwidth": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gpu_phs_plan_get_streams": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p]),
     "gpu_phs_plan_set_specialize": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "gpu_phs_grid_create": (ctypes.c_int, [ctypes.c_int]),
@@ -165,6 +166,14 @@ class ProcessHandle:
             self.close()
         except Exception:
             pass
+
+
+def copy_bandwidth(nbytes: int = 2 << 30, reps: int = 5, stream=None) -> float:
+    """The device's streaming copy bandwidth in GB/s (read + written bytes per second)."""
+    v = ctypes.c_double()
+    _check(lib().gpu_phs_copy_bandwidth(int(nbytes), int(reps), _stream_ptr(stream), ctypes.byref(v)),
+           "copy_bandwidth")
+    return v.value
 
 
 def load_ref(path: str, E1: float, E2: float) -> int:

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "phs_shapes.h"
@@ -726,13 +727,30 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
     return set_error(GPU_PHS_E_HIP, "hipEventCreate");
   // time one class's launch (cls >= 0) or the whole plan (cls < 0) under the current tables
+  // Warm-up before a timing: the launches themselves, back to back, for at least 20 ms of GPU
+  // time (at most 256 of them) -- after a rebuild of the tables the GPU has idled, and a
+  // single warm-up launch left the next timings of a 2 ms step 20-30% slow.
+  auto warm_up = [&](const std::function<int()>& run) -> int {
+    float total = 0;
+    for (int w = 0; w < 256 && total < 20.0f; w++) {
+      (void)hipEventRecord(a, stream);
+      int rc = run();
+      if (rc) return rc;
+      (void)hipEventRecord(b, stream);
+      if (hipEventSynchronize(b) != hipSuccess) return set_error(GPU_PHS_E_HIP, "hipEventSynchronize");
+      float x = 0;
+      (void)hipEventElapsedTime(&x, a, b);
+      total += x;
+    }
+    return GPU_PHS_OK;
+  };
   auto time_now = [&](int cls, float* ms) -> int {
-    auto run = [&]() -> int {
+    std::function<int()> run = [&]() -> int {
       if (cls < 0) return plan_run(*pl, stream);
       hipError_t e = launch_class(*pl, cls, stream);
       return e == hipSuccess ? GPU_PHS_OK : hip_error(e, "forward launch");
     };
-    int rc = run();   // warm-up
+    int rc = warm_up(run);
     if (rc) return rc;
     std::vector<float> t;
     for (int r = 0; r < reps; r++) {
@@ -830,55 +848,88 @@ int gpu_phs_plan_autotune(int plan, void* stream_v, int reps, char* report, int 
       rc = plan_build(*pl);
     }
   }
-  // tile order, and whether two classes share the GPU or run one after the other, for the
-  // whole plan
+  // The whole-plan stages time their candidates in interleaved rounds (each candidate once per
+  // round, the median over the rounds), so that a clock still settling during a stage favours
+  // none of them; a later candidate has to beat the best earlier one by 2%.
+  auto stage = [&](int n, const std::function<int(int)>& apply, float* best_ms) -> int {
+    std::vector<std::vector<float>> t(n);
+    const int rounds = std::max(3, reps);
+    for (int r = 0; r < rounds && !rc; r++)
+      for (int i = 0; i < n && !rc; i++) {
+        rc = apply(i);
+        if (rc) break;
+        rc = warm_up([&]() { return plan_run(*pl, stream); });
+        if (rc) break;
+        (void)hipEventRecord(a, stream);
+        rc = plan_run(*pl, stream);
+        if (rc) break;
+        (void)hipEventRecord(b, stream);
+        if (hipEventSynchronize(b) != hipSuccess) { rc = set_error(GPU_PHS_E_HIP, "hipEventSynchronize"); break; }
+        float x = 0;
+        (void)hipEventElapsedTime(&x, a, b);
+        t[i].push_back(x);
+      }
+    if (rc) return 0;
+    int best = -1;
+    float bm = 1e30f;
+    for (int i = 0; i < n; i++) {
+      std::sort(t[i].begin(), t[i].end());
+      const float m = t[i][t[i].size() / 2];
+      if (m < 0.98f * bm) { best = i; bm = m; }
+    }
+    *best_ms = bm;
+    return best;
+  };
+  // tile order, and whether two classes share the GPU or run one after the other
   pl->n_streams = 1;
-  float best_order_ms = 1e30f;
-  int best_order = pl->order, best_serial = pl->serial_classes;
   const bool two = pl->class_end[0] > pl->class_begin[0] && pl->class_end[1] > pl->class_begin[1];
+  std::vector<std::pair<int, int>> scheds;
   for (int order : {0, 1})
-    for (int serial : {0, 1}) {
-      if (rc || (serial && !two)) continue;
-      pl->order = order;
-      pl->serial_classes = serial;
-      drop_graph(*pl);
-      rc = plan_build(*pl);
-      float ms = 0;
-      if (!rc) rc = time_now(-1, &ms);
-      if (!rc && ms < 0.98f * best_order_ms) { best_order_ms = ms; best_order = order; best_serial = serial; }
-    }
+    for (int serial : {0, 1})
+      if (!serial || two) scheds.push_back({order, serial});
+  float best_order_ms = 0;
+  int k = stage((int)scheds.size(), [&](int i) {
+    pl->serial_classes = scheds[i].second;
+    drop_graph(*pl);
+    if (pl->order == scheds[i].first) return (int)GPU_PHS_OK;
+    pl->order = scheds[i].first;
+    return plan_build(*pl);
+  }, &best_order_ms);
+  const int best_order = rc ? pl->order : scheds[k].first, best_serial = rc ? pl->serial_classes : scheds[k].second;
   // then the size below which a tree shape joins its class's generic launch
-  pl->order = best_order;
-  pl->serial_classes = best_serial;
+  if (!rc) {
+    pl->order = best_order;
+    pl->serial_classes = best_serial;
+    drop_graph(*pl);
+    rc = plan_build(*pl);
+  }
   int64_t best_merge = pl->merge_below;
-  float best_merge_ms = 1e30f;
-  if (pl->specialize)
-    for (int64_t mb : {kMergeBelowDefault, (int64_t)0, (int64_t)1 << 16, (int64_t)1 << 20}) {
-      if (rc) break;
-      pl->merge_below = mb;
+  float best_merge_ms = 0;
+  if (!rc && pl->specialize) {
+    const int64_t merges[] = {kMergeBelowDefault, 0, (int64_t)1 << 16, (int64_t)1 << 20};
+    k = stage(4, [&](int i) {
+      pl->merge_below = merges[i];
       drop_graph(*pl);
-      rc = plan_build(*pl);
-      float ms = 0;
-      if (!rc) rc = time_now(-1, &ms);
-      if (!rc && ms < 0.98f * best_merge_ms) { best_merge_ms = ms; best_merge = mb; }
-    }
+      return plan_build(*pl);
+    }, &best_merge_ms);
+    if (!rc) best_merge = merges[k];
+  }
   if (!rc) {
     pl->merge_below = best_merge;
     drop_graph(*pl);
     rc = plan_build(*pl);
   }
   // then over how many streams the launches spread (1: the classes as chosen above)
-  int best_streams = pl->n_streams;
-  float best_streams_ms = 1e30f;
-  if (!rc && pl->jobs.size() > 1)
-    for (int ns : {1, 2, 3, 4}) {
-      if (rc) break;
-      pl->n_streams = ns;
+  int best_streams = 1;
+  float best_streams_ms = 0;
+  if (!rc && pl->jobs.size() > 1) {
+    k = stage(kMaxStreams, [&](int i) {
+      pl->n_streams = i + 1;
       drop_graph(*pl);
-      float ms = 0;
-      rc = time_now(-1, &ms);
-      if (!rc && ms < 0.98f * best_streams_ms) { best_streams_ms = ms; best_streams = ns; }
-    }
+      return (int)GPU_PHS_OK;
+    }, &best_streams_ms);
+    if (!rc) best_streams = k + 1;
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) return rc;

@@ -176,6 +176,12 @@ int gpu_phs_plan_set_merge_below(int plan, int64_t events);
  * its slowest workgroups while the others' slots idle; launches on several streams fill those
  * slots.  Default 1.  No counterpart in the reference. */
 int gpu_phs_plan_set_streams(int plan, int n_streams);
+/* The device's streaming copy bandwidth in GB/s (bytes read plus bytes written per second,
+ * median of `reps` copies of `bytes` bytes with a kernel of one 16-byte word per thread and
+ * streaming stores): a practical ceiling for the memory-bound kernels, next to the published
+ * peak.  Allocates two buffers of `bytes` bytes for the call.  No counterpart in the
+ * reference. */
+int gpu_phs_copy_bandwidth(int64_t bytes, int reps, void* stream, double* gb_s);
 int gpu_phs_plan_get_streams(int plan, int* n_streams);
 int gpu_phs_plan_get_merge_below(int plan, int64_t* events);
 int gpu_phs_plan_get_schedule(int plan, int* order, int* serial);

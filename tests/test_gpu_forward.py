@@ -233,3 +233,17 @@ def test_autotune_tuning_round_trip():
         assert torch.equal(o0, b.ok)
         okm = o0.bool()
         assert torch.equal(p0[okm], b.p[okm]) and torch.equal(f0[okm], b.factor[okm])
+
+
+def test_copy_bandwidth_calibration():
+    """The streaming-copy ceiling the benchmark reports beside the HBM peak: a plausible rate,
+    and sizes outside the accepted range rejected."""
+    _torch()
+    import gpu_phs_amd as G
+    # 2 GiB each way: well beyond the device's 256 MB last-level cache (a 256 MiB copy, half of
+    # it served from that cache, measured above the 8 TB/s HBM peak)
+    gb_s = G.copy_bandwidth(2 << 30, 3)
+    print(f"streaming copy {gb_s:.0f} GB/s")
+    assert 1000 < gb_s < 8000
+    with pytest.raises(G.GpuPhsError):
+        G.copy_bandwidth(1024, 1)
