@@ -9,7 +9,11 @@ the inverse mapping and the Whizard entry point as extra fields, the HBM and FP6
 the forward kernels and the reference's own CPU algorithm timed on the host's cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3] [--no-extra]
-                    [--tuned '<json of Plan.tuning()>' | --config block,ept,grid,order,wpe,lds]
+                    [--tuned '<json of Plan.tuning()>' | --config block,ept,grid,order,wpe,lds | --autotune]
+
+The c3 mix runs the launch tuning C3_TUNING (the autotuner's choice, pinned so that the line and
+the committed counter profiles describe the same kernels); the autotuner's own result on the same
+mix is reported beside it.
 """
 from __future__ import annotations
 
@@ -311,6 +315,16 @@ def cpu_baseline(wl_name, W, threads, scale, budget_s):
                        f"reference phs_cpu.cpp built -O3")
 
 
+# The launch tuning of the c3 mix that the headline line runs and the committed profiles were
+# taken with: the autotuner's choice on MI355X, confirmed by interleaved repeats of the shapes
+# around it (class 0: 128-thread workgroups, 2 events per thread, a budget of 3 waves per SIMD;
+# class 1: 64 threads, 2 events per thread; the classes share the GPU).  --autotune tunes
+# instead, and the full run reports the autotuner's own result beside it (extra.c3_autotuned).
+C3_TUNING = {"classes": [{"block": 128, "ept": 2, "grid": 0, "waves_per_eu": 3, "lds": 0},
+                         {"block": 64, "ept": 2, "grid": 0, "waves_per_eu": 0, "lds": 0}],
+             "order": 0, "serial": 0, "merge_below": 262144, "streams": 1}
+
+
 def measured_counters(workload):
     """The committed counter profile of the forward kernels of one step of `workload`
     (profiles/forward_counters.json): HBM bytes per step from FETCH_SIZE and WRITE_SIZE (KiB per
@@ -341,7 +355,10 @@ def main():
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--tuned", default="",
                     help="apply this launch tuning (JSON of Plan.tuning(), as the bench line reports it in "
-                         "config.tuned) instead of autotuning")
+                         "config.tuned); c3 runs C3_TUNING unless this, --config or --autotune is given")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner choose the tuning of the workload (the default for workloads "
+                         "other than c3)")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-specialize", action="store_true",
                     help="run every tree on the generic forward kernel (no shape-specialised kernels)")
@@ -366,8 +383,9 @@ def main():
                          "classes shared / back to back, 3 interleaved repeats of --steps steps each, on top of "
                          "--tuned (or the autotuner's tuning), and print the times")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
-                    help="only run a plain device-to-device copy of this many GiB (3 times), to scale the "
-                         "FETCH_SIZE / WRITE_SIZE counters")
+                    help="only run a plain device-to-device copy of this many GiB (3 times with torch's copy, "
+                         "twice with the library's streaming-store copy), to scale the FETCH_SIZE / WRITE_SIZE "
+                         "counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (got --gpus {args.gpus}).",
@@ -387,6 +405,11 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
+        # the library's copy kernel (stream_copy), whose stores are streaming stores like the
+        # generator's: two dispatches of the same size
+        G.copy_bandwidth(n * 8, 1)
         print(json.dumps({"calibrate_copy_bytes": n * 8, "fetch_expected_kib": n * 8 / 1024,
                           "write_expected_kib": n * 8 / 1024}))
         return
@@ -410,6 +433,8 @@ def main():
         plan = G.Plan(segs)
         if args.tuned:
             plan.apply_tuning(json.loads(args.tuned))
+        elif args.workload == "c3" and not args.autotune:
+            plan.apply_tuning(C3_TUNING)
         else:
             plan.autotune(reps=3)
         torch.cuda.synchronize()
@@ -436,6 +461,8 @@ def main():
         v = [int(t) for t in args.config.split(",")]
         fixed = dict(zip(("block", "ept", "grid", "order", "waves_per_eu", "lds"), v))
     tuning = json.loads(args.tuned) if args.tuned else None
+    if tuning is None and fixed is None and args.workload == "c3" and not args.autotune:
+        tuning = C3_TUNING
     log(f"workload {args.workload}: setup")
     main_res, plan, segs, wl = run_workload(args.workload, args.steps, args.warmup, torch, G, W, fixed=fixed,
                                             tuning=tuning, specialize=not args.no_specialize)
@@ -459,15 +486,16 @@ def main():
             f"{extra['c4']['fwd_adjoint_over_forward']:.2f}x forward")
         torch.cuda.empty_cache()
         for name in ("c1", "c0", "c2", "c3"):
-            if name == args.workload:
+            if name == args.workload and tuning is None and fixed is None:
                 continue
             log(f"workload {name}: setup")
             r, p2, s2, _ = run_workload(name, max(3, args.steps // 2), 5, torch, G, W)
-            extra[name] = {k: r[k] for k in ("n_events", "batched_ev_s", "plain_ev_s", "batched_vs_plain",
-                                              "batched_kernel_ms", "tuning")}
-            extra[name]["tune_summary"] = r["tune"].split(" c0:")[0].split(" c1:")[0]
-            extra[name]["launch"] = r["launch"]
-            extra[name]["hbm_frac"] = r["algorithmic_bytes"] / (r["batched_kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
+            key = name if name != args.workload else f"{name}_autotuned"
+            extra[key] = {k: r[k] for k in ("n_events", "batched_ev_s", "plain_ev_s", "batched_vs_plain",
+                                             "batched_kernel_ms", "tuning")}
+            extra[key]["tune_summary"] = r["tune"].split(" c0:")[0].split(" c1:")[0]
+            extra[key]["launch"] = r["launch"]
+            extra[key]["hbm_frac"] = r["algorithmic_bytes"] / (r["batched_kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS
             log(f"workload {name}: {r['batched_ev_s']:.4g} ev/s batched, {r['plain_ev_s']:.4g} plain")
             del p2, s2
             torch.cuda.empty_cache()
