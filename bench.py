@@ -104,7 +104,10 @@ def run_workload(name, steps, warmup, torch, G, W, tune=True, fixed=None, tuning
 
 def run_adjoint(steps, warmup, torch, G, W, specialize=True):
     """c4: forward + reverse mode on the c1 trees, 1e6 events, loss = sum ok * factor * sum pT^2 / s.
-    One adjoint launch produces the forward outputs and d loss / d (x, M, Gamma)."""
+    One adjoint launch runs the forward pass and writes the per-event loss and d loss / d (x, M,
+    Gamma): 64 B read and 72 B written per event, the c4 configuration (randoms in, their
+    gradient and the loss out).  The same launch that also writes the forward outputs (momenta,
+    factor, volume, ok: 145 B more per event) is timed beside it."""
     wl = W.workload("c4")
     handles, segs = make_inputs(wl, torch, G)
     plan = G.Plan(segs)
@@ -115,11 +118,16 @@ def run_adjoint(steps, warmup, torch, G, W, specialize=True):
     loss = [torch.empty(b.x.shape[0], dtype=torch.float64, device="cuda") for _, _, b in segs]
     gparam = plan.adjoint(gx, loss=loss, loss_mode=1)
     fw_wall, fw_ev = time_steps(plan.run_graph, steps, warmup, torch)
-    ad_wall, ad_ev = time_steps(lambda: plan.adjoint(gx, loss=loss, loss_mode=1, gparam=gparam), steps, warmup, torch)
+    ad_wall, ad_ev = time_steps(lambda: plan.adjoint(gx, loss=loss, loss_mode=1, gparam=gparam, write_forward=False),
+                                steps, warmup, torch)
+    am_wall, am_ev = time_steps(lambda: plan.adjoint(gx, loss=loss, loss_mode=1, gparam=gparam, write_forward=True),
+                                steps, warmup, torch)
     n = wl.n_events
     return dict(n_events=n, forward_ev_s=n * steps / fw_wall, fwd_adjoint_ev_s=n * steps / ad_wall,
                 forward_ms=fw_ev / steps * 1e3, fwd_adjoint_ms=ad_ev / steps * 1e3,
                 fwd_adjoint_over_forward=ad_wall / fw_wall,
+                fwd_adjoint_with_outputs_ms=am_ev / steps * 1e3,
+                fwd_adjoint_with_outputs_over_forward=am_wall / fw_wall,
                 loss=float(sum(l.sum().item() for l in loss)))
 
 

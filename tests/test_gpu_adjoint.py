@@ -363,3 +363,75 @@ def test_shape_adjoint_identical_to_generic(n_out, loss_mode):
     for k in ("gx", "gparam"):
         scale = np.abs(b[k]).max(axis=-1, keepdims=True) if k == "gx" else np.abs(b[k]).max()
         assert np.all(np.abs(a[k] - b[k]) <= 1e-12 * scale), (k, np.abs(a[k] - b[k]).max())
+
+
+def test_adjoint_channel_without_events_gets_zero_gradients():
+    """A segment of no events makes a plan channel without tiles: its (M, Gamma) slots are set to
+    0 (every other slot is the sum over its channel's units, as in a plan without that segment),
+    and two calls in a row, the second on fewer events, give the gradients of their own inputs
+    (the reduction's counters are back at 0 after each call)."""
+    import torch
+    import gpu_phs_amd as G
+    proc = W.process_c1()
+    ph = G.ProcessHandle(proc)
+    rng = np.random.default_rng(5)
+    xs = [torch.from_numpy(0.02 + 0.96 * rng.random((n, proc.n_x))).cuda() for n in (3000, 0, 5000)]
+    bufs = [G.EventBuffers.allocate(x, proc.n_out) for x in xs]
+    plan = G.Plan([(ph, c, b) for c, b in zip((0, 1, 2), bufs)])
+    gx = [torch.empty_like(b.x) for b in bufs]
+    gparam = torch.full((plan.n_params(),), float("nan"), dtype=torch.float64, device="cuda")
+    print("adjoint with an empty channel", flush=True)
+    plan.adjoint(gx, loss_mode=1, gparam=gparam)
+    torch.cuda.synchronize()
+    pr, pch, pbc = plan.param_map()
+    g = gparam.cpu().numpy().reshape(-1, 2)
+    assert (g[pch == 1] == 0).all()
+    assert np.isfinite(g).all()
+    # the same channels without the empty segment
+    ref_plan = G.Plan([(ph, 0, bufs[0]), (ph, 2, bufs[2])])
+    gx2 = [torch.empty_like(bufs[0].x), torch.empty_like(bufs[2].x)]
+    gref = ref_plan.adjoint(gx2, loss_mode=1).cpu().numpy().reshape(-1, 2)
+    _, rch, rbc = ref_plan.param_map()
+    for c in (0, 2):
+        mine = {int(b): g[i] for i, (cc, b) in enumerate(zip(pch, pbc)) if cc == c}
+        theirs = {int(b): gref[i] for i, (cc, b) in enumerate(zip(rch, rbc)) if cc == c}
+        assert mine.keys() == theirs.keys()
+        for b in mine:
+            assert np.array_equal(mine[b], theirs[b]), (c, b)
+    # a second call on fewer events of the same plan layout: its own gradients
+    small = [torch.from_numpy(0.02 + 0.96 * rng.random((n, proc.n_x))).cuda() for n in (700, 0, 900)]
+    sb = [G.EventBuffers.allocate(x, proc.n_out) for x in small]
+    p2 = G.Plan([(ph, c, b) for c, b in zip((0, 1, 2), sb)])
+    sgx = [torch.empty_like(b.x) for b in sb]
+    first = p2.adjoint(sgx, loss_mode=1).clone()
+    again = p2.adjoint(sgx, loss_mode=1)
+    torch.cuda.synchronize()
+    assert torch.equal(first, again)
+
+
+def test_adjoint_reduction_over_many_units_per_channel():
+    """Channels of more than 8,192 64-event units reduce through 16 range sums per channel
+    (fewer units: one workgroup per channel): the (M, Gamma) gradients of 537,600 events per
+    channel equal the sum of those of its four quarters, each reduced in one range, to 1e-12."""
+    import torch
+    import gpu_phs_amd as G
+    proc = W.process_c1()
+    ph = G.ProcessHandle(proc)
+    rng = np.random.default_rng(8)
+    per = 8400 * 64
+    xs = [torch.from_numpy(0.02 + 0.96 * rng.random((per, proc.n_x))).cuda() for _ in range(proc.n_channels)]
+    bufs = [G.EventBuffers.allocate(x, proc.n_out) for x in xs]
+    plan = G.Plan([(ph, c, b) for c, b in enumerate(bufs)])
+    print("adjoint over 8,400 units per channel", flush=True)
+    full = plan.adjoint([torch.empty_like(b.x) for b in bufs], loss_mode=1).cpu().numpy()
+    q = per // 4
+    parts = 0
+    for k in range(4):
+        qb = [G.EventBuffers.allocate(x[k * q:(k + 1) * q], proc.n_out) for x in xs]
+        qp = G.Plan([(ph, c, b) for c, b in enumerate(qb)])
+        assert list(zip(*qp.param_map())) == list(zip(*plan.param_map()))
+        parts = parts + qp.adjoint([torch.empty_like(b.x) for b in qb], loss_mode=1).cpu().numpy()
+    torch.cuda.synchronize()
+    sc = np.abs(full).max()
+    print("max rel diff", np.abs(full - parts).max() / sc)
+    assert np.abs(full - parts).max() <= 1e-12 * sc
