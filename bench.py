@@ -390,6 +390,10 @@ def main():
                     help="with --forward-only: time the step for merge_below in {0, 2^14, 2^16, 2^18, 2^20} x "
                          "classes shared / back to back, 3 interleaved repeats of --steps steps each, on top of "
                          "--tuned (or the autotuner's tuning), and print the times")
+    ap.add_argument("--launch-modes", action="store_true",
+                    help="with --forward-only: time the step submitted directly, replayed from a hipGraph, with "
+                         "persistent grids of 2, 4 and 8 workgroups per CU (per 256 threads) in both classes, and "
+                         "as plain per-segment launches; 3 interleaved repeats of --steps steps each")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run a plain device-to-device copy of this many GiB (3 times with torch's copy, "
                          "twice with the library's streaming-store copy), to scale the FETCH_SIZE / WRITE_SIZE "
@@ -446,6 +450,26 @@ def main():
         else:
             plan.autotune(reps=3)
         torch.cuda.synchronize()
+        if args.launch_modes:
+            base, times = plan.tuning(), {}
+            n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+            modes = [("direct", None, "run"), ("graph", None, "run_graph"), ("plain", None, "run_plain")]
+            for k in (2, 4, 8):
+                modes.append((f"persistent {k}/CU direct", k, "run"))
+                modes.append((f"persistent {k}/CU graph", k, "run_graph"))
+            for _ in range(3):
+                for name, k, fn in modes:
+                    t = json.loads(json.dumps(base))
+                    if k:
+                        for lc in t["classes"]:
+                            lc["grid"] = n_cu * k * (256 // lc["block"])
+                    plan.apply_tuning(t)
+                    torch.cuda.synchronize()
+                    _, ev = time_steps(getattr(plan, fn), args.steps, args.warmup, torch)
+                    times.setdefault(name, []).append(ev / args.steps * 1e3)
+            print(json.dumps({"workload": args.workload, "base_tuning": base, "n_cu": n_cu,
+                              "ms_per_step": {k: sorted(v) for k, v in times.items()}}))
+            return
         if args.sweep_schedule:
             base, times = plan.tuning(), {}
             for _ in range(3):
