@@ -150,7 +150,8 @@ def run_inverse(name, steps, warmup, torch, G, W, scale=1.0, block=0):
     ms = ev_s / steps * 1e3
     return dict(n_events=n, channels=nc, kernel_ms=ms, events_s=n * steps / wall,
                 pairs_s=n * nc * steps / wall, bytes_per_launch=bytes_,
-                hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, block=block or 128)
+                hbm_frac=bytes_ / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                block=block or ("library default: 256 for up to 16 channels, else 128"))
 
 
 def run_event_beams(steps, torch, G, W):

@@ -115,10 +115,17 @@ int gphs::plan_inverse(PlanObj& pl, const int64_t* seg_row0, int64_t n_rows, dou
                        hipStream_t stream) {
   if (!seg_row0 || n_rows <= 0 || !factors || !x_out)
     return set_error(GPU_PHS_E_ARG, "inverse: need seg_row0, n_rows > 0, factors and x_out");
-  if (block == 0) block = 128;
-  if (block < 64 || block > 256 || block % 64) return set_error(GPU_PHS_E_ARG, "inverse: block must be 64, 128, 192 or 256");
+  if (block != 0 && (block < 64 || block > 256 || block % 64))
+    return set_error(GPU_PHS_E_ARG, "inverse: block must be 64, 128, 192 or 256");
   int rc = inverse_prepare(pl, seg_row0, n_rows);
   if (rc) return rc;
+  // the default workgroup: 256 threads for processes of a few channels (the 3 c1 channels: 1.72
+  // against 1.79 ms for 1e7 events at 128), 128 for many (the 120 c2 channels: 7.65 against
+  // 7.70-7.83 ms for 1e6 events at 256), each where its LDS tables fit
+  if (block == 0) {
+    block = pl.inv.n_ch <= 16 ? 256 : 128;
+    while (block > 64 && inverse_lds_bytes(pl.inv, block) > forward_lds_limit()) block -= 64;
+  }
   if (inverse_lds_bytes(pl.inv, block) > forward_lds_limit())
     return set_error(GPU_PHS_E_ARG, "inverse: the trees need more LDS than a workgroup of this size may use");
   hipError_t e = launch_inverse(pl.inv, pl.dp.tiles, pl.dp.n_tiles, pl.dp.segs, block, 0, factors, x_out, n_rows, stream);

@@ -265,7 +265,8 @@ int gpu_phs_generate_noalloc(int process, int channel, int64_t n_events, const d
  * seg_row0[s] is the output row of segment s's first event; outputs are device arrays
  * factors [n_rows][n_channels] and x_out [n_channels][n_rows][n_x] (channel-major, as
  * Whizard reads them, phs.cu:222, 1585).  The generating channel's slot gets the forward
- * factor and randoms.  block: threads per workgroup (64..256, 0 = 128).  Where the reference
+ * factor and randoms.  block: threads per workgroup (64..256; 0: 256 for processes of at most
+ * 16 channels, 128 for more, smaller where the LDS tables do not fit).  Where the reference
  * has defects (the STEP_H inverse, the T/U inverse of the CPU version, the friend rotation)
  * the inverse of the generating channel returns its randoms instead. */
 int gpu_phs_plan_inverse(int plan, const int64_t* seg_row0, int64_t n_rows, double* factors, double* x_out,

@@ -68,7 +68,7 @@ PROCS = {"c1": W.process_c1, "c2": W.process_c2, "2to6": W.process_2to6,
 
 
 @pytest.mark.parametrize("name", sorted(PROCS))
-@pytest.mark.parametrize("block", [0, 64])
+@pytest.mark.parametrize("block", [0, 64, 128])
 def test_inverse_matches_oracle(name, block):
     proc = PROCS[name]()
     rng = np.random.default_rng(17)
