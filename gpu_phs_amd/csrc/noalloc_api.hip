@@ -35,8 +35,8 @@ __device__ __align__(256) unsigned char scratch[kScratchBytes];
 constexpr int kMaxDevices = 64;
 void* g_scratch[kMaxDevices] = {};
 
-// Launch shapes: the small trees as the autotuner settles them on the batched mix (256 threads,
-// 4 events per thread), the others at 64 threads.
+// Fixed launch shapes (there is no autotuning here): the trees of up to four final-state
+// particles at 256 threads and 4 events per thread, the others at 64 threads.
 constexpr int kBlockSmall = 256, kBlockLarge = 64, kEpt = 4;
 
 // A chunk of the records, by value in the kernel arguments (well under the 4 KiB limit).
