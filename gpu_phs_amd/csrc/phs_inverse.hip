@@ -32,6 +32,14 @@ namespace gphs {
 namespace {
 
 constexpr int kInvBankWords = 19;
+
+// The lane's column of the sub-tile's randoms table x[k][event] (rows of `stride` doubles).
+struct XColumn {
+  double* p;
+  int stride;
+  __device__ __forceinline__ double& operator[](int k) const { return p[k * stride]; }
+};
+
 // With RB (every deferred subtree is a daughter of the root, and no root has a friend) a bank
 // entry is the root's cos theta, sin theta, cos phi, sin phi and the subtree's msq, m and
 // p_decay: the root's G is its rotation applied to the identity (the root's bg is 0), so the
@@ -286,11 +294,11 @@ __device__ __forceinline__ void static_for(F&& f) {
 // `g`: the randoms go to xo[0..n_x), returns the factor.  SID >= 0: the channel's program has
 // shape SID of the compile-time table and every branch's structure is a constant; SID < 0: it
 // is read from the program.  No branch has a friend.
-template <int SID, bool PEV>
+template <int SID, bool PEV, class XO>
 __device__ __forceinline__ double inverse_channel(const ChannelDesc& cd, const EvBeams& eb,
                                                   const NodeCmd* __restrict__ prog, const RootGeo& g,
                                                   const double* __restrict__ pl, int S, double* bank, int B,
-                                                  gdouble* __restrict__ xo) {
+                                                  XO xo) {
   using W = Walk<SID>;
   const double s = eb.s;
   double factor = 1;
@@ -600,8 +608,9 @@ __global__ __launch_bounds__(256) void inverse_kernel(const TileDesc* __restrict
 // which only the GEN instances contain (it needs some thirty more registers than the
 // specialised walks).  WPE: waves per SIMD the register allocation aims at (0: the compiler's
 // choice).
-// LDS per workgroup: the momenta column (4 n_out rows of blockDim.x + 1 doubles), then the
-// save banks of subtrees deferred below the root (19 rows of blockDim.x doubles per level).
+// LDS per workgroup: the momenta column (4 n_out rows of blockDim.x + 1 doubles), the save
+// banks of subtrees deferred below the root (19 rows of blockDim.x doubles per level), then the
+// randoms table of the channel being inverted (n_x rows of blockDim.x + 1 doubles).
 template <bool PEV, bool GEN, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void inverse_kernel_shared(const TileDesc* __restrict__ tiles, int n_tiles,
                                                              const SegDesc* __restrict__ segs,
@@ -609,7 +618,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
                                                              const InvStep* __restrict__ steps,
                                                              const ChannelDesc* __restrict__ chans,
                                                              const NodeCmd* __restrict__ nodes, int n_ch, int n_out,
-                                                             int n_x, int64_t n_rows, double* __restrict__ factors,
+                                                             int n_x, int64_t n_rows, int rest_banks,
+                                                             double* __restrict__ factors,
                                                              double* __restrict__ x_out) {
   const int B = (int)blockDim.x;
   const int S = B + 1;
@@ -617,6 +627,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   double* ps = inv_lds;
   double* bank = ps + 4 * n_out * S + lane;
   double* pl = ps + lane;
+  // each channel's randoms of the sub-tile go through the table xs[k][event], then to x_out as
+  // one contiguous run of rows (whole lines, streaming stores) instead of 8-byte pieces 8 n_x
+  // bytes apart
+  double* xs = ps + 4 * n_out * S + kInvBankWords * rest_banks * B;
+  const XColumn xl{xs + lane, S};
+  const int q_step_r = B / n_x, q_step_k = B % n_x;
   for (int t = (int)blockIdx.x; t < n_tiles; t += (int)gridDim.x) {
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
@@ -646,32 +662,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         const ChannelDesc cd = chans[c];
         const NodeCmd* prog = nodes + cd.node0;
         if (st.new_group) geo = root_geometry(prog[0], eb, pl, S, n_out);
-        gdouble* xo = as_global(x_out) + ((int64_t)c * n_rows + row) * n_x;
         if (c == is.own) {
           if (valid) {
             const gdouble* xi = as_global(sd.x) + e * n_x;
-            for (int q = 0; q < n_x; q++) xo[q] = xi[q];
+            for (int q = 0; q < n_x; q++) xl[q] = xi[q];
             as_global(factors)[row * n_ch + c] = as_global((const double*)sd.factor)[e];
           }
-          continue;
+        } else if (valid) {
+          double f;
+          switch (cd.shape) {
+            case 0: f = inverse_channel<0, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 1: f = inverse_channel<1, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 2: f = inverse_channel<2, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 3: f = inverse_channel<3, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 4: f = inverse_channel<4, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 5: f = inverse_channel<5, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 6: f = inverse_channel<6, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            case 7: f = inverse_channel<7, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl); break;
+            default:
+              if constexpr (GEN) f = inverse_channel<-1, PEV>(cd, eb, prog, geo, pl, S, bank, B, xl);
+              else f = 0;   // not reached: the host runs channels of other programs with GEN
+              break;
+          }
+          as_global(factors)[row * n_ch + c] = f;
         }
-        if (!valid) continue;
-        double f;
-        switch (cd.shape) {
-          case 0: f = inverse_channel<0, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 1: f = inverse_channel<1, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 2: f = inverse_channel<2, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 3: f = inverse_channel<3, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 4: f = inverse_channel<4, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 5: f = inverse_channel<5, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 6: f = inverse_channel<6, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          case 7: f = inverse_channel<7, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo); break;
-          default:
-            if constexpr (GEN) f = inverse_channel<-1, PEV>(cd, eb, prog, geo, pl, S, bank, B, xo);
-            else f = 0;   // not reached: the host runs channels of other programs with GEN
-            break;
+        // the sub-tile's rows of channel c, x_out[c][row0 + e0 + base ...][n_x]: element q of
+        // the run is row r = q / n_x, random k = q % n_x of the table
+        __syncthreads();
+        gdouble* dst = as_global(x_out) + ((int64_t)c * n_rows + is.row0 + td.e0 + base) * n_x;
+        int r = lane / n_x, kx = lane % n_x;
+        for (int q = lane; q < cnt * n_x; q += B) {
+          __builtin_nontemporal_store(xs[kx * S + r], dst + q);
+          r += q_step_r;
+          kx += q_step_k;
+          if (kx >= n_x) { kx -= n_x; r++; }
         }
-        as_global(factors)[row * n_ch + c] = f;
+        __syncthreads();
       }
     }
   }
@@ -681,7 +707,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 
 size_t inverse_lds_bytes(const InverseTables& it, int block) {
   if (it.shared_roots)
-    return ((size_t)4 * it.n_out * (block + 1) + (size_t)kInvBankWords * it.rest_banks * block) * sizeof(double);
+    return ((size_t)4 * it.n_out * (block + 1) + (size_t)kInvBankWords * it.rest_banks * block +
+            (size_t)it.n_x * (block + 1)) * sizeof(double);
   const size_t words = it.root_bank ? kInvRootBankWords : kInvBankWords;
   return ((size_t)4 * it.n_out * (block + 1) + words * it.n_banks * block) * sizeof(double);
 }
@@ -697,7 +724,7 @@ hipError_t launch_inverse(const InverseTables& it, const TileDesc* tiles, int n_
     auto k = it.per_event_beams ? (it.generic ? inverse_kernel_shared<true, true, 0> : inverse_kernel_shared<true, false, 3>)
                                 : (it.generic ? inverse_kernel_shared<false, true, 0> : inverse_kernel_shared<false, false, 3>);
     hipLaunchKernelGGL(k, dim3(g), dim3(block), shmem, stream, tiles, n_tiles, segs, it.isegs, it.steps, it.chans,
-                       it.nodes, it.n_ch, it.n_out, it.n_x, n_rows, factors, x_out);
+                       it.nodes, it.n_ch, it.n_out, it.n_x, n_rows, it.rest_banks, factors, x_out);
     return hipGetLastError();
   }
   using K = decltype(&inverse_kernel<false, false, false>);
