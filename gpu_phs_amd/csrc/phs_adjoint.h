@@ -495,9 +495,13 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
     for (int i = 0; i < N; i++) fwd_branch(i);
   }
   *f_out = F;
-  for (int k = 0; k < cd.n_x; k++) gx[k] = 0;
   if (loss_out) *loss_out = ok && loss_mode == 1 ? F * S2 / s : 0.0;
-  if (!ok) return false;
+  if (!ok) {
+    for (int k = 0; k < cd.n_x; k++) gx[k] = 0;
+    return false;
+  }
+  // (an ok event's sweep below writes every entry of gx: one per internal daughter's mass and
+  // two per branch's angles are all of the channel's randoms)
   {
   // derivatives only from here on: products and sums may fuse (the outputs above are final)
 #pragma clang fp contract(fast)
