@@ -380,6 +380,8 @@ def main():
                     help="only run the all-channel inverse mapping of c1 and c2 and print its figures")
     ap.add_argument("--grid-only", action="store_true",
                     help="only run the adaptive-grid generation and update on c1 and print the figures")
+    ap.add_argument("--whizard-only", action="store_true",
+                    help="only run the Whizard entry point end to end on 1e6 c1 events and print the figures")
     ap.add_argument("--beams-only", action="store_true",
                     help="only run c1 with per-event beam energies and print the figures")
     ap.add_argument("--forward-only", action="store_true",
@@ -431,6 +433,9 @@ def main():
         return
     if args.grid_only:
         print(json.dumps(run_grid(max(3, args.steps), torch, G, W)))
+        return
+    if args.whizard_only:
+        print(json.dumps(run_whizard_abi(1_000_000, max(3, args.steps // 4), G, W)))
         return
     if args.beams_only:
         print(json.dumps(run_event_beams(max(3, args.steps), torch, G, W)))
