@@ -239,3 +239,37 @@ def test_whizard_abi_repeated_calls_reuse_and_resplit():
     for a, b in zip(outs[0][1:], outs[1][1:]):
         assert np.array_equal(a, b, equal_nan=True)
     L.c_whizard_finalize_gpu_phs()
+
+
+def test_whizard_abi_large_call_identical_to_plan():
+    """A call whose arrays span several staging chunks (16 MiB, each copied on the host by
+    several threads), with a partial last chunk: every output equals, bit for bit, the plan API's
+    forward pass and inverse mapping of the same events on the device."""
+    import torch
+    proc = W.process_c1()
+    L = setup_module_from(proc)
+    counts = [100_003, 100_003, 100_003]
+    n = sum(counts)
+    rng = np.random.default_rng(33)
+    x = rng.random((n, proc.n_x))
+    print("large Whizard call", n, flush=True)
+    p, factors, volumes, oks, x_out = gen(L, proc, x, counts, 250.0, 250.0)
+    L.c_whizard_finalize_gpu_phs()
+    ph = G.ProcessHandle(proc)
+    xt = torch.from_numpy(x).cuda()
+    bufs, segs, start = [], [], 0
+    for c, k in enumerate(counts):
+        b = G.EventBuffers.allocate(xt[start:start + k], proc.n_out)
+        bufs.append(b)
+        segs.append((ph, c, b))
+        start += k
+    plan = G.Plan(segs)
+    plan.run()
+    f_plan, x_plan = plan.inverse()
+    torch.cuda.synchronize()
+    cat = lambda k: torch.cat([getattr(b, k) for b in bufs]).cpu().numpy()
+    assert np.array_equal(oks, cat("ok").astype(bool))
+    assert np.array_equal(p, cat("p"), equal_nan=True)
+    assert np.array_equal(volumes, cat("volume"))
+    assert np.array_equal(factors, f_plan.cpu().numpy(), equal_nan=True)
+    assert np.array_equal(x_out, x_plan.cpu().numpy(), equal_nan=True)
