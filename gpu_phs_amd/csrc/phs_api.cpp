@@ -263,14 +263,17 @@ int plan_build(PlanObj& pl) {
   }
   for (const SegDesc& sd : pl.segs)
     if (sd.e1) pl.dp.per_event_beams = pl.dps[pl.chan_class[sd.channel]].per_event_beams = 1;
-  // each class by shape, when its kernels may be specialised (fixed beams)
+  // each class by shape (with per-event beams, only the trees of up to four final-state
+  // particles have specialised kernels; the others run the generic one)
   for (int c = 0; c < kTileClasses; c++) {
     std::vector<ShapeRun>& runs = pl.shape_runs[c];
     runs.clear();
-    if (!pl.specialize || pl.dps[c].per_event_beams) continue;
+    if (!pl.specialize) continue;
+    const bool pev = pl.dps[c].per_event_beams != 0;
     for (int t = pl.class_begin[c]; t < pl.class_end[c]; t++) {
       int sh = pl.chans[pl.segs[pl.tiles[t].seg].channel].shape;
       if (sh >= 0 && shape_events[(size_t)c * kNumShapes + sh] < pl.merge_below) sh = -1;
+      if (pev && sh >= kPevShapes) sh = -1;
       if (runs.empty() || runs.back().sid != sh) runs.push_back(ShapeRun{sh, t, t + 1});
       else runs.back().end = t + 1;
     }

@@ -391,6 +391,8 @@ FwdKernel shape_kernel() {
 // The shape kernels of shapes [lo, hi) (one translation unit each): the variant for register
 // budget `wpe` and workgroup size `block`, or null for a shape outside the range.
 FwdKernel pick_shape_0_8(int sid, int wpe, int block);
+// per-event beam energies: shapes [0, kPevShapes), the trees of up to four final-state particles
+FwdKernel pick_shape_pev_0_8(int sid, int wpe, int block);
 FwdKernel pick_shape_8_22(int sid, int wpe, int block);
 FwdKernel pick_shape_22_36(int sid, int wpe, int block);
 FwdKernel pick_shape_36_50(int sid, int wpe, int block);

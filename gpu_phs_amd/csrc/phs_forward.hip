@@ -84,8 +84,12 @@ hipError_t launch_forward_shape(const DevicePlan& dp, int sid, int t_begin, int 
   const int n = t_end - t_begin;
   const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
   FwdKernel k = nullptr;
-  for (auto pick : {pick_shape_0_8, pick_shape_8_22, pick_shape_22_36, pick_shape_36_50, pick_shape_50_64})
-    if (!k) k = pick(sid, lc.waves_per_eu, lc.block);
+  if (dp.per_event_beams) {
+    if (sid < kPevShapes) k = pick_shape_pev_0_8(sid, lc.waves_per_eu, lc.block);
+  } else {
+    for (auto pick : {pick_shape_0_8, pick_shape_8_22, pick_shape_22_36, pick_shape_36_50, pick_shape_50_64})
+      if (!k) k = pick(sid, lc.waves_per_eu, lc.block);
+  }
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,
                      dp.nodes, (int)kShapes[sid].n_x, n_out);

@@ -39,7 +39,10 @@ size_t forward_lds_limit();
 // Forward pass over tiles [t_begin, t_end).
 hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const LaunchConfig& lc, hipStream_t stream);
 // The same for tiles whose channels all have tree shape `sid` (phs_shapes.h), with the kernel
-// specialised for it (fixed beams, no friends; lc.lds is not used: randoms in registers).
+// specialised for it (no friends; lc.lds is not used: randoms in registers).  With per-event
+// beams (dp.per_event_beams) only shapes [0, kPevShapes) -- the trees of up to four final-state
+// particles -- have such kernels.
+constexpr int kPevShapes = 8;
 hipError_t launch_forward_shape(const DevicePlan& dp, int sid, int t_begin, int t_end, const LaunchConfig& lc,
                                 hipStream_t stream);
 

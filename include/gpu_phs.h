@@ -199,9 +199,9 @@ int gpu_phs_plan_set_event_beams(int plan, int segment, const double* E1, const 
  * NULL (the default) turns it off; it costs no memory traffic then.  The array must outlive
  * the plan's use. */
 int gpu_phs_plan_set_msq_output(int plan, int segment, double* msq);
-/* 1 (the default): channels whose trees have at most four final-state particles (and no
- * friends, fixed beams) run forward kernels specialised for their tree shape; 0: every channel
- * runs the generic kernel.  Both give identical results. */
+/* 1 (the default): channels without friends run forward kernels specialised for their tree
+ * shape (every ordered tree of 2-6 final-state particles; with per-event beam energies, those of
+ * 2-4); 0: every channel runs the generic kernel.  Both give identical results. */
 int gpu_phs_plan_set_specialize(int plan, int on);
 /* Forward pass of every segment in one kernel launch on `stream` (hipStream_t, 0 = null). */
 int gpu_phs_plan_run(int plan, void* stream);

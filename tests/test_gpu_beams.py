@@ -19,7 +19,9 @@ def _beams(proc, n, rng):
     return h * rng.uniform(0.8, 1.3, n), h * rng.uniform(0.8, 1.3, n)
 
 
-def run_beams(proc, x, ch, E1, E2, **cfg):
+def run_beams(proc, x, ch, E1, E2, specialize=True, **cfg):
+    """specialize: the trees of up to four final-state particles on their shape's kernel (every
+    shape its own launch, however few its events), else every tree on the generic kernel."""
     import torch
     import gpu_phs_amd as G
     ph = G.ProcessHandle(proc)
@@ -40,6 +42,9 @@ def run_beams(proc, x, ch, E1, E2, **cfg):
         beams.append((e1s[start:start + k], e2s[start:start + k]))
         start += k
     plan = G.Plan(segs, **cfg)
+    plan.set_merge_below(0)
+    if not specialize:
+        plan.set_specialize(False)
     for i, (a, b) in enumerate(beams):
         plan.set_event_beams(i, a, b)
     plan.run()
@@ -103,3 +108,23 @@ def test_per_event_beams_built_in_loss_is_rejected():
     _, plan, bufs, _ = run_beams(proc, rng.random((n, proc.n_x)), ch, E1, E2)
     with pytest.raises(G.GpuPhsError, match="per-event beam"):
         plan.adjoint([torch.zeros_like(b.x) for b in bufs], loss_mode=1)
+
+
+@pytest.mark.parametrize("n_out", [2, 3, 4])
+@pytest.mark.parametrize("cfg", [dict(), dict(block=64, ept=3, waves_per_eu=3), dict(block=128, ept=2, waves_per_eu=5)])
+def test_per_event_beams_shape_kernels_identical_to_generic(n_out, cfg):
+    """With per-event beams, the kernels specialised for the trees of 2-4 final-state particles
+    give the generic kernel's bits: momenta, factors, volumes and ok flags of every event."""
+    proc = W.process_shapes(n_out)
+    rng = np.random.default_rng(40 + n_out)
+    n = 20011
+    ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
+    x = rng.random((n, proc.n_x))
+    E1, E2 = _beams(proc, n, rng)
+    print("per-event beams shapes", n_out, cfg, flush=True)
+    spec, _, _, _ = run_beams(proc, x, ch, E1, E2, **cfg)
+    gen, _, _, _ = run_beams(proc, x, ch, E1, E2, specialize=False, **cfg)
+    for k in ("p", "factor", "volume", "ok"):
+        assert np.array_equal(spec[k], gen[k], equal_nan=k != "ok"), (k, cfg)
+    assert spec["ok"].sum() > 1000
+    assert_parity(forward_check(proc, x, ch, spec, E1=E1, E2=E2))
