@@ -107,7 +107,8 @@ def run_adjoint(steps, warmup, torch, G, W, specialize=True):
     One adjoint launch runs the forward pass and writes the per-event loss and d loss / d (x, M,
     Gamma): 64 B read and 72 B written per event, the c4 configuration (randoms in, their
     gradient and the loss out).  The same launch that also writes the forward outputs (momenta,
-    factor, volume, ok: 145 B more per event) is timed beside it."""
+    factor, volume, ok: 145 B more per event) is timed beside it, and so is loss mode 0 (the
+    cotangents of momenta and factor given), with the fixed beams and with per-event beams."""
     wl = W.workload("c4")
     handles, segs = make_inputs(wl, torch, G)
     plan = G.Plan(segs)
@@ -122,12 +123,30 @@ def run_adjoint(steps, warmup, torch, G, W, specialize=True):
                                 steps, warmup, torch)
     am_wall, am_ev = time_steps(lambda: plan.adjoint(gx, loss=loss, loss_mode=1, gparam=gparam, write_forward=True),
                                 steps, warmup, torch)
+    # loss mode 0: seeded cotangents of the momenta and factors in (another 136 B read per
+    # event), first with the fixed beams, then with beam energies that differ per event
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+    gp = [torch.rand(b.p.shape, dtype=torch.float64, device="cuda", generator=gen) for _, _, b in segs]
+    gf = [torch.rand(b.factor.shape, dtype=torch.float64, device="cuda", generator=gen) for _, _, b in segs]
+    mode0 = lambda: plan.adjoint(gx, g_p=gp, g_f=gf, loss_mode=0, gparam=gparam, write_forward=False)
+    m0_wall, m0_ev = time_steps(mode0, steps, warmup, torch)
+    for i, (_, _, b) in enumerate(segs):
+        k = b.x.shape[0]
+        plan.set_event_beams(i, 250.0 * (0.8 + 0.5 * torch.rand(k, dtype=torch.float64, device="cuda", generator=gen)),
+                             250.0 * (0.8 + 0.5 * torch.rand(k, dtype=torch.float64, device="cuda", generator=gen)))
+    fb_wall, fb_ev = time_steps(plan.run_graph, steps, warmup, torch)
+    pb_wall, pb_ev = time_steps(mode0, steps, warmup, torch)
     n = wl.n_events
     return dict(n_events=n, forward_ev_s=n * steps / fw_wall, fwd_adjoint_ev_s=n * steps / ad_wall,
                 forward_ms=fw_ev / steps * 1e3, fwd_adjoint_ms=ad_ev / steps * 1e3,
                 fwd_adjoint_over_forward=ad_wall / fw_wall,
                 fwd_adjoint_with_outputs_ms=am_ev / steps * 1e3,
                 fwd_adjoint_with_outputs_over_forward=am_wall / fw_wall,
+                loss_mode0_ms=m0_ev / steps * 1e3, loss_mode0_over_forward=m0_wall / fw_wall,
+                event_beams_forward_ms=fb_ev / steps * 1e3,
+                event_beams_loss_mode0_ms=pb_ev / steps * 1e3,
+                event_beams_loss_mode0_over_forward=pb_wall / fb_wall,
                 loss=float(sum(l.sum().item() for l in loss)))
 
 

@@ -86,11 +86,13 @@ GPHS_HD MapEval msq_eval(const ChildCmd& c, double x, double s) {
   return r;
 }
 
-// Adjoint of msq_eval: given gmsq, gf, accumulate d/dx and d/d(M, Gamma) of the map.
-GPHS_HD void msq_eval_adj(const ChildCmd& c, double x, double s, const MapEval& r, double gmsq, double gf, double& gx,
-                          double& gM, double& gW) {
+// Adjoint of msq_eval: given gmsq, gf, d/dx, the adjoints of the map constants a[0..2] and the
+// direct d/d(M, Gamma) of the map (through its own m and w, not through the constants).
+GPHS_HD void msq_eval_adj_core(const ChildCmd& c, double x, double s, const MapEval& r, double gmsq, double gf,
+                               double& gx, double& ga0, double& ga1, double& ga2, double& gm_direct,
+                               double& gw_direct) {
 #pragma clang fp contract(fast)
-  double ga0 = 0, ga1 = 0, ga2 = 0, gm_direct = 0, gw_direct = 0;
+  ga0 = ga1 = ga2 = gm_direct = gw_direct = 0;
   gx = 0;
   switch (c.kind) {
     case MK_LINEAR:
@@ -151,6 +153,14 @@ GPHS_HD void msq_eval_adj(const ChildCmd& c, double x, double s, const MapEval& 
       break;
     }
   }
+}
+
+// Adjoint of msq_eval: given gmsq, gf, accumulate d/dx and d/d(M, Gamma) of the map.
+GPHS_HD void msq_eval_adj(const ChildCmd& c, double x, double s, const MapEval& r, double gmsq, double gf, double& gx,
+                          double& gM, double& gW) {
+#pragma clang fp contract(fast)
+  double ga0, ga1, ga2, gm_direct, gw_direct;
+  msq_eval_adj_core(c, x, s, r, gmsq, gf, gx, ga0, ga1, ga2, gm_direct, gw_direct);
   gM += gm_direct + ga0 * c.da_dm[0] + ga1 * c.da_dm[1] + ga2 * c.da_dm[2];
   gW += gw_direct + ga0 * c.da_dw[0] + ga1 * c.da_dw[1] + ga2 * c.da_dw[2];
 }
@@ -396,6 +406,27 @@ GPHS_HD void friend_adj(const double (&P)[16], const double (&G)[16], bool has_p
 //   FR: branches with a friend (nc.friend_sign != 0) take their polar axis from a beam
 //   (phs_device.h compose_friend); their L1 depends on their parent boost, whose adjoint then
 //   has a second term, through the beam's direction.
+// A copy of v that the compiler cannot prove equal to v.  With per-event beams the maps' and
+// cos-theta constants depend on the event's sqrt(s); recomputed from such a copy they are
+// evaluated again at each use, instead of being kept in registers from the forward pass to
+// the end of the reverse sweep (which spilled them to scratch).
+GPHS_HD double opaque(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v));
+#endif
+  return v;
+}
+// The same, and not available before `after` has been computed: what is derived from it is
+// not scheduled earlier either.
+GPHS_HD double opaque_after(double v, double after) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v) : "v"(after));
+#else
+  (void)after;
+#endif
+  return v;
+}
+
 template <int SID, int LM, bool PEV, bool FR, class Store, class GPar, class XR>
 GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeCmd* prog, const XR& xr,
                            const double* gp, double gf_in, double* p_out, double* f_out, double* loss_out, double* gx,
@@ -409,14 +440,22 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
     if constexpr (!PEV) {
       return nc.k[c];
     } else {
-      ChildCmd r = child_at(nc.k[c], eb.sqrts, s, cd.m_tot);
+      const double rs = opaque(eb.sqrts);
+      ChildCmd r = child_at(nc.k[c], rs, opaque(s), cd.m_tot);
       mc::DualMW a[3], b[3];
-      mc::map_constants_d(prog[Wk::child(nc, i, c)].map_id, nc.k[c].m, nc.k[c].w, nc.k[c].m_min, cd.m_tot, eb.sqrts,
-                          a, b);
+      mc::map_constants_d(prog[Wk::child(nc, i, c)].map_id, nc.k[c].m, nc.k[c].w, nc.k[c].m_min, cd.m_tot, rs, a, b);
 #pragma unroll
       for (int k = 0; k < 3; k++) { r.da_dm[k] = a[k].dm; r.da_dw[k] = a[k].dw; }
       return r;
     }
+  };
+  // the event's beams through opaque(): what node_at derives from them is recomputed per use
+  auto beams_here = [&]() {
+    EvBeams b = eb;
+    b.s = opaque(eb.s);
+    b.sqrts = opaque(eb.sqrts);
+    b.s_friend = opaque(eb.s_friend);
+    return b;
   };
   const int N = Wk::n_nodes(cd);
   // ---------------- forward (phs_forward.hip, plus the tape)
@@ -435,13 +474,13 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; }
       else {
         MapEval r;
-        if constexpr (PEV) r = msq_eval(child_at(nc.k[c], eb.sqrts, s, cd.m_tot), xr[Wk::xi(nc, i, c)], s);
+        if constexpr (PEV) r = msq_eval(child_at(nc.k[c], opaque(eb.sqrts), opaque(s), cd.m_tot), xr[Wk::xi(nc, i, c)], s);
         else r = msq_eval(nc.k[c], xr[Wk::xi(nc, i, c)], s);   // no copy of the record
         msq[c] = r.msq; f[c] = r.f; mk[c] = math::sqrt_phys(r.msq);
       }
     }
     double m_max = nc.m_max, cb0 = nc.b0, cb1 = nc.b1;
-    if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max, cb0, cb1);
+    if constexpr (PEV) node_at(nc, i == 0, beams_here(), cd.m_tot, m_max, cb0, cb1);
     const double m = cur_m;
     const double lda = (cur_msq - msq[0] - msq[1]) * (cur_msq - msq[0] - msq[1]) - 4 * msq[0] * msq[1];
     ok = ok && (lda > 0 && m > mk[0] + mk[1] && m <= m_max);
@@ -516,7 +555,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
   auto rev_branch = [&](const int i) {
     const NodeCmd& nc = prog[i];
     double m_max_ev, cb0 = nc.b0, cb1 = nc.b1;
-    if constexpr (PEV) node_at(nc, i == 0, eb, cd.m_tot, m_max_ev, cb0, cb1);
+    if constexpr (PEV) node_at(nc, i == 0, beams_here(), cd.m_tot, m_max_ev, cb0, cb1);
     const bool leaf[2] = {Wk::leaf(nc, i, 0), Wk::leaf(nc, i, 1)};
     const int push = Wk::push(nc, i), desc = Wk::desc(nc, i);
     double G[16] = {0};
@@ -545,7 +584,7 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       if (leaf[c]) { msq[c] = nc.k[c].msq_leaf; f[c] = 1; mk[c] = nc.k[c].m_leaf; xk[c] = 0; }
       else {
         xk[c] = xr[Wk::xi(nc, i, c)];
-        if constexpr (PEV) r[c] = msq_eval(child_at(nc.k[c], eb.sqrts, s, cd.m_tot), xk[c], s);
+        if constexpr (PEV) r[c] = msq_eval(child_at(nc.k[c], opaque(eb.sqrts), opaque(s), cd.m_tot), xk[c], s);
         else r[c] = msq_eval(nc.k[c], xk[c], s);
         msq[c] = r[c].msq; f[c] = r[c].f; mk[c] = math::sqrt_phys(msq[c]);
       }
@@ -674,10 +713,14 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
     ct_eval_adj(nc.ct_kind, xct, cb0, cb1, ce, gct, gst, gfct, gxc, gb0, gb1);
     gx[Wk::xphi(nc, i) + 1] = gxc;
     if constexpr (PEV) {   // the cos-theta constants at this event's s: b0 = M^2 / s, b1 = log((b0 + 1) / b0)
-      mc::DualMW a[3], b[3];
-      mc::map_constants_d(nc.map_id, nc.m, 0.0, nc.m_min, cd.m_tot, eb.sqrts, a, b,
-                          nc.friend_sign != 0 ? eb.s_friend : 0.0);
-      gpar[2 * i] += gb0 * b[0].dm + gb1 * b[1].dm;
+      double db0 = 0, db1 = 0;
+      if (nc.ct_kind == CK_PEAKED) {
+        mc::DualMW b[2];
+        mc::ct_constants_d(nc.m, nc.friend_sign != 0 ? opaque(eb.s_friend) : opaque(eb.s), b);
+        db0 = b[0].dm;
+        db1 = b[1].dm;
+      }
+      gpar[2 * i] += gb0 * db0 + gb1 * db1;
     } else {
       gpar[2 * i] += gb0 * nc.db_dm[0] + gb1 * nc.db_dm[1];
     }
@@ -710,8 +753,19 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
       if (leaf[c]) continue;
       if (mk[c] > 0) g_msq[c] += math::quot(g_m[c], 2 * mk[c]);
       double gxk = 0, gM = 0, gW = 0;
-      if constexpr (PEV) msq_eval_adj(child(nc, i, c), xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
-      else msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      if constexpr (PEV && SID >= 0) {
+        // the derivatives of the map constants at this event's sqrt(s) wait for the end of
+        // the sweep, where fewer registers are live; the adjoints of the constants wait in
+        // this branch's tape entries, which nothing reads after this point
+        double ga[3];
+        msq_eval_adj_core(child_at(nc.k[c], opaque(eb.sqrts), opaque(s), cd.m_tot), xk[c], s, r[c], g_msq[c],
+                          gfk[c], gxk, ga[0], ga[1], ga[2], gM, gW);
+        S.T(i, 3 * c) = ga[0]; S.T(i, 3 * c + 1) = ga[1]; S.T(i, 3 * c + 2) = ga[2];
+      } else if constexpr (PEV) {
+        msq_eval_adj(child(nc, i, c), xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      } else {
+        msq_eval_adj(nc.k[c], xk[c], s, r[c], g_msq[c], gfk[c], gxk, gM, gW);
+      }
       gx[Wk::xi(nc, i, c)] = gxk;
       gpar[2 * Wk::child(nc, i, c)] += gM;
       gpar[2 * Wk::child(nc, i, c) + 1] += gW;
@@ -728,6 +782,30 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
   if constexpr (SID >= 0) {
 #pragma unroll
     for (int i = Wk::n_nodes(cd) - 1; i >= 0; i--) rev_branch(i);
+    if constexpr (PEV) {
+      // d a[k] / d (M, Gamma) of every internal daughter's map at this event's sqrt(s), one
+      // daughter after the other (each waits for the previous one's result)
+      double chain = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("" ::: "memory");   // the adjoints are read back from the tape, not kept in registers
+#endif
+#pragma unroll
+      for (int i = 0; i < Wk::n_nodes(cd); i++) {
+        const NodeCmd& nc = prog[i];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          if (Wk::leaf(nc, i, c)) continue;
+          const int j = Wk::child(nc, i, c);
+          const double ga0 = S.T(i, 3 * c), ga1 = S.T(i, 3 * c + 1), ga2 = S.T(i, 3 * c + 2);
+          mc::DualMW a[3], b[3];
+          mc::map_constants_d(prog[j].map_id, nc.k[c].m, nc.k[c].w, nc.k[c].m_min, cd.m_tot,
+                              opaque_after(opaque_after(eb.sqrts, ga0), chain), a, b);
+          gpar[2 * j] += ga0 * a[0].dm + ga1 * a[1].dm + ga2 * a[2].dm;
+          gpar[2 * j + 1] += ga0 * a[0].dw + ga1 * a[1].dw + ga2 * a[2].dw;
+          chain = gpar[2 * j + 1];
+        }
+      }
+    }
   } else {
     for (int i = N - 1; i >= 0; i--) rev_branch(i);
   }

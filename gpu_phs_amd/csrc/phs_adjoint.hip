@@ -82,7 +82,8 @@ __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const Ti
     for (int v = 0; v < np; v++) acc_lds[v] = 0;
   }
   const int end = min(td.count, 64 * (sub + 1));
-  for (int local = 64 * sub + lane; local < end; local += 64) {
+  // a unit has at most 64 events: one per lane, no loop (constants are not hoisted out of one)
+  if (const int local = 64 * sub + lane; local < end) {
     const int64_t e = td.e0 + local;
     const double* xg = sd.x + e * cd.n_x;
     const double* gp = sd.g_p ? sd.g_p + e * (int64_t)cd.n_out * 4 : nullptr;
@@ -127,8 +128,9 @@ __device__ __forceinline__ void adjoint_unit(int t, int sub, int n_sub, const Ti
 // every tile's channel has one of the shapes of up to four final-state particles, and each
 // workgroup takes the specialised code of its tile's shape (one launch for several shapes:
 // a launch lasts at least one wave's lifetime, so a launch per shape would add that much per
-// shape).  The register budget keeps two waves per SIMD (at most 256 VGPRs).  FR: plans with
-// friend branches (the generic walk, randoms read from memory, accumulators in LDS).
+// shape).  The register budget keeps two waves per SIMD (at most 256 VGPRs).  PEV: per-event
+// beam energies (loss mode 0).  FR: plans with friend branches (the generic walk, randoms read
+// from memory, accumulators in LDS).
 constexpr int kAdjMulti = 1000;
 template <int XN, bool COMPACT, int SID, int LM, bool PEV = false, bool FR = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void adjoint_kernel(
@@ -136,7 +138,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ad
     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, double* __restrict__ gpart,
     int part_stride, AdjLayout lay) {
   extern __shared__ double lds[];
-  for (int u = (int)blockIdx.x; u < n_units; u += (int)gridDim.x) {
+  // one workgroup per unit (the host launches n_units workgroups): no loop
+  if (const int u = (int)blockIdx.x; u < n_units) {
     const int tl = u / n_sub, sub = u - tl * n_sub, t = t_begin + tl;
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
@@ -144,14 +147,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ad
     const NodeCmd* prog = nodes + cd.node0;
     if constexpr (SID == kAdjMulti) {
       switch (cd.shape) {
-        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
-        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM, false, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 0: adjoint_unit<kShapes[0].n_x, COMPACT, 0, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 1: adjoint_unit<kShapes[1].n_x, COMPACT, 1, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 2: adjoint_unit<kShapes[2].n_x, COMPACT, 2, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 3: adjoint_unit<kShapes[3].n_x, COMPACT, 3, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 4: adjoint_unit<kShapes[4].n_x, COMPACT, 4, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 5: adjoint_unit<kShapes[5].n_x, COMPACT, 5, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 6: adjoint_unit<kShapes[6].n_x, COMPACT, 6, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
+        case 7: adjoint_unit<kShapes[7].n_x, COMPACT, 7, LM, PEV, false>(t, sub, n_sub, td, sd, cd, prog, gpart, part_stride, lay, lds); break;
         default: break;   // not reached: the host launches only tiles of shapes 0-7 here
       }
     } else {
@@ -272,6 +275,10 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
                                              : adjoint_kernel<kAdjXReg, false, kAdjMulti, 1>)
                               : (lay.compact ? adjoint_kernel<kAdjXReg, true, kAdjMulti, 0>
                                              : adjoint_kernel<kAdjXReg, false, kAdjMulti, 0>);
+  // per-event beams: the shapes of up to four final-state particles take the multi-shape
+  // kernel also for a run of one shape (one kernel, not one per shape)
+  AdjKernel multi_pev = lay.compact ? adjoint_kernel<kAdjXReg, true, kAdjMulti, 0, true>
+                                    : adjoint_kernel<kAdjXReg, false, kAdjMulti, 0, true>;
   auto small = [](int sid) { return sid >= 0 && sid < kAdjMultiShapes; };
   for (int r = 0; r < n_runs;) {
     // consecutive runs of the shapes of up to four final-state particles: one launch
@@ -280,7 +287,8 @@ hipError_t launch_adjoint(const DevicePlan& dp, const AdjLayout& lay, const Tile
       while (r1 < n_runs && small(runs[r1].sid) && runs[r1].begin == end) end = runs[r1++].end;
     const int units = (end - runs[r].begin) * n_sub;
     AdjKernel k = nullptr;
-    if (dp.per_event_beams || dp.has_friends) k = generic;
+    if (dp.has_friends) k = generic;
+    else if (dp.per_event_beams) k = small(runs[r].sid) ? multi_pev : generic;
     else if (r1 - r > 1) k = multi;
     else if (runs[r].sid >= 0)
       k = adj_pick_seq(runs[r].sid, lay.compact != 0, loss_mode, std::make_integer_sequence<int, kNumShapes>{});

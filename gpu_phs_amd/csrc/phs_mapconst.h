@@ -45,6 +45,14 @@ GPHS_MC DualMW pow2(DualMW a) { return d2chain(a, std::pow(a.v, 2), 2 * a.v); } 
 GPHS_MC DualMW dmax(DualMW a, double b) { return a.v < b ? DualMW(b) : a; }           // std::max(a, b)
 GPHS_MC double val(const DualMW& a) { return a.v; }
 
+// The cos-theta constants of a branch whose cos-theta map is peaked (the collinear, t- and
+// u-channel maps) at s = sb: b0 = M^2 / sb, b1 = log((b0 + 1) / b0), phs_cpu.cpp:515-520.
+GPHS_MC void ct_constants_d(double m_, double sb, DualMW b[2]) {
+  const DualMW m(m_, 1, 0);
+  b[0] = m * m / sb;
+  b[1] = log((b[0] + 1) / b[0]);
+}
+
 // phs_cpu.cpp:452-531 (init_mapping_constants_cpu) for one branch, same expressions and
 // evaluation order; the values are those of the reference, bit for bit, and dm / dw carry
 // their derivatives with respect to the map mass and width.  s_ct, when nonzero, is the s the
@@ -87,7 +95,7 @@ GPHS_MC void map_constants_d(int id, double m_, double w_, double m_min, double 
   }
   switch (id) {
     case MAP_TCHANNEL: case MAP_UCHANNEL: case MAP_COLLINEAR:
-      b[0] = m * m / sb; b[1] = log((b[0] + 1) / b[0]); b[2] = 0; break;
+      ct_constants_d(m_, sb, b); b[2] = 0; break;
     default:
       b[0] = b[1] = b[2] = DualMW(0); break;
   }

@@ -365,6 +365,43 @@ def test_shape_adjoint_identical_to_generic(n_out, loss_mode):
         assert np.all(np.abs(a[k] - b[k]) <= 1e-12 * scale), (k, np.abs(a[k] - b[k]).max())
 
 
+@pytest.mark.parametrize("n_out", [2, 3, 4])
+def test_shape_adjoint_identical_to_generic_event_beams(n_out):
+    """Per-event beam energies (loss mode 0): the multi-shape adjoint of the trees of 2-4
+    final-state particles gives bit for bit the generic adjoint's forward outputs, and its
+    gradients to 1e-12 (it takes the derivatives of the maps' constants at the event's sqrt(s)
+    after the sweep, a different order of the same sums)."""
+    import torch
+    proc = W.process_shapes(n_out)
+    res = []
+    for spec in (True, False):
+        rng, ch, x, ph, plan, bufs, counts = _setup(proc, 12001, 15 + n_out)
+        if not spec:
+            plan.set_specialize(False)
+        h2 = proc.sqrts / 2
+        for i, b in enumerate(bufs):
+            k = b.x.shape[0]
+            plan.set_event_beams(i, torch.from_numpy(h2 * rng.uniform(0.8, 1.3, k)).cuda(),
+                                 torch.from_numpy(h2 * rng.uniform(0.8, 1.3, k)).cuda())
+        g_x = [torch.zeros_like(b.x) for b in bufs]
+        gp = [torch.from_numpy(rng.standard_normal((b.x.shape[0], proc.n_out, 4))).cuda() for b in bufs]
+        gf = [torch.from_numpy(rng.standard_normal(b.x.shape[0])).cuda() for b in bufs]
+        print("event-beams adjoint, n_out", n_out, "specialised" if spec else "generic", flush=True)
+        gparam = plan.adjoint(g_x, g_p=gp, g_f=gf, loss_mode=0)
+        torch.cuda.synchronize()
+        cat = lambda ts: torch.cat(ts).cpu().numpy()
+        res.append(dict(gx=cat(g_x), gparam=gparam.cpu().numpy(), p=cat([b.p for b in bufs]),
+                        f=cat([b.factor for b in bufs]), v=cat([b.volume for b in bufs]),
+                        ok=cat([b.ok for b in bufs])))
+    a, b = res
+    assert a["ok"].sum() > 100
+    for k in ("p", "f", "v", "ok"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+    for k in ("gx", "gparam"):
+        scale = np.abs(b[k]).max(axis=-1, keepdims=True) if k == "gx" else np.abs(b[k]).max()
+        assert np.all(np.abs(a[k] - b[k]) <= 1e-12 * scale), (k, np.abs(a[k] - b[k]).max())
+
+
 def test_adjoint_channel_without_events_gets_zero_gradients():
     """A segment of no events makes a plan channel without tiles: its (M, Gamma) slots are set to
     0 (every other slot is the sum over its channel's units, as in a plan without that segment),
