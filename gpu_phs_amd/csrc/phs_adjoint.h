@@ -797,9 +797,11 @@ GPHS_HD bool event_adjoint(const ChannelDesc& cd, const EvBeams& eb, const NodeC
           if (Wk::leaf(nc, i, c)) continue;
           const int j = Wk::child(nc, i, c);
           const double ga0 = S.T(i, 3 * c), ga1 = S.T(i, 3 * c + 1), ga2 = S.T(i, 3 * c + 2);
+          // a[0] of the Breit-Wigner map is the plan's (the map itself uses the plan's a[0])
+          const mc::DualMW a0_bw(nc.k[c].a0, nc.k[c].da_dm[0], nc.k[c].da_dw[0]);
           mc::DualMW a[3], b[3];
           mc::map_constants_d(prog[j].map_id, nc.k[c].m, nc.k[c].w, nc.k[c].m_min, cd.m_tot,
-                              opaque_after(opaque_after(eb.sqrts, ga0), chain), a, b);
+                              opaque_after(opaque_after(eb.sqrts, ga0), chain), a, b, 0.0, &a0_bw);
           gpar[2 * j] += ga0 * a[0].dm + ga1 * a[1].dm + ga2 * a[2].dm;
           gpar[2 * j + 1] += ga0 * a[0].dw + ga1 * a[1].dw + ga2 * a[2].dw;
           chain = gpar[2 * j + 1];

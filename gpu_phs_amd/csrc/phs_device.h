@@ -116,7 +116,7 @@ __device__ __forceinline__ ChildCmd child_at(const ChildCmd& c, double sqrts, do
     case MK_LINEAR: r.a1 = d; r.a2 = r.a1 / s; break;
     case MK_BREIT_WIGNER: {
       const double msq0 = c.m * c.m;
-      r.a1 = atan((msq_max - msq0) / (c.m * c.w));
+      r.a1 = math::atan_radians((msq_max - msq0) / (c.m * c.w));
       r.a2 = (r.a1 - c.a0) * (c.m * c.w) / s;
       break;
     }

@@ -1,5 +1,5 @@
 // Bounded-range elementary functions for the kernels: sin and cos of 2 pi x, tan, exp, log,
-// and the azimuth atan2(y, x) in turns.
+// atan, and the azimuth atan2(y, x) in turns.
 //
 // The maps need sin and cos of 2*pi*x for a random x in [0, 1] (the azimuth, phs_cpu.cpp:301-307),
 // tan of an angle inside (-pi/2, pi/2) (the Breit-Wigner map, phs_cpu.cpp:25-28), and exp and
@@ -21,7 +21,7 @@
 //     e^r to r^13/13! (first dropped term 4.2e-18), then scaling by 2^k;
 //   * log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m - 1)/(m + 1),
 //     |s| <= 0.1716, series to s^21/21 (first dropped term 2.3e-17 relative), plus e ln 2.
-//   * atan2 in turns: see atan2_turns.
+//   * atan2 in turns: see atan2_turns; atan in radians: see atan_radians.
 // The tests check them against 300-bit arithmetic: within 2 units in the last place (sin,
 // cos, exp, log) and 3 (tan, atan2 in turns).  tan(-0) is +0.  The header has no HIP dependency, so the same
 // code is compiled for the host in those tests.
@@ -275,6 +275,65 @@ GPHS_MATH_FN double log_bounded(double x) {
   const double y = __builtin_fma(de, GPHS_PIN(kLn2Hi, lm), __builtin_fma(de, GPHS_PIN(kLn2Lo, lm), lm));
   if (!(x > 0)) return x == 0 ? -__builtin_inf() : __builtin_nan("");
   return x == __builtin_inf() ? x : y;
+}
+
+// atan(k / 4), k = 1 .. 4, as double-doubles (hi correctly rounded, lo the rest).
+constexpr double kAtanR1 = 0x1.f5b75f92c80ddp-3, kAtanR1Lo = 0x1.8ab6e3cf7afbdp-57;
+constexpr double kAtanR2 = 0x1.dac670561bb4fp-2, kAtanR2Lo = 0x1.a2b7f222f65e2p-56;
+constexpr double kAtanR3 = 0x1.4978fa3269ee1p-1, kAtanR3Lo = 0x1.2419a87f2a458p-56;
+constexpr double kAtanR4 = 0x1.921fb54442d18p-1, kAtanR4Lo = 0x1.1a62633145c07p-55;
+
+// atan(x) in radians, for any x (the Breit-Wigner map's constant atan((m_max^2 - M^2) / (M Gamma))
+// at each event's sqrt(s), phs_cpu.cpp:479-480), to about half an ulp, so that it agrees with a
+// correctly rounded atan for nearly every argument (the constant enters differences a1 - a0 that
+// can cancel).  With a = |x| for |x| <= 1 and 1 / |x| above (as a double-double),
+// c = k / 4 for k = rint(4 a) and t = (a - c) / (1 + a c), |t| <= 1/8: atan a = atan c + atan t
+// (the addition theorem), atan c from a table of double-doubles, t to double-double accuracy
+// (its quotient's residual by fused multiply-adds), atan t = t + t^3 (-1/3 + t^2/5 - ...) cut
+// after t^17/17 (first dropped term below 4e-18 of t); the sum with its rounding error carried
+// (Dekker's fast two-sum); for |x| > 1, pi/2 - atan a with pi/2 as a double-double; the sign
+// of x last.  atan(+-inf) = +-pi/2, atan(+-0) = +-0, NaN in, NaN out.
+GPHS_MATH_FN double atan_radians(double x) {
+  const double ax = __builtin_fabs(x);
+  const bool swap = ax > 1.0;
+  const double a = swap ? 1.0 / ax : ax;
+  // 1 / ax = a + a_lo (the residual of the division; 0 where 1 / ax is no longer normal)
+  const double a_lo = swap && ax < 0x1p1000 ? __builtin_fma(-a, ax, 1.0) / ax : 0.0;
+  const double k = __builtin_rint(4.0 * a);
+  const double c = 0.25 * k;
+  const double num = a - c;                                  // exact (Sterbenz)
+  // den = 1 + (a + a_lo) c = den_hi + den_lo; a c <= 1, so the fast two-sum holds
+  const double pc = a * c;
+  const double pce = __builtin_fma(a, c, -pc) + a_lo * c;
+  const double den_hi = 1.0 + pc;
+  const double den_lo = ((1.0 - den_hi) + pc) + pce;
+  const double t = num / den_hi;
+  const double t_lo = (__builtin_fma(-t, den_hi, num) + (a_lo - t * den_lo)) / den_hi;
+  const double z = t * t;
+  double p = GPHS_LIN(z, 1.0 / 17.0, -1.0 / 15.0);
+  p = GPHS_FMA(p, z, 1.0 / 13.0);
+  p = GPHS_FMA(p, z, -1.0 / 11.0);
+  p = GPHS_FMA(p, z, 1.0 / 9.0);
+  p = GPHS_FMA(p, z, -1.0 / 7.0);
+  p = GPHS_FMA(p, z, 1.0 / 5.0);
+  p = GPHS_FMA(p, z, -1.0 / 3.0);
+  double th = 0.0, tl = 0.0;   // atan(k / 4)
+  if (k >= 4.0) { th = GPHS_PIN(kAtanR4, k); tl = GPHS_PIN(kAtanR4Lo, k); }
+  else if (k >= 3.0) { th = GPHS_PIN(kAtanR3, k); tl = GPHS_PIN(kAtanR3Lo, k); }
+  else if (k >= 2.0) { th = GPHS_PIN(kAtanR2, k); tl = GPHS_PIN(kAtanR2Lo, k); }
+  else if (k >= 1.0) { th = GPHS_PIN(kAtanR1, k); tl = GPHS_PIN(kAtanR1Lo, k); }
+  // atan a = th + t + (tl + t_lo + t z p); |t| <= 1/8 < th unless th = 0
+  const double s = th + t;
+  const double lo = (((th - s) + t) + tl) + __builtin_fma(t * z, p, t_lo);
+  const double rh = s + lo;
+  double r = rh;
+  if (swap) {
+    const double rl = (s - rh) + lo;
+    const double ph = GPHS_PIN(kPio2Hi, rh);
+    const double d = ph - rh;                               // pi/2 > rh: the fast two-sum holds
+    r = d + (((ph - d) - rh) + (GPHS_PIN(kPio2Lo, rh) - rl));
+  }
+  return __builtin_copysign(r, x);
 }
 
 // atan2(y, x) / (2 pi), plus 1 where it is negative: the azimuth of (x, y) in turns, in [0, 1]

@@ -15,6 +15,9 @@ void host_exp(const double* x, long n, double* y) {
 void host_log(const double* x, long n, double* y) {
   for (long i = 0; i < n; i++) y[i] = gphs::math::log_bounded(x[i]);
 }
+void host_atan(const double* x, long n, double* y) {
+  for (long i = 0; i < n; i++) y[i] = gphs::math::atan_radians(x[i]);
+}
 void host_atan2_turns(const double* y, const double* x, long n, double* t) {
   for (long i = 0; i < n; i++) t[i] = gphs::math::atan2_turns(y[i], x[i]);
 }
