@@ -146,15 +146,21 @@ int gpu_phs_plan_adjoint(int plan, const gpu_phs_adjoint_segment* adj, int loss_
                                     "give the cotangents (loss_mode 0)");
   int rc = adjoint_prepare(*pl, adj, write_forward);
   if (rc) return rc;
-  // the tile table by class and, within a class, by tree shape (the forward launches' runs):
-  // runs of a shape of up to four final-state particles take the adjoint specialised for it.
-  // One workgroup per 64-event unit: a persistent grid measured slower (its workgroups do not
-  // all fit the LDS at once), and units without events would cost a workgroup each.
+  // the tile table in runs of one tree shape: a shape of up to four final-state particles
+  // takes the adjoint specialised for it, and consecutive runs of such shapes one launch of the
+  // multi-shape adjoint (so the shapes the forward pass folds into its generic launch, those
+  // with fewer than merge_below events, keep their specialised adjoint); the others the generic
+  // adjoint.  One workgroup per 64-event unit: a persistent grid measured slower (its
+  // workgroups do not all fit the LDS at once), and units without events would cost a
+  // workgroup each.
   std::vector<TileRun> runs;
   for (int c = 0; c < kTileClasses; c++) {
-    if (pl->class_end[c] <= pl->class_begin[c]) continue;
-    if (pl->shape_runs[c].empty()) runs.push_back(TileRun{-1, pl->class_begin[c], pl->class_end[c]});
-    else runs.insert(runs.end(), pl->shape_runs[c].begin(), pl->shape_runs[c].end());
+    for (int t = pl->class_begin[c]; t < pl->class_end[c]; t++) {
+      const int sh = pl->chans[pl->segs[pl->tiles[t].seg].channel].shape;
+      const int sid = pl->specialize && sh >= 0 && sh < kAdjShapes ? sh : -1;
+      if (runs.empty() || runs.back().sid != sid || runs.back().end != t) runs.push_back(TileRun{sid, t, t + 1});
+      else runs.back().end = t + 1;
+    }
   }
   hipError_t e = launch_adjoint(pl->adj_dp, pl->adj_lay, runs.data(), (int)runs.size(), loss_mode, pl->adj_part,
                                 pl->adj_part_stride, pl->adj_sub, pl->adj_ranges, pl->adj_done, pl->adj_chan_tiles,

@@ -56,7 +56,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 constexpr int kAdjXReg = 8;             // n_x <= 8 <=> n_out <= 4
-constexpr int kAdjMultiShapes = 8;      // shapes 0-7 of the table: the trees of 2-4 final-state particles
+constexpr int kAdjMultiShapes = kAdjShapes;   // shapes 0-7 of the table: the trees of 2-4 final-state particles
 constexpr int kSmallParams = 2 * 3;     // (M, Gamma) of the 3 branches of an n_out = 4 tree
 
 // One 64-event unit of the adjoint: sub-tile `sub` of tile t, events [64 sub, 64 sub + 64).

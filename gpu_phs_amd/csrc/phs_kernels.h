@@ -43,6 +43,9 @@ hipError_t launch_forward(const DevicePlan& dp, int t_begin, int t_end, const La
 // beams (dp.per_event_beams) only shapes [0, kPevShapes) -- the trees of up to four final-state
 // particles -- have such kernels.
 constexpr int kPevShapes = 8;
+// The adjoint's specialised shapes: [0, kAdjShapes), the trees of up to four final-state
+// particles (the others run the generic adjoint).
+constexpr int kAdjShapes = 8;
 hipError_t launch_forward_shape(const DevicePlan& dp, int sid, int t_begin, int t_end, const LaunchConfig& lc,
                                 hipStream_t stream);
 
