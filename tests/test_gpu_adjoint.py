@@ -402,6 +402,27 @@ def test_shape_adjoint_identical_to_generic_event_beams(n_out):
         assert np.all(np.abs(a[k] - b[k]) <= 1e-12 * scale), (k, np.abs(a[k] - b[k]).max())
 
 
+def test_adjoint_kernels_do_not_depend_on_merge_below():
+    """The adjoint forms its own runs by tree shape: with the forward pass's default merge_below
+    (which folds these small segments into its generic launch) the adjoint still runs the
+    shape-specialised code, and gives the same bits as with merge_below 0."""
+    import torch
+    proc = W.process_shapes(4)
+    res = []
+    for mb in (None, 0):
+        rng, ch, x, ph, plan, bufs, counts = _setup(proc, 6001, 31)
+        if mb is None:
+            plan.set_merge_below(1 << 18)
+        g_x = [torch.zeros_like(b.x) for b in bufs]
+        loss = [torch.zeros(b.x.shape[0], dtype=torch.float64, device="cuda") for b in bufs]
+        print("adjoint, merge_below", mb, flush=True)
+        gparam = plan.adjoint(g_x, loss=loss, loss_mode=1)
+        torch.cuda.synchronize()
+        res.append((torch.cat(g_x).cpu().numpy(), torch.cat(loss).cpu().numpy(), gparam.cpu().numpy()))
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+
 def test_adjoint_channel_without_events_gets_zero_gradients():
     """A segment of no events makes a plan channel without tiles: its (M, Gamma) slots are set to
     0 (every other slot is the sum over its channel's units, as in a plan without that segment),
