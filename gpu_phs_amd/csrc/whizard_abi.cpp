@@ -5,9 +5,11 @@
 // module object, so the calls keep their order-dependent semantics.
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -128,24 +130,79 @@ int ensure_staging(WhizardModule& w) {
   return GPU_PHS_OK;
 }
 
-// A host copy of one staging chunk, split over kCopyThreads threads (one thread copies some
-// 10-15 GB/s between pinned and pageable memory, well below what PCIe moves, so a single
-// thread would set the entry point's rate).
-constexpr int kCopyThreads = 4;
+// Host copies of the staging chunks, each split over kCopyThreads threads: the calling thread
+// and kCopyThreads - 1 workers started at the first large copy and kept until the library is
+// unloaded.  One thread copies some 10-15 GB/s between pinned and pageable memory, well below
+// what PCIe moves, so a single thread would set the entry point's rate; starting threads for
+// every 16 MiB chunk would cost tens of microseconds per chunk.
+constexpr int kCopyThreads = 8;
+class CopyPool {
+ public:
+  void copy(void* dst, const void* src, size_t n) {
+    if (n < ((size_t)1 << 20)) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    const size_t part = ((n + kCopyThreads - 1) / kCopyThreads + 63) & ~(size_t)63;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      if (th_.empty())
+        for (int k = 1; k < kCopyThreads; k++) th_.emplace_back([this, k] { worker(k); });
+      dst_ = (char*)dst;
+      src_ = (const char*)src;
+      n_ = n;
+      part_ = part;
+      pending_ = kCopyThreads - 1;
+      gen_++;
+    }
+    work_.notify_all();
+    std::memcpy(dst, src, std::min(part, n));
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [&] { return pending_ == 0; });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    work_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+
+ private:
+  void worker(int k) {
+    uint64_t seen = 0;
+    for (;;) {
+      char* dst;
+      const char* src;
+      size_t n, part;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        work_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        dst = dst_; src = src_; n = n_; part = part_;
+      }
+      const size_t off = part * k;
+      if (off < n) std::memcpy(dst + off, src + off, std::min(part, n - off));
+      std::lock_guard<std::mutex> lk(m_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable work_, done_;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t n_ = 0, part_ = 0;
+  int pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
 void chunk_copy(void* dst, const void* src, size_t n) {
-  if (n < ((size_t)1 << 20)) {
-    std::memcpy(dst, src, n);
-    return;
-  }
-  const size_t part = ((n + kCopyThreads - 1) / kCopyThreads + 63) & ~(size_t)63;
-  std::vector<std::thread> th;
-  for (int k = 1; k < kCopyThreads; k++) {
-    const size_t off = part * k;
-    if (off >= n) break;
-    th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, n - off)); });
-  }
-  std::memcpy(dst, src, std::min(part, n));
-  for (auto& t : th) t.join();
+  static CopyPool pool;
+  pool.copy(dst, src, n);
 }
 
 // Host -> device through the pinned slots: each chunk is copied into a free slot on the host and
