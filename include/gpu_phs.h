@@ -86,7 +86,9 @@ void c_whizard_finalize_gpu_phs(void);
  * the others: the inverse mapping of its momenta, phs.cu:1491-1554), volumes [n_events], oks
  * [n_events], and x_out [n_channels][n_events][n_x] (own channel: the input randoms; the
  * others: the inverse mapping; may be NULL).  requires_boost_to_lab is read but, as in the
- * reference (phs.cu:1559), the lab boost is applied whenever E1 != E2. */
+ * reference (phs.cu:1559), the lab boost is applied whenever E1 != E2.  The host copies go
+ * through pinned buffers, each chunk split over the calling thread and 7 worker threads that
+ * the library starts at the first large call and keeps until it is unloaded. */
 void c_whizard_gen_phs_from_x_gpu(double* E1_in, double* E2_in, bool* requires_boost_to_lab, int* n_events,
                                   int* n_channels, int* n_x, double* x, double* factors, double* volumes,
                                   bool* oks, double* p, double* x_out);
