@@ -322,7 +322,12 @@ __device__ __forceinline__ void store_momenta(const double* __restrict__ ps, int
 //
 // BLK > 0: the workgroup size is that compile-time constant (shape kernels), so the LDS offsets
 // of the momentum table and of the store phase are immediates.
-template <int BANK, int WPE, int XN, bool FR, bool PEV, int SID, int BLK>
+//
+// ONE: the launch has one workgroup per tile and tiles of at most blockDim.x events, so each
+// loop below makes one pass; both end after it, so that they are no loops to the compiler,
+// which would otherwise keep constants hoisted out of them in vector registers (the kernels of
+// 2-4 final-state particles: 118 instead of 159 VGPRs, four waves per SIMD instead of three).
+template <int BANK, int WPE, int XN, bool FR, bool PEV, int SID, int BLK, bool ONE = false>
 __global__ __launch_bounds__(BLK > 0 ? BLK : 256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, const SegDesc* __restrict__ segs,
                     const ChannelDesc* __restrict__ chans, const NodeCmd* __restrict__ nodes, int nx_max,
@@ -372,7 +377,9 @@ void forward_kernel(const TileDesc* __restrict__ tiles, int t_begin, int t_end, 
       }
       __syncthreads();
       store_momenta<kNout, BLK>(ps, S, cnt, cd, sd.align16 & 2, sd.p + e0 * (int64_t)cd.n_out * 4);
+      if constexpr (ONE) break;
     }
+    if constexpr (ONE) break;
   }
 }
 
@@ -381,16 +388,18 @@ using FwdKernel = void (*)(const TileDesc*, int, int, const SegDesc*, const Chan
 // Kernels specialised for one tree shape: randoms in the shape's n_x registers, the root's
 // boost parameters as the save bank when only the root defers a daughter, one level of
 // registers when another branch does, fixed beams, no friends.
-template <int SID, int WPE, int BLK>
+template <int SID, int WPE, int BLK, bool ONE = false>
 FwdKernel shape_kernel() {
   constexpr const ShapeProg& sh = kShapes[SID];
   constexpr int bank = sh.banks == 0 ? BK_NONE : (sh.root_only ? BK_ROOT : BK_VGPR);
-  return forward_kernel<bank, WPE, sh.n_x, false, false, SID, BLK>;
+  return forward_kernel<bank, WPE, sh.n_x, false, false, SID, BLK, ONE>;
 }
 
 // The shape kernels of shapes [lo, hi) (one translation unit each): the variant for register
 // budget `wpe` and workgroup size `block`, or null for a shape outside the range.
-FwdKernel pick_shape_0_8(int sid, int wpe, int block);
+// one: the launch has one workgroup per tile and tiles of at most `block` events (the
+// one-pass variants of the trees of up to four final-state particles)
+FwdKernel pick_shape_0_8(int sid, int wpe, int block, bool one);
 // per-event beam energies: shapes [0, kPevShapes), the trees of up to four final-state particles
 FwdKernel pick_shape_pev_0_8(int sid, int wpe, int block);
 FwdKernel pick_shape_8_22(int sid, int wpe, int block);

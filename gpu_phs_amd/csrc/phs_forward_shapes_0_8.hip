@@ -1,8 +1,10 @@
 // Shape-specialised forward kernels of shapes [0, 8) of the shape table (phs_shapes.h);
 // one translation unit per range so that they compile in parallel.  Trees of up to four
 // final-state particles have variants for workgroups of 64, 128 and 256 threads and every
-// register budget; those of five and six, for the 64-thread workgroups their launches take,
-// at the compiler's budget; other workgroup sizes run the variant with a run-time size.
+// register budget, and one-pass variants (one workgroup per tile, tiles of one sub-tile) at
+// the compiler's budget, which is then within four waves per SIMD; those of five and six, for
+// the 64-thread workgroups their launches take, at the compiler's budget; other workgroup
+// sizes run the variant with a run-time size.
 #include "phs_forward.h"
 
 namespace gphs {
@@ -19,8 +21,13 @@ FwdKernel pick_wpe(int wpe) {
 }
 
 template <int SID>
-FwdKernel pick_one(int wpe, int block) {
+FwdKernel pick_one(int wpe, int block, bool one) {
   if constexpr (kShapes[SID].n_out <= 4) {
+    if (one && wpe != 5) {   // the one-pass kernels take 90-118 VGPRs: within budgets 3 and 4
+      if (block == 64) return shape_kernel<SID, 0, 64, true>();
+      if (block == 128) return shape_kernel<SID, 0, 128, true>();
+      if (block == 256) return shape_kernel<SID, 0, 256, true>();
+    }
     if (block == 64) return pick_wpe<SID, 64>(wpe);
     if (block == 128) return pick_wpe<SID, 128>(wpe);
     if (block == 256) return pick_wpe<SID, 256>(wpe);
@@ -32,17 +39,17 @@ FwdKernel pick_one(int wpe, int block) {
 }
 
 template <int... S>
-FwdKernel pick_seq(int sid, int wpe, int block, std::integer_sequence<int, S...>) {
+FwdKernel pick_seq(int sid, int wpe, int block, bool one, std::integer_sequence<int, S...>) {
   FwdKernel k = nullptr;
-  (void)((sid == 0 + S ? (k = pick_one<0 + S>(wpe, block), true) : false) || ...);
+  (void)((sid == 0 + S ? (k = pick_one<0 + S>(wpe, block, one), true) : false) || ...);
   return k;
 }
 
 }  // namespace
 
-FwdKernel pick_shape_0_8(int sid, int wpe, int block) {
+FwdKernel pick_shape_0_8(int sid, int wpe, int block, bool one) {
   static_assert(8 <= kNumShapes, "shape range");
-  return pick_seq(sid, wpe, block, std::make_integer_sequence<int, 8>{});
+  return pick_seq(sid, wpe, block, one, std::make_integer_sequence<int, 8>{});
 }
 
 }  // namespace fwd
