@@ -352,11 +352,11 @@ def cpu_baseline(wl_name, W, threads, scale, budget_s):
 # The launch tuning of the c3 mix that the headline line runs and the committed profiles were
 # taken with, confirmed by interleaved repeats of the shapes around it (class 0: 128-thread
 # workgroups, one event per thread, so one workgroup per tile of one sub-tile and the one-pass
-# kernels at the compiler's register budget, four waves per SIMD; class 1: 64 threads, 2 events
-# per thread; the classes share the GPU).  --autotune tunes
+# kernels at the compiler's register budget, four waves per SIMD; class 1: 64 threads, one event
+# per thread, its one-pass kernels at three waves per SIMD; the classes share the GPU).  --autotune tunes
 # instead, and the full run reports the autotuner's own result beside it (extra.c3_autotuned).
 C3_TUNING = {"classes": [{"block": 128, "ept": 1, "grid": 0, "waves_per_eu": 0, "lds": 0},
-                         {"block": 64, "ept": 2, "grid": 0, "waves_per_eu": 0, "lds": 0}],
+                         {"block": 64, "ept": 1, "grid": 0, "waves_per_eu": 0, "lds": 0}],
              "order": 0, "serial": 0, "merge_below": 262144, "streams": 1}
 
 

@@ -401,11 +401,11 @@ FwdKernel shape_kernel() {
 // one-pass variants of the trees of up to four final-state particles)
 FwdKernel pick_shape_0_8(int sid, int wpe, int block, bool one);
 // per-event beam energies: shapes [0, kPevShapes), the trees of up to four final-state particles
-FwdKernel pick_shape_pev_0_8(int sid, int wpe, int block);
-FwdKernel pick_shape_8_22(int sid, int wpe, int block);
-FwdKernel pick_shape_22_36(int sid, int wpe, int block);
-FwdKernel pick_shape_36_50(int sid, int wpe, int block);
-FwdKernel pick_shape_50_64(int sid, int wpe, int block);
+FwdKernel pick_shape_pev_0_8(int sid, int wpe, int block, bool one);
+FwdKernel pick_shape_8_22(int sid, int wpe, int block, bool one);
+FwdKernel pick_shape_22_36(int sid, int wpe, int block, bool one);
+FwdKernel pick_shape_36_50(int sid, int wpe, int block, bool one);
+FwdKernel pick_shape_50_64(int sid, int wpe, int block, bool one);
 
 }  // namespace fwd
 

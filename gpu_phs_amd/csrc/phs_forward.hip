@@ -84,13 +84,13 @@ hipError_t launch_forward_shape(const DevicePlan& dp, int sid, int t_begin, int 
   const int n = t_end - t_begin;
   const int grid = lc.grid > 0 ? (lc.grid < n ? lc.grid : n) : n;
   FwdKernel k = nullptr;
+  // one workgroup per tile (grid 0) and tiles of one sub-tile (ept 1): the one-pass kernels
+  const bool one = lc.grid == 0 && lc.ept == 1;
   if (dp.per_event_beams) {
-    if (sid < kPevShapes) k = pick_shape_pev_0_8(sid, lc.waves_per_eu, lc.block);
+    if (sid < kPevShapes) k = pick_shape_pev_0_8(sid, lc.waves_per_eu, lc.block, one);
   } else {
-    // one workgroup per tile (grid 0) and tiles of one sub-tile (ept 1): the one-pass kernels
-    k = pick_shape_0_8(sid, lc.waves_per_eu, lc.block, lc.grid == 0 && lc.ept == 1);
-    for (auto pick : {pick_shape_8_22, pick_shape_22_36, pick_shape_36_50, pick_shape_50_64})
-      if (!k) k = pick(sid, lc.waves_per_eu, lc.block);
+    for (auto pick : {pick_shape_0_8, pick_shape_8_22, pick_shape_22_36, pick_shape_36_50, pick_shape_50_64})
+      if (!k) k = pick(sid, lc.waves_per_eu, lc.block, one);
   }
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(grid), dim3(lc.block), shmem, stream, dp.tiles, t_begin, t_end, dp.segs, dp.chans,

@@ -2,7 +2,9 @@
 // one translation unit per range so that they compile in parallel.  Trees of up to four
 // final-state particles have variants for workgroups of 64, 128 and 256 threads and every
 // register budget; those of five and six, for the 64-thread workgroups their launches take,
-// at the compiler's budget; other workgroup sizes run the variant with a run-time size.
+// at the compiler's budget, looping or one-pass (one workgroup per tile of one sub-tile: 150-160
+// instead of 199-205 VGPRs, three waves per SIMD instead of two); other workgroup sizes run
+// the variant with a run-time size.
 #include "phs_forward.h"
 
 namespace gphs {
@@ -19,30 +21,30 @@ FwdKernel pick_wpe(int wpe) {
 }
 
 template <int SID>
-FwdKernel pick_one(int wpe, int block) {
+FwdKernel pick_one(int wpe, int block, bool one) {
   if constexpr (kShapes[SID].n_out <= 4) {
     if (block == 64) return pick_wpe<SID, 64>(wpe);
     if (block == 128) return pick_wpe<SID, 128>(wpe);
     if (block == 256) return pick_wpe<SID, 256>(wpe);
     return pick_wpe<SID, 0>(wpe);
   } else {
-    if (block == 64) return shape_kernel<SID, 0, 64>();
+    if (block == 64) return one ? shape_kernel<SID, 0, 64, true>() : shape_kernel<SID, 0, 64>();
     return shape_kernel<SID, 0, 0>();
   }
 }
 
 template <int... S>
-FwdKernel pick_seq(int sid, int wpe, int block, std::integer_sequence<int, S...>) {
+FwdKernel pick_seq(int sid, int wpe, int block, bool one, std::integer_sequence<int, S...>) {
   FwdKernel k = nullptr;
-  (void)((sid == 22 + S ? (k = pick_one<22 + S>(wpe, block), true) : false) || ...);
+  (void)((sid == 22 + S ? (k = pick_one<22 + S>(wpe, block, one), true) : false) || ...);
   return k;
 }
 
 }  // namespace
 
-FwdKernel pick_shape_22_36(int sid, int wpe, int block) {
+FwdKernel pick_shape_22_36(int sid, int wpe, int block, bool one) {
   static_assert(36 <= kNumShapes, "shape range");
-  return pick_seq(sid, wpe, block, std::make_integer_sequence<int, 14>{});
+  return pick_seq(sid, wpe, block, one, std::make_integer_sequence<int, 14>{});
 }
 
 }  // namespace fwd

@@ -111,7 +111,8 @@ def test_per_event_beams_built_in_loss_is_rejected():
 
 
 @pytest.mark.parametrize("n_out", [2, 3, 4])
-@pytest.mark.parametrize("cfg", [dict(), dict(block=64, ept=3, waves_per_eu=3), dict(block=128, ept=2, waves_per_eu=5)])
+@pytest.mark.parametrize("cfg", [dict(), dict(block=64, ept=3, waves_per_eu=3), dict(block=128, ept=2, waves_per_eu=5),
+                                 dict(block=128, ept=1)])
 def test_per_event_beams_shape_kernels_identical_to_generic(n_out, cfg):
     """With per-event beams, the kernels specialised for the trees of 2-4 final-state particles
     give the generic kernel's bits: momenta, factors, volumes and ok flags of every event."""

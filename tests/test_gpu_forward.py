@@ -114,9 +114,10 @@ def test_shape_kernels_match_generic_and_oracle(n_out):
     ch = rng.integers(0, proc.n_channels, n).astype(np.int32)
     x = rng.random((n, proc.n_x))
     print("shapes", n_out, flush=True)
-    # block 128, ept 1: one workgroup per tile of one sub-tile, the one-pass kernels (n_out <= 4)
+    # ept 1 at grid 0: one workgroup per tile of one sub-tile, the one-pass kernels (block 64, 128
+    # and 256 for n_out <= 4; block 64 for 5 and 6)
     for cfg in (dict(), dict(block=64, ept=3, waves_per_eu=4), dict(block=128, ept=2, grid=11),
-                dict(block=128, ept=1), dict(block=256, ept=1, waves_per_eu=3)):
+                dict(block=128, ept=1), dict(block=256, ept=1, waves_per_eu=3), dict(block=64, ept=1)):
         spec, _, _ = run_gpu(proc, x, ch, msq=True, **cfg)
         gen, _, _ = run_gpu(proc, x, ch, msq=True, specialize=False, **cfg)
         for k in ("p", "factor", "volume", "ok", "msq"):
