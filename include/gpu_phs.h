@@ -149,7 +149,10 @@ int gpu_phs_plan_destroy(int plan);
  * at least that many waves per SIMD (0 = compiler's choice, 3, 4, or 5: shape-specialised
  * kernels only, others take 5 as 0), lds = bit flags: 1 keeps
  * the boosts of deferred subtrees in LDS instead of VGPRs (trees that defer two levels always
- * use LDS), 2 keeps each event's randoms in LDS instead of VGPRs (always for n_x > 16).  Changing block, ept or order
+ * use LDS), 2 keeps each event's randoms in LDS instead of VGPRs (always for n_x > 16).  ept 1
+ * with grid 0 (one workgroup per tile of one sub-tile) runs the shape-specialised kernels'
+ * one-pass variants at the compiler's register budget (fewer registers: four waves per SIMD for
+ * trees of up to four final-state particles, three for five and six).  Changing block, ept or order
  * rebuilds the tile table.  A block whose LDS tables (4 n_out rows of block + 1 doubles, n_x
  * more with randoms in LDS, plus the LDS save banks) exceed the device's per-workgroup LDS is
  * rejected. */
